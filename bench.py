@@ -1,0 +1,142 @@
+"""Benchmark of the PT-ICM hot path on one MI355X.
+
+Default workload (the GPU configuration of the metric): plain PT-ICM on G022
+(Erdos-Renyi n=2000, M=19,990, unit weights) with the 64-slot ladder from the
+FP64 multihistogram preprocess (GPU pilot: 5000 burn-in + 2000 logged steps),
+2 chains, ICM by the reference's slot rule.  A "step" is one PT step: one
+Metropolis sweep of all 128 replicas, 2(Nt-1) exchange attempts, ICM.
+
+Prints ONE JSON line (rank 0).  value = attempted spin updates per second over
+the timed steps (inputs resident in HBM); roofline = the sweep kernel's
+algorithmic LDS bytes (4 d_i + 6 per attempted update) per launch over its
+average launch time, measured with HIP events on the engine stream;
+cpu_baseline = the oracle in reference order (index-order sweep, maintained
+dE, draws only for uphill moves) on one host core over a bounded sample.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+GRAPHS = os.path.join(REPO, "tests", "golden", "graphs")
+LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
+
+
+def load_ptx():
+    spec = importlib.util.spec_from_file_location("ptx", os.path.join(REPO, "pt-icm_amd", "ptx.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["ptx"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def preprocess_ladder(ptx, problem, nt, T0, T1, seed, burn_in=5000, samples=2000):
+    """--preprocess (src/initialization.cc:109-111 -> get_beta): GPU pilot run
+    of one chain with per-slot energy histograms, then the FP64 DOS and
+    ladder search on the host."""
+    hb, bs, be = ptx.pilot_betas(T0, T1, nt)
+    pilot = ptx.Engine(problem, hb, seed=seed, chains=1, flags=ptx.FLAG_PILOT | ptx.FLAG_HISTOGRAM,
+                       hist_start=burn_in)
+    pilot.step(burn_in + samples)
+    counts, e_base = pilot.histograms()
+    del pilot
+    return ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_FAST)
+
+
+def cpu_baseline(problem_path, betas, seconds, seed=0):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as O
+    g = O.Graph.read(problem_path)
+    eng = O.Engine(g, betas, chains=2, seed=seed, flags=O.FLAG_ICM | O.FLAG_REFERENCE_ORDER)
+    t0 = time.perf_counter()
+    eng.step(2)
+    dt = time.perf_counter() - t0
+    steps = max(2, int(seconds / max(dt / 2, 1e-6)))
+    t0 = time.perf_counter()
+    eng.step(steps)
+    dt = time.perf_counter() - t0
+    updates = 2 * len(betas) * g.n * steps
+    return {"value": updates / dt, "unit": "spin updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle reference-order PT-ICM on G022, same ladder ({len(betas)} slots x 2 chains), "
+                      f"{steps} PT steps in {dt:.1f} s on 1 host core"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
+                                   "(instances or temperature slots per GPU) and is not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+    ptx = load_ptx()
+    path = os.path.join(GRAPHS, "G022")
+    p = ptx.Problem.read(path)
+    t0 = time.perf_counter()
+    betas = preprocess_ladder(ptx, p, 64, 0.6, 2.8, args.seed)
+    t_pre = time.perf_counter() - t0
+    nt = len(betas)
+    log(f"preprocess: {nt} temperatures in {t_pre:.2f} s; beta {betas[0]:.4f} .. {betas[-1]:.4f}")
+    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM)
+    nc = eng.instance_info().nc
+    eng.step(args.warmup)
+    eng.sync()
+    eng.timing(True)
+    eng.timing()  # reset accumulated counts
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    tm = eng.timing()
+    info = eng.info()
+    updates = info.spin_updates_per_step * args.steps
+    value = updates / dt
+    sweep_avg_ms = tm.sweep_ms / max(tm.sweep_launches, 1)
+    bytes_per_launch = info.lds_bytes_per_step
+    achieved = bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
+    be, bs, _ = eng.best(0, with_state=False)
+    best = int(be.min())
+    out = {
+        "metric": "spin updates/s",
+        "value": value,
+        "unit": "spin updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "G022 from the reference's example_graphs (ER n=2000, M=19990, w=+1); random initial states",
+        "config": {"workload": "plain PT-ICM single instance: G022, multihistogram ladder (64 requested, "
+                               f"{nt} chosen), 2 chains, ICM slots {nc}, 1 sweep/step",
+                   "n": p.n, "replicas": 2 * nt, "icm_slots": nc, "preprocess_s": round(t_pre, 3),
+                   "best_energy": best, "best_cut": (19990 - best) // 2},
+        "kernel_ms": {"sweep": sweep_avg_ms, "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
+                      "icm": tm.icm_ms / max(tm.icm_launches, 1)},
+        "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sweep", "bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,211 @@
+/*
+ * ptx.h -- C ABI of the MI355X parallel-tempering + ICM engine (libptx.so).
+ *
+ * Plain pointers and sizes only; opaque handles; every entry point returns a
+ * ptx_status (0 = success, negative = error, never exits the process); the
+ * message of the last error on the calling thread is available through
+ * ptx_last_error.  Buffers are caller-owned host memory.  An engine owns one
+ * HIP stream and is not thread-safe; distinct engines are independent.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * mxburns2022/PT-ICM).
+ */
+#ifndef PTX_H
+#define PTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ptx_status;
+#define PTX_OK 0
+#define PTX_ERR_INVALID (-1)     /* bad argument or malformed input */
+#define PTX_ERR_IO (-2)          /* file could not be read */
+#define PTX_ERR_UNSUPPORTED (-3) /* input outside the engine's limits */
+#define PTX_ERR_DEVICE (-4)      /* HIP runtime error (no device, launch failure) */
+#define PTX_ERR_NOMEM (-5)       /* host or device allocation failed */
+
+typedef struct ptx_problem ptx_problem;
+typedef struct ptx_engine ptx_engine;
+
+/* Copies the last error message of this thread into buf (NUL-terminated). */
+ptx_status ptx_last_error(char* buf, size_t len);
+/* Library version string and the device it would run on ("" if none). */
+const char* ptx_version(void);
+int32_t ptx_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* Problem (replaces pticm::Problem, include/problem.hh:7-98)          */
+/* ------------------------------------------------------------------ */
+
+/* Problem(std::string path, seed, burn_in), src/problem.cc:183-247: gset or
+ * .ising text; 0-based iff the path contains ".gset" or ".ising", else
+ * 1-based; missing weight = 1; sparse iff M / (N(N-1)/2) < 0.1.  Weights and
+ * linear terms must be integers (PTX_ERR_UNSUPPORTED otherwise). */
+ptx_status ptx_problem_read_gset(const char* path, ptx_problem** out);
+
+/* Problem(double* weights, int* src, int* dest, double* linear, seed, burn_in,
+ * N, M), include/problem.hh:40-62 (dense == 0), and the dense-matrix rules of
+ * src/problem.cc:210-215 (dense != 0: last weight of a pair wins, zero = no
+ * edge).  h may be NULL (all zero). */
+ptx_status ptx_problem_create_coo(int32_t n, int64_t m, const int32_t* src, const int32_t* dst,
+                                  const double* w, const double* h, int32_t dense, ptx_problem** out);
+void ptx_problem_destroy(ptx_problem* p);
+
+typedef struct {
+    int32_t n;            /* vcount(), problem.hh:67 */
+    int64_t m;            /* ecount(), problem.hh:68 (header M) */
+    int64_t nnz;          /* directed adjacency entries */
+    int32_t sparse;       /* is_sparse(), problem.hh:74 */
+    double char_scale;    /* get_char_scale(), problem.hh:66 */
+    int32_t n_colours;    /* greedy colour classes (index order) */
+    int32_t fmax;         /* max_i |h_i| + sum_j |w_ij| */
+    int64_t e_min_bound;  /* -(sum |w| over edges + sum |h|) */
+    int64_t e_max_bound;
+    int32_t has_field;    /* any h_i != 0 */
+} ptx_problem_info;
+ptx_status ptx_problem_info_get(const ptx_problem* p, ptx_problem_info* out);
+
+/* Problem::energy, src/problem.cc:121-142.  state: n bytes, 0/1. */
+ptx_status ptx_problem_energy(const ptx_problem* p, const uint8_t* state, int64_t* energy);
+/* Problem::maxcut, src/problem.cc:145-162. */
+ptx_status ptx_problem_maxcut(const ptx_problem* p, const uint8_t* state, int64_t* cut);
+/* Problem::calculate_dE, src/problem.cc:33-63: dE[n] and the energy. */
+ptx_status ptx_problem_delta_e(const ptx_problem* p, const uint8_t* state, int64_t* dE, int64_t* energy);
+/* The colour-class order the engine sweeps in: pos2orig[n], class_start[n_colours+1]. */
+ptx_status ptx_problem_colouring(const ptx_problem* p, int32_t* pos2orig, int32_t* class_start);
+/* Adjacency in index order: rowptr[n+1], col[nnz], w[nnz], h[n]. */
+ptx_status ptx_problem_csr(const ptx_problem* p, int32_t* rowptr, int32_t* col, int32_t* w, int32_t* h);
+
+/* ------------------------------------------------------------------ */
+/* Ladders (host, FP64)                                                */
+/* ------------------------------------------------------------------ */
+
+/* beta_linspace, src/mpi_utils.cc:6-22 (linear in beta, ascending). */
+ptx_status ptx_beta_linspace(double* out, double T0, double T1, int64_t n, int32_t inverse);
+/* Nc rule of initialize_temperatures, src/initialization.cc:158-164. */
+int32_t ptx_icm_count(const double* beta, int32_t nt, double char_scale);
+/* hist_betas of get_beta, src/cae_estimator.cc:309-328: nt values from
+ * max(b0,b1) down to min(b0,b1) by accumulated steps; returns both ends. */
+ptx_status ptx_pilot_betas(double T0, double T1, int32_t nt, int32_t inverse, double* out,
+                           double* beta_start, double* beta_end);
+
+#define PTX_LADDER_EXACT 0 /* the reference's O(nbins^2) acceptance sum */
+#define PTX_LADDER_FAST 1  /* O(nbins) prefix-sum form of the same sum */
+/* get_beta after the pilot, src/cae_estimator.cc:330-428: multihistogram DOS
+ * (Ferrenberg-Swendsen in log space) from per-slot energy histograms, then the
+ * constant-acceptance ladder search toward rate r.  counts is [nt][nbins] in
+ * the engine's histogram layout (energy = e_base + 2*bin); estep as the
+ * reference (2).  betas_out holds nt entries; *nt_out = temperatures chosen. */
+ptx_status ptx_ladder_from_histograms(const uint32_t* counts, int32_t nt, int64_t nbins, int64_t e_base,
+                                      const double* hist_betas, double estep, double step_size, double r,
+                                      double beta_start, double beta_end, int32_t mode, double* betas_out,
+                                      int32_t* nt_out);
+/* Predicted swap acceptance between adjacent ladder entries from the same
+ * DOS (used to judge a proposed ladder against target +-0.05). */
+ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t nbins, int64_t e_base,
+                                    const double* hist_betas, double estep, const double* ladder,
+                                    int32_t n_ladder, int32_t mode, double* rates_out);
+
+/* ------------------------------------------------------------------ */
+/* Engine (replaces the solve() loop, src/solver.cc:16-166, with       */
+/* Replica::sweep replica.cc:5-38, try_replica_exchange                */
+/* replica_exchange.cc:5-100, icm icm.cc:8-215, gather_samples         */
+/* cae_estimator.cc:76-151)                                            */
+/* ------------------------------------------------------------------ */
+
+#define PTX_FLAG_ICM (1u << 0)         /* isoenergetic cluster moves (--icm) */
+#define PTX_FLAG_PILOT (1u << 2)       /* pilot run: one chain, all pairs per step */
+#define PTX_FLAG_NO_EXCHANGE (1u << 3) /* sweeps only (testing) */
+#define PTX_FLAG_HISTOGRAM (1u << 5)   /* accumulate per-slot energy histograms */
+
+#define PTX_MODE_AUTO 0
+#define PTX_MODE_SPLIT 1 /* sweep / exchange / ICM kernels per step */
+#define PTX_MODE_FUSED 2 /* one workgroup per instance runs whole PT steps */
+
+typedef struct {
+    int32_t chains;          /* 2 for the main run (initialization.cc:166), 1 for a pilot */
+    int32_t sweeps;          /* --sweeps */
+    uint32_t flags;          /* PTX_FLAG_* */
+    int32_t nc;              /* ICM slots; -1 = reference rule per instance */
+    int64_t hist_start;      /* first step whose energies enter the histogram */
+    int32_t mode;            /* PTX_MODE_* */
+    int32_t waves_per_group; /* replicas per workgroup (0 = autotuned) */
+    int32_t steps_per_launch;/* fused mode: PT steps per launch (0 = auto) */
+    int32_t use_graph;       /* capture step sequences in a hipGraph */
+} ptx_engine_config;
+void ptx_engine_config_default(ptx_engine_config* cfg);
+
+/* Creates an engine over n_instances independent problems.  Instance i has
+ * n_temps[i] temperatures per chain, its ladder in betas starting at
+ * sum_{j<i} n_temps[j].  All replicas start from Bernoulli(1/2) states drawn
+ * from the counter-based stream (seed, instance). */
+ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
+                             const int32_t* n_temps, uint64_t seed, const ptx_engine_config* cfg,
+                             ptx_engine** out);
+void ptx_engine_destroy(ptx_engine* e);
+/* Enqueues n_steps PT steps (asynchronous; errors of the launch are reported). */
+ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps);
+ptx_status ptx_engine_sync(ptx_engine* e);
+
+typedef struct {
+    int32_t n_instances;
+    int64_t steps_done;
+    int32_t mode;            /* resolved mode */
+    int32_t waves_per_group; /* resolved */
+    int32_t steps_per_launch;
+    int64_t total_replicas;
+    int64_t spin_updates_per_step; /* sum over replicas of n * sweeps */
+    int64_t lds_bytes_per_step;    /* algorithmic LDS bytes of the sweep (4 d_i + 6 per update) */
+} ptx_engine_info;
+ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out);
+
+typedef struct {
+    int32_t n, nt, nrep, nc, n_colours;
+    int32_t hist_bins;
+    int64_t e_base;
+} ptx_instance_info;
+ptx_status ptx_engine_instance_info(const ptx_engine* e, int32_t inst, ptx_instance_info* out);
+
+/* Readers (synchronize the engine stream first).  Replica order; spins in the
+ * problem's index order, one byte per spin. */
+ptx_status ptx_engine_read_spins(ptx_engine* e, int32_t inst, uint8_t* out);        /* [nrep][n] */
+ptx_status ptx_engine_write_spins(ptx_engine* e, int32_t inst, const uint8_t* in);  /* recomputes energies */
+ptx_status ptx_engine_read_energies(ptx_engine* e, int32_t inst, int64_t* out);     /* [nrep] */
+ptx_status ptx_engine_read_perm(ptx_engine* e, int32_t inst, int32_t* slot2rep);    /* [nrep] */
+ptx_status ptx_engine_read_swap_stats(ptx_engine* e, int32_t inst, uint32_t* attempts, uint32_t* successes,
+                                      uint64_t* swaps);
+ptx_status ptx_engine_read_flips(ptx_engine* e, int32_t inst, uint64_t* flips);     /* [nrep] */
+/* Per-replica best energy, the sweep it was reached in, and the state
+ * (replica.cc:23-27; checked after every colour class that accepted a flip). */
+ptx_status ptx_engine_read_best(ptx_engine* e, int32_t inst, int64_t* best_e, int64_t* best_sweep,
+                                uint8_t* best_state);
+/* Histogram counts [nrep_slots][hist_bins], energy = e_base + 2*bin. */
+ptx_status ptx_engine_read_histograms(ptx_engine* e, int32_t inst, uint32_t* counts);
+/* Energies of every instance's replica at slot 0..nrep-1 in one call:
+ * out[sum of nrep before inst + slot].  Used by time-to-target loops. */
+ptx_status ptx_engine_read_min_energies(ptx_engine* e, int64_t* out_per_instance);
+
+/* Device time of the kernels enqueued since the last reset, measured with
+ * HIP events on the engine stream: total ms and launch count per kernel. */
+typedef struct {
+    double sweep_ms, exchange_ms, icm_ms, fused_ms;
+    int64_t sweep_launches, exchange_launches, icm_launches, fused_launches;
+} ptx_engine_timing;
+ptx_status ptx_engine_timing_enable(ptx_engine* e, int32_t on);
+ptx_status ptx_engine_timing_get(ptx_engine* e, ptx_engine_timing* out);
+
+/* Device self-test of the FP64 helpers the exchange uses (exp/log portable
+ * forms), for bitwise comparison against the host forms. */
+ptx_status ptx_selftest_fp64(const double* x, int64_t n, double* exp_out, double* log_out);
+ptx_status ptx_host_fp64(const double* x, int64_t n, double* exp_out, double* log_out);
+/* Device Philox4x32-10 for counters ctr[n][4] under key[2] -> out[n][4]. */
+ptx_status ptx_selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,477 @@
+// engine.hip -- host orchestration of the gfx950 kernels (see engine.hpp).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "kernels.hip"
+#include "ladder.hpp"
+
+namespace ptx {
+
+Status hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return Status::ok();
+    const int code = (e == hipErrorOutOfMemory) ? PTX_ERR_NOMEM : PTX_ERR_DEVICE;
+    return Status::err(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PTX_TRY(expr)                  \
+    do {                               \
+        ::ptx::Status _s = (expr);     \
+        if (!_s) return _s;            \
+    } while (0)
+
+template <class T>
+Status DevBuf<T>::alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    PTX_TRY(hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"));
+    n = count;
+    return hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+}
+template <class T>
+Status DevBuf<T>::upload(const std::vector<T>& v, hipStream_t s) {
+    PTX_TRY(alloc(v.size()));
+    if (v.empty()) return Status::ok();
+    return hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+template <class T>
+Status DevBuf<T>::upload(const T* v, size_t count, size_t offset, hipStream_t s) {
+    if (offset + count > n) return Status::err(PTX_ERR_INVALID, "upload out of range");
+    return hip_check(hipMemcpyAsync(p + offset, v, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+template <class T>
+Status DevBuf<T>::download(T* v, size_t count, size_t offset, hipStream_t s) const {
+    if (offset + count > n) return Status::err(PTX_ERR_INVALID, "download out of range");
+    PTX_TRY(hip_check(hipMemcpyAsync(v, p + offset, count * sizeof(T), hipMemcpyDeviceToHost, s), "download"));
+    return hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+Engine::~Engine() {
+    for (auto& t : pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+}  // namespace
+
+Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
+                      const int32_t* n_temps, uint64_t seed_, const ptx_engine_config& c) {
+    cfg = c;
+    seed = seed_;
+    probs = problems;
+    ninst = (int32_t)problems.size();
+    if (ninst <= 0) return Status::err(PTX_ERR_INVALID, "no instances");
+    if (ninst > (1 << 24)) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^24 instances");
+    if (cfg.chains != 1 && cfg.chains != 2) return Status::err(PTX_ERR_INVALID, "chains must be 1 or 2");
+    if (cfg.sweeps < 0) return Status::err(PTX_ERR_INVALID, "negative sweeps");
+    const bool pilot = (cfg.flags & PTX_FLAG_PILOT) != 0;
+    if (cfg.chains == 1 && !pilot && !(cfg.flags & PTX_FLAG_NO_EXCHANGE))
+        return Status::err(PTX_ERR_INVALID, "a single chain needs the pilot exchange (PTX_FLAG_PILOT)");
+    if (pilot && cfg.chains != 1) return Status::err(PTX_ERR_INVALID, "the pilot run has one chain");
+    int device_count = 0;
+    if (hipGetDeviceCount(&device_count) != hipSuccess || device_count == 0)
+        return Status::err(PTX_ERR_DEVICE, "no HIP device available");
+    PTX_TRY(hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate"));
+
+    // ---- host layout ----
+    desc.resize(ninst);
+    ladders.resize(ninst);
+    std::vector<int32_t> rowptr, cls, hv, p2o;
+    std::vector<uint32_t> adj, thr;
+    std::vector<double> beta;
+    int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0, adj_total = 0;
+    const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
+    int64_t t_off = 0;
+    for (int32_t i = 0; i < ninst; ++i) {
+        const Problem& P = *problems[i];
+        const int32_t nt = n_temps[i];
+        if (nt < 1 || nt > 256) return Status::err(PTX_ERR_UNSUPPORTED, "temperatures per chain must be in [1, 256]");
+        ladders[i].assign(betas + t_off, betas + t_off + nt);
+        t_off += nt;
+        InstDesc& d = desc[i];
+        d.n = P.n;
+        d.nw = (P.n + 31) / 32;
+        d.nt = nt;
+        d.nrep = nt * cfg.chains;
+        d.ncls = P.ncls;
+        d.fmax = P.fmax;
+        d.has_field = P.has_field ? 1 : 0;
+        d.max_deg = P.max_deg;
+        int32_t nc = 0;
+        if (cfg.chains == 2 && (cfg.flags & PTX_FLAG_ICM)) {
+            nc = cfg.nc >= 0 ? std::min(cfg.nc, nt) : icm_count(ladders[i].data(), nt, P.char_scale);
+        }
+        d.nc = nc;
+        d.inst_id = (uint32_t)i;
+        // adjacency by position
+        d.rowptr_off = (int64_t)rowptr.size();
+        for (int32_t p = 0; p < P.n; ++p) {
+            rowptr.push_back((int32_t)adj_total);
+            const int32_t o = P.pos2orig[p];
+            for (int32_t q = P.rowptr[o]; q < P.rowptr[o + 1]; ++q) {
+                const uint32_t pos = (uint32_t)P.orig2pos[P.col[q]];
+                adj.push_back(pos | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
+                ++adj_total;
+            }
+        }
+        rowptr.push_back((int32_t)adj_total);
+        if (adj_total > 0x7FFFFFFF) return Status::err(PTX_ERR_UNSUPPORTED, "adjacency larger than 2^31 entries");
+        d.cls_off = (int64_t)cls.size();
+        cls.insert(cls.end(), P.class_start.begin(), P.class_start.end());
+        d.h_off = (int64_t)hv.size();
+        d.p2o_off = (int64_t)p2o.size();
+        for (int32_t p = 0; p < P.n; ++p) {
+            hv.push_back(P.h[P.pos2orig[p]]);
+            p2o.push_back(P.pos2orig[p]);
+        }
+        d.thr_off = (int64_t)thr.size();
+        if (P.fmax > 0) {
+            std::vector<uint32_t> t((size_t)nt * P.fmax);
+            threshold_table(ladders[i].data(), nt, P.fmax, t.data());
+            thr.insert(thr.end(), t.begin(), t.end());
+        }
+        d.beta_off = beta_off;
+        for (int32_t ch = 0; ch < cfg.chains; ++ch) beta.insert(beta.end(), ladders[i].begin(), ladders[i].end());
+        beta_off += d.nrep;
+        d.rep_off = rep_off;
+        rep_off += d.nrep;
+        d.spin_off = spin_off;
+        spin_off += (int64_t)d.nrep * d.nw;
+        d.e_base = (int32_t)(-P.abs_sum);
+        d.hist_bins = hist ? (int32_t)(P.abs_sum + 1) : 0;
+        d.hist_off = hist_off;
+        if (hist) hist_off += (int64_t)d.nrep * d.hist_bins;
+        nrep_max = std::max(nrep_max, d.nrep);
+        nw_max = std::max(nw_max, d.nw);
+        n_max = std::max(n_max, d.n);
+        fmax_max = std::max(fmax_max, std::max(1, d.fmax));
+        updates_per_step += (int64_t)d.nrep * P.n * cfg.sweeps;
+        lds_bytes_per_step += (int64_t)d.nrep * cfg.sweeps * (4 * P.nnz() + 6ll * P.n);
+    }
+    total_rep = rep_off;
+
+    // ---- launch shape ----
+    W = cfg.waves_per_group;
+    if (W <= 0) W = (total_rep < 4 * 256) ? 1 : 4;  // spread few replicas over CUs, group many
+    if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8");
+    spin_stride = (int32_t)align_up((int64_t)nw_max * 32, 16);
+    thr_stride = (int32_t)align_up(fmax_max, 4);
+    const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
+    if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
+    // work items: largest instances first so the tail of the grid holds the small ones
+    std::vector<int32_t> order(ninst);
+    for (int32_t i = 0; i < ninst; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return (int64_t)desc[x].n * desc[x].max_deg > (int64_t)desc[y].n * desc[y].max_deg;
+    });
+    items.clear();
+    for (int32_t i : order)
+        for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
+    pairs.clear();
+    for (int32_t i = 0; i < ninst; ++i)
+        for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
+
+    // ---- device buffers ----
+    PTX_TRY(d_desc.upload(desc, stream));
+    PTX_TRY(d_rowptr.upload(rowptr, stream));
+    PTX_TRY(d_adj.upload(adj, stream));
+    PTX_TRY(d_cls.upload(cls, stream));
+    PTX_TRY(d_h.upload(hv, stream));
+    PTX_TRY(d_p2o.upload(p2o, stream));
+    PTX_TRY(d_thr.upload(thr, stream));
+    PTX_TRY(d_beta.upload(beta, stream));
+    PTX_TRY(d_items.upload(items, stream));
+    PTX_TRY(d_pairs.upload(pairs, stream));
+    PTX_TRY(d_spins.alloc((size_t)spin_off));
+    PTX_TRY(d_best_spins.alloc((size_t)spin_off));
+    PTX_TRY(d_E.alloc((size_t)total_rep));
+    PTX_TRY(d_bestE.alloc((size_t)total_rep));
+    PTX_TRY(d_best_sweep.alloc((size_t)total_rep));
+    PTX_TRY(d_slot2rep.alloc((size_t)total_rep));
+    PTX_TRY(d_rep2slot.alloc((size_t)total_rep));
+    PTX_TRY(d_flips.alloc((size_t)total_rep));
+    PTX_TRY(d_att.alloc((size_t)total_rep));
+    PTX_TRY(d_succ.alloc((size_t)total_rep));
+    PTX_TRY(d_swaps.alloc((size_t)ninst));
+    PTX_TRY(d_hist.alloc((size_t)std::max<int64_t>(hist_off, 1)));
+    PTX_TRY(hip_check(hipStreamSynchronize(stream), "upload sync"));
+    PTX_TRY(launch_init(0));
+    return sync();
+}
+
+static DevArgs build_args(Engine& e) {
+    DevArgs a{};
+    a.desc = e.d_desc.p;
+    a.rowptr = e.d_rowptr.p;
+    a.adj = e.d_adj.p;
+    a.cls = e.d_cls.p;
+    a.hfield = e.d_h.p;
+    a.p2o = e.d_p2o.p;
+    a.thr = e.d_thr.p;
+    a.beta = e.d_beta.p;
+    a.spins = e.d_spins.p;
+    a.best_spins = e.d_best_spins.p;
+    a.E = e.d_E.p;
+    a.bestE = e.d_bestE.p;
+    a.best_sweep = e.d_best_sweep.p;
+    a.flips = e.d_flips.p;
+    a.slot2rep = e.d_slot2rep.p;
+    a.rep2slot = e.d_rep2slot.p;
+    a.swap_att = e.d_att.p;
+    a.swap_succ = e.d_succ.p;
+    a.swaps = e.d_swaps.p;
+    a.hist = e.d_hist.p;
+    a.items = e.d_items.p;
+    a.pairs = e.d_pairs.p;
+    a.seed = e.seed;
+    a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
+    a.sweeps = e.cfg.sweeps;
+    a.spin_stride = e.spin_stride;
+    a.thr_stride = e.thr_stride;
+    a.pilot = (e.cfg.flags & PTX_FLAG_PILOT) ? 1 : 0;
+    a.no_exchange = (e.cfg.flags & PTX_FLAG_NO_EXCHANGE) ? 1 : 0;
+    a.hist_on = (e.cfg.flags & PTX_FLAG_HISTOGRAM) ? 1 : 0;
+    a.hist_start = e.cfg.hist_start;
+    return a;
+}
+
+Status Engine::launch_init(int mode) {
+    DevArgs a = build_args(*this);
+    const size_t lds = (size_t)W * spin_stride;
+    const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
+    switch (W) {
+        case 1: hipLaunchKernelGGL(k_init<1>, grid, block, lds, stream, a, mode); break;
+        case 2: hipLaunchKernelGGL(k_init<2>, grid, block, lds, stream, a, mode); break;
+        case 4: hipLaunchKernelGGL(k_init<4>, grid, block, lds, stream, a, mode); break;
+        default: hipLaunchKernelGGL(k_init<8>, grid, block, lds, stream, a, mode); break;
+    }
+    return hip_check(hipGetLastError(), "k_init launch");
+}
+
+Status Engine::launch_step(uint32_t s) {
+    DevArgs a = build_args(*this);
+    auto timed = [&](int kind, auto&& fn) -> Status {
+        if (!timing) {
+            fn();
+            return hip_check(hipGetLastError(), "kernel launch");
+        }
+        Timed t;
+        t.kind = kind;
+        PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
+        PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
+        PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
+        fn();
+        PTX_TRY(hip_check(hipGetLastError(), "kernel launch"));
+        PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
+        pending.push_back(t);
+        return Status::ok();
+    };
+    if (cfg.sweeps > 0) {
+        const size_t lds = (size_t)W * (spin_stride + 4 * (size_t)thr_stride);
+        const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
+        PTX_TRY(timed(0, [&] {
+            switch (W) {
+                case 1: hipLaunchKernelGGL(k_sweep<1>, grid, block, lds, stream, a, s); break;
+                case 2: hipLaunchKernelGGL(k_sweep<2>, grid, block, lds, stream, a, s); break;
+                case 4: hipLaunchKernelGGL(k_sweep<4>, grid, block, lds, stream, a, s); break;
+                default: hipLaunchKernelGGL(k_sweep<8>, grid, block, lds, stream, a, s); break;
+            }
+        }));
+    }
+    {
+        const size_t lds = (size_t)nrep_max * 40 + 64;
+        PTX_TRY(timed(1, [&] {
+            hipLaunchKernelGGL(k_exchange, dim3((unsigned)ninst), dim3(kWave), lds, stream, a, s, nrep_max);
+        }));
+    }
+    if (!pairs.empty()) {
+        const size_t lds = (size_t)nw_max * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
+        PTX_TRY(timed(2, [&] {
+            hipLaunchKernelGGL(k_icm, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
+        }));
+    }
+    return Status::ok();
+}
+
+Status Engine::step(int64_t n) {
+    if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
+    for (int64_t i = 0; i < n; ++i) {
+        PTX_TRY(launch_step((uint32_t)steps_done));
+        ++steps_done;
+    }
+    return Status::ok();
+}
+
+Status Engine::sync() { return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+Status Engine::collect_timing() {
+    PTX_TRY(sync());
+    for (auto& t : pending) {
+        float m = 0.f;
+        PTX_TRY(hip_check(hipEventElapsedTime(&m, t.a, t.b), "hipEventElapsedTime"));
+        ms[t.kind] += m;
+        launches[t.kind] += 1;
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+    return Status::ok();
+}
+
+Status Engine::timing_get(ptx_engine_timing* out) {
+    PTX_TRY(collect_timing());
+    out->sweep_ms = ms[0];
+    out->exchange_ms = ms[1];
+    out->icm_ms = ms[2];
+    out->fused_ms = ms[3];
+    out->sweep_launches = launches[0];
+    out->exchange_launches = launches[1];
+    out->icm_launches = launches[2];
+    out->fused_launches = launches[3];
+    return Status::ok();
+}
+
+static Status check_inst(const Engine& e, int32_t inst) {
+    if (inst < 0 || inst >= e.ninst) return Status::err(PTX_ERR_INVALID, "instance index out of range");
+    return Status::ok();
+}
+
+Status Engine::read_spins(int32_t inst, uint8_t* out) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    const Problem& P = *probs[inst];
+    std::vector<uint32_t> w((size_t)d.nrep * d.nw);
+    PTX_TRY(d_spins.download(w.data(), w.size(), (size_t)d.spin_off, stream));
+    for (int32_t r = 0; r < d.nrep; ++r)
+        for (int32_t p = 0; p < d.n; ++p)
+            out[(size_t)r * d.n + P.pos2orig[p]] = (uint8_t)((w[(size_t)r * d.nw + (p >> 5)] >> (p & 31)) & 1u);
+    return Status::ok();
+}
+
+Status Engine::write_spins(int32_t inst, const uint8_t* in) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    const Problem& P = *probs[inst];
+    std::vector<uint32_t> w((size_t)d.nrep * d.nw, 0u);
+    for (int32_t r = 0; r < d.nrep; ++r)
+        for (int32_t p = 0; p < d.n; ++p)
+            if (in[(size_t)r * d.n + P.pos2orig[p]]) w[(size_t)r * d.nw + (p >> 5)] |= 1u << (p & 31);
+    PTX_TRY(d_spins.upload(w.data(), w.size(), (size_t)d.spin_off, stream));
+    PTX_TRY(launch_init(1));
+    return sync();
+}
+
+Status Engine::read_energies(int32_t inst, int64_t* out) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    std::vector<int32_t> e(d.nrep);
+    PTX_TRY(d_E.download(e.data(), e.size(), (size_t)d.rep_off, stream));
+    for (int32_t r = 0; r < d.nrep; ++r) out[r] = e[r];
+    return Status::ok();
+}
+
+Status Engine::read_perm(int32_t inst, int32_t* out) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    return d_slot2rep.download(out, (size_t)d.nrep, (size_t)d.rep_off, stream);
+}
+
+Status Engine::read_swap_stats(int32_t inst, uint32_t* att, uint32_t* succ, uint64_t* swaps) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    PTX_TRY(d_att.download(att, (size_t)d.nrep, (size_t)d.rep_off, stream));
+    PTX_TRY(d_succ.download(succ, (size_t)d.nrep, (size_t)d.rep_off, stream));
+    unsigned long long s = 0;
+    PTX_TRY(d_swaps.download(&s, 1, (size_t)inst, stream));
+    *swaps = s;
+    return Status::ok();
+}
+
+Status Engine::read_flips(int32_t inst, uint64_t* out) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    std::vector<unsigned long long> f(d.nrep);
+    PTX_TRY(d_flips.download(f.data(), f.size(), (size_t)d.rep_off, stream));
+    for (int32_t r = 0; r < d.nrep; ++r) out[r] = f[r];
+    return Status::ok();
+}
+
+Status Engine::read_best(int32_t inst, int64_t* be, int64_t* bs, uint8_t* st) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    const Problem& P = *probs[inst];
+    std::vector<int32_t> e(d.nrep), s(d.nrep);
+    PTX_TRY(d_bestE.download(e.data(), e.size(), (size_t)d.rep_off, stream));
+    PTX_TRY(d_best_sweep.download(s.data(), s.size(), (size_t)d.rep_off, stream));
+    for (int32_t r = 0; r < d.nrep; ++r) {
+        be[r] = e[r] == 0x7FFFFFFF ? INT64_MAX : (int64_t)e[r];
+        bs[r] = s[r];
+    }
+    if (st) {
+        std::vector<uint32_t> w((size_t)d.nrep * d.nw);
+        PTX_TRY(d_best_spins.download(w.data(), w.size(), (size_t)d.spin_off, stream));
+        for (int32_t r = 0; r < d.nrep; ++r)
+            for (int32_t p = 0; p < d.n; ++p)
+                st[(size_t)r * d.n + P.pos2orig[p]] = (uint8_t)((w[(size_t)r * d.nw + (p >> 5)] >> (p & 31)) & 1u);
+    }
+    return Status::ok();
+}
+
+Status Engine::read_histograms(int32_t inst, uint32_t* out) {
+    PTX_TRY(check_inst(*this, inst));
+    const InstDesc& d = desc[inst];
+    if (!(cfg.flags & PTX_FLAG_HISTOGRAM)) return Status::err(PTX_ERR_INVALID, "engine built without histograms");
+    return d_hist.download(out, (size_t)d.nrep * d.hist_bins, (size_t)d.hist_off, stream);
+}
+
+Status Engine::read_min_energies(int64_t* out) {
+    std::vector<int32_t> e((size_t)total_rep);
+    PTX_TRY(d_bestE.download(e.data(), e.size(), 0, stream));
+    for (int32_t i = 0; i < ninst; ++i) {
+        int64_t m = INT64_MAX;
+        for (int32_t r = 0; r < desc[i].nrep; ++r) {
+            const int32_t v = e[(size_t)desc[i].rep_off + r];
+            if (v != 0x7FFFFFFF) m = std::min<int64_t>(m, v);
+        }
+        out[i] = m;
+    }
+    return Status::ok();
+}
+
+// ---- self-tests ----
+Status selftest_fp64(const double* x, int64_t n, double* eo, double* lo) {
+    if (n <= 0) return Status::ok();
+    DevBuf<double> dx, de, dl;
+    PTX_TRY(dx.alloc((size_t)n));
+    PTX_TRY(de.alloc((size_t)n));
+    PTX_TRY(dl.alloc((size_t)n));
+    PTX_TRY(hip_check(hipMemcpy(dx.p, x, n * sizeof(double), hipMemcpyHostToDevice), "copy"));
+    hipLaunchKernelGGL(k_selftest_fp64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx.p, n, de.p, dl.p);
+    PTX_TRY(hip_check(hipGetLastError(), "k_selftest_fp64"));
+    PTX_TRY(hip_check(hipMemcpy(eo, de.p, n * sizeof(double), hipMemcpyDeviceToHost), "copy"));
+    return hip_check(hipMemcpy(lo, dl.p, n * sizeof(double), hipMemcpyDeviceToHost), "copy");
+}
+
+Status selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out) {
+    if (n <= 0) return Status::ok();
+    DevBuf<uint32_t> dc, dout;
+    PTX_TRY(dc.alloc((size_t)n * 4));
+    PTX_TRY(dout.alloc((size_t)n * 4));
+    PTX_TRY(hip_check(hipMemcpy(dc.p, ctr, n * 16, hipMemcpyHostToDevice), "copy"));
+    hipLaunchKernelGGL(k_selftest_philox, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dc.p, n, key[0], key[1],
+                       dout.p);
+    PTX_TRY(hip_check(hipGetLastError(), "k_selftest_philox"));
+    return hip_check(hipMemcpy(out, dout.p, n * 16, hipMemcpyDeviceToHost), "copy");
+}
+
+template struct DevBuf<uint32_t>;
+template struct DevBuf<int32_t>;
+template struct DevBuf<double>;
+
+}  // namespace ptx

@@ -1,0 +1,94 @@
+// engine.hpp -- host side of the engine: device buffers, instance layout,
+// launch orchestration on one HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "../../include/ptx.h"
+#include "problem.hpp"
+#include "ptx_common.h"
+
+namespace ptx {
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    Status alloc(size_t count);
+    Status upload(const std::vector<T>& v, hipStream_t s);
+    Status upload(const T* v, size_t count, size_t offset, hipStream_t s);
+    Status download(T* v, size_t count, size_t offset, hipStream_t s) const;
+};
+
+struct Engine {
+    ptx_engine_config cfg{};
+    uint64_t seed = 0;
+    int32_t ninst = 0;
+    std::vector<std::shared_ptr<const Problem>> probs;
+    std::vector<InstDesc> desc;
+    std::vector<std::vector<double>> ladders;  // per instance, nt entries
+    int64_t total_rep = 0;
+    int64_t steps_done = 0;
+    int32_t W = 1;               // waves per sweep workgroup
+    int32_t nrep_max = 0, nw_max = 0, n_max = 0, fmax_max = 1;
+    int32_t spin_stride = 0, thr_stride = 1;
+    int64_t updates_per_step = 0, lds_bytes_per_step = 0;
+    std::vector<int2> items, pairs;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    struct Timed {
+        hipEvent_t a, b;
+        int kind;
+    };
+    std::vector<Timed> pending;
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+
+    DevBuf<InstDesc> d_desc;
+    DevBuf<int32_t> d_rowptr, d_cls, d_h, d_p2o;
+    DevBuf<uint32_t> d_adj, d_thr;
+    DevBuf<double> d_beta;
+    DevBuf<uint32_t> d_spins, d_best_spins;
+    DevBuf<int32_t> d_E, d_bestE, d_best_sweep, d_slot2rep, d_rep2slot;
+    DevBuf<unsigned long long> d_flips, d_swaps;
+    DevBuf<uint32_t> d_att, d_succ, d_hist;
+    DevBuf<int2> d_items, d_pairs;
+
+    ~Engine();
+    Status create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
+                  const int32_t* n_temps, uint64_t seed, const ptx_engine_config& c);
+    Status step(int64_t n);
+    Status sync();
+    Status read_spins(int32_t inst, uint8_t* out);
+    Status write_spins(int32_t inst, const uint8_t* in);
+    Status read_energies(int32_t inst, int64_t* out);
+    Status read_perm(int32_t inst, int32_t* out);
+    Status read_swap_stats(int32_t inst, uint32_t* att, uint32_t* succ, uint64_t* swaps);
+    Status read_flips(int32_t inst, uint64_t* out);
+    Status read_best(int32_t inst, int64_t* be, int64_t* bs, uint8_t* st);
+    Status read_histograms(int32_t inst, uint32_t* out);
+    Status read_min_energies(int64_t* out);
+    Status timing_get(ptx_engine_timing* out);
+
+  private:
+    Status launch_init(int mode);
+    Status launch_step(uint32_t s);
+    Status collect_timing();
+};
+
+Status hip_check(hipError_t e, const char* what);
+
+}  // namespace ptx

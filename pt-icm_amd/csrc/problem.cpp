@@ -1,0 +1,242 @@
+// problem.cpp -- see problem.hpp.
+#include "problem.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+
+#include "../../include/ptx.h"
+
+namespace ptx {
+
+namespace {
+
+struct RawEdge {
+    int64_t u, v;
+    double w;
+};
+
+bool integral(double x) { return std::isfinite(x) && x == std::floor(x) && std::fabs(x) <= 32767.0; }
+
+// Builds the CSR.  Sparse instances keep every listed entry, duplicates and
+// zero weights included, at both ends in file order (problem.cc:226-229; the
+// sparse branch of bfs_icm, icm.cc:53-58, follows every listed entry).  Dense
+// instances follow the matrix: the last weight of a pair wins and zero
+// entries are no edge (problem.cc:230-232, icm.cc:61-66).
+Status assemble(int32_t n, int64_t m_header, bool sparse, const std::vector<RawEdge>& edges,
+                const std::vector<double>& hin, double char_scale, Problem& P) {
+    P = Problem();
+    P.n = n;
+    P.m_header = m_header;
+    P.sparse = sparse;
+    P.char_scale = char_scale;
+    P.h.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        if (!integral(hin[i]))
+            return Status::err(PTX_ERR_UNSUPPORTED, "linear term is not an integer in [-32767, 32767]");
+        P.h[i] = (int32_t)hin[i];
+    }
+    std::vector<int64_t> deg(n, 0);
+    for (const auto& e : edges) {
+        if (e.u < 0 || e.u >= n || e.v < 0 || e.v >= n) return Status::err(PTX_ERR_INVALID, "edge endpoint out of range");
+        if (!integral(e.w)) return Status::err(PTX_ERR_UNSUPPORTED, "edge weight is not an integer in [-32767, 32767]");
+    }
+    if (sparse) {
+        for (const auto& e : edges) {
+            if (e.u == e.v) return Status::err(PTX_ERR_UNSUPPORTED, "self-loop in a sparse instance");
+            deg[e.u]++;
+            deg[e.v]++;
+        }
+        P.rowptr.assign(n + 1, 0);
+        for (int32_t i = 0; i < n; ++i) P.rowptr[i + 1] = P.rowptr[i] + (int32_t)deg[i];
+        P.col.assign(P.rowptr[n], 0);
+        P.w.assign(P.rowptr[n], 0);
+        std::vector<int32_t> fill(P.rowptr.begin(), P.rowptr.end() - 1);
+        for (const auto& e : edges) {
+            P.col[fill[e.u]] = (int32_t)e.v;
+            P.w[fill[e.u]++] = (int32_t)e.w;
+            P.col[fill[e.v]] = (int32_t)e.u;
+            P.w[fill[e.v]++] = (int32_t)e.w;
+        }
+    } else {
+        // dense: row-major matrix, last write wins
+        std::vector<double> mat((size_t)n * n, 0.0);
+        for (const auto& e : edges) {
+            mat[(size_t)e.u * n + e.v] = e.w;
+            mat[(size_t)e.v * n + e.u] = e.w;
+        }
+        P.rowptr.assign(n + 1, 0);
+        for (int32_t i = 0; i < n; ++i) {
+            int32_t c = 0;
+            for (int32_t j = 0; j < n; ++j)
+                if (j != i && std::fabs(mat[(size_t)i * n + j]) > 1e-10) ++c;
+            P.rowptr[i + 1] = P.rowptr[i] + c;
+        }
+        P.col.reserve(P.rowptr[n]);
+        P.w.reserve(P.rowptr[n]);
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t j = 0; j < n; ++j)
+                if (j != i && std::fabs(mat[(size_t)i * n + j]) > 1e-10) {
+                    P.col.push_back(j);
+                    P.w.push_back((int32_t)mat[(size_t)i * n + j]);
+                }
+    }
+    return P.finalize();
+}
+
+}  // namespace
+
+Status Problem::finalize() {
+    if (n <= 0) return Status::err(PTX_ERR_INVALID, "empty problem");
+    if (n > 65536) return Status::err(PTX_ERR_UNSUPPORTED, "more than 65536 spins");
+    // greedy colouring in index order: smallest colour unused by lower-index neighbours
+    colour.assign(n, 0);
+    max_deg = 0;
+    for (int32_t i = 0; i < n; ++i) max_deg = std::max(max_deg, rowptr[i + 1] - rowptr[i]);
+    std::vector<int32_t> seen(max_deg + 2, -1);
+    ncls = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+            const int32_t j = col[q];
+            if (j < i && colour[j] <= max_deg) seen[colour[j]] = i;
+        }
+        int32_t c = 0;
+        while (seen[c] == i) ++c;
+        colour[i] = c;
+        ncls = std::max(ncls, c + 1);
+    }
+    class_start.assign(ncls + 1, 0);
+    for (int32_t i = 0; i < n; ++i) class_start[colour[i] + 1]++;
+    for (int32_t c = 0; c < ncls; ++c) class_start[c + 1] += class_start[c];
+    pos2orig.assign(n, 0);
+    orig2pos.assign(n, 0);
+    std::vector<int32_t> fill(class_start.begin(), class_start.end() - 1);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t p = fill[colour[i]]++;
+        pos2orig[p] = i;
+        orig2pos[i] = p;
+    }
+    // field bound and energy range
+    fmax = 0;
+    abs_sum = 0;
+    has_field = false;
+    int64_t wsum2 = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        int64_t f = std::llabs((long long)h[i]);
+        if (h[i] != 0) has_field = true;
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) f += std::llabs((long long)w[q]);
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) wsum2 += std::llabs((long long)w[q]);
+        fmax = (int32_t)std::max<int64_t>(fmax, f);
+        abs_sum += std::llabs((long long)h[i]);
+    }
+    abs_sum += wsum2 / 2;
+    if (abs_sum > (1ll << 30)) return Status::err(PTX_ERR_UNSUPPORTED, "energy range exceeds 32-bit engine energies");
+    if (fmax > 8192) return Status::err(PTX_ERR_UNSUPPORTED, "local field bound above 8192 (threshold table too large)");
+    return Status::ok();
+}
+
+int64_t Problem::energy(const uint8_t* s) const {
+    int64_t pair2 = 0, lin = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t si = s[i] ? 1 : -1;
+        lin += si * h[i];
+        int64_t acc = 0;
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) acc += (s[col[q]] ? w[q] : -w[q]);
+        pair2 += si * acc;
+    }
+    return lin + pair2 / 2;
+}
+
+int64_t Problem::maxcut(const uint8_t* s) const {
+    int64_t c2 = 0;
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+            if (s[i] != s[col[q]]) c2 += w[q];
+    return c2 / 2;
+}
+
+int64_t Problem::delta_e(const uint8_t* s, int64_t* dE) const {
+    for (int32_t i = 0; i < n; ++i) {
+        int64_t f = h[i];
+        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) f += (s[col[q]] ? w[q] : -w[q]);
+        dE[i] = (s[i] ? -2 : 2) * f;
+    }
+    return energy(s);
+}
+
+Status read_gset(const std::string& path, Problem& out) {
+    std::ifstream in(path);
+    if (!in.is_open()) return Status::err(PTX_ERR_IO, "cannot open problem file: " + path);
+    const bool ising = path.find(".ising") != std::string::npos;
+    const bool zero_based = path.find(".gset") != std::string::npos || ising;
+    long long N = 0, L = 0, M = 0;
+    if (ising) in >> N >> L >> M;
+    else in >> N >> M;
+    if (!in || N <= 0 || M < 0 || L < 0 || N > 65536) {
+        if (N > 65536) return Status::err(PTX_ERR_UNSUPPORTED, "more than 65536 spins");
+        return Status::err(PTX_ERR_INVALID, "malformed header in " + path);
+    }
+    const bool sparse = ((double)M / (double)(N * (N - 1) / 2)) < 0.1;  // problem.cc:198
+    const long long off = zero_based ? 0 : -1;                          // problem.cc:208
+    std::vector<double> h(N, 0.0);
+    std::vector<RawEdge> edges;
+    edges.reserve((size_t)M);
+    double sum = 0.0, sqsum = 0.0;
+    long long nlin = 0;
+    std::string line;
+    std::getline(in, line);  // remainder of the header line
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.size() <= 1) continue;  // problem.cc:211
+        std::istringstream ls(line);
+        if (nlin < L) {
+            long long u;
+            double w;
+            if (!(ls >> u >> w)) return Status::err(PTX_ERR_INVALID, "malformed linear-term line: " + line);
+            u += off;
+            if (u < 0 || u >= N) return Status::err(PTX_ERR_INVALID, "linear-term index out of range: " + line);
+            h[u] = w;
+            ++nlin;
+            continue;
+        }
+        long long u, v;
+        if (!(ls >> u >> v)) return Status::err(PTX_ERR_INVALID, "malformed edge line: " + line);
+        double w = 1.0;  // problem.cc:229-233
+        double tmp;
+        if (ls >> tmp) w = tmp;
+        u += off;
+        v += off;
+        if (u < 0 || u >= N || v < 0 || v >= N) return Status::err(PTX_ERR_INVALID, "edge endpoint out of range: " + line);
+        edges.push_back({u, v, w});
+        sum += w;
+        sqsum += w * w;
+    }
+    const double mean = sum / M;
+    const double variance = sqsum / M - (sum / M * sum / M);  // problem.cc:244
+    const double cs = mean < variance ? variance : mean;     // std::max
+    return assemble((int32_t)N, M, sparse, edges, h, cs, out);
+}
+
+Status build_from_coo(int32_t n, int64_t m, const int32_t* src, const int32_t* dst, const double* w,
+                      const double* h, bool dense, Problem& out) {
+    if (n <= 0 || m < 0 || (m > 0 && (!src || !dst || !w))) return Status::err(PTX_ERR_INVALID, "bad COO arguments");
+    std::vector<RawEdge> edges((size_t)m);
+    double sum = 0.0, sqsum = 0.0;
+    for (int64_t e = 0; e < m; ++e) {
+        edges[e] = {src[e], dst[e], w[e]};
+        sum += w[e];
+        sqsum += w[e] * w[e];
+    }
+    std::vector<double> hv(n, 0.0);
+    if (h) std::copy(h, h + n, hv.begin());
+    double cs = 0.0;
+    if (m > 0) {
+        const double mean = sum / m, var = sqsum / m - (sum / m * sum / m);
+        cs = mean < var ? var : mean;
+    }
+    return assemble(n, m, !dense, edges, hv, cs, out);
+}
+
+}  // namespace ptx

@@ -1,0 +1,244 @@
+"""Bit-exact parity of the HIP engine with the CPU oracle (colour-class order,
+same counter-based random stream): spins, int energies, flips, best states,
+temperature permutation, swap counters, ICM clusters and histograms.  Every
+test runs through the C ABI of libptx.so on the GPU."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REPO, graph_path
+
+sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+import instances  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_pair(ptx, oracle, kind, **kw):
+    """Returns (product Problem, oracle Graph) for a named test instance."""
+    if kind.endswith(".gset") or kind.startswith("G0"):
+        return ptx.Problem.read(graph_path(kind)), oracle.Graph.read(graph_path(kind))
+    if kind == "pm_torus20":  # C1's instance: 2D +-J torus L=20
+        n, u, v, w = instances.torus_2d(20, seed=0)
+        h = None
+    elif kind == "ea3d_6":
+        n, u, v, w = instances.ea_3d(6, seed=kw.get("seed", 1))
+        h = None
+    elif kind == "field_random":
+        rng = np.random.default_rng(12)
+        n = 300
+        m = 1500
+        u = rng.integers(0, n, m).astype(np.int32)
+        v = rng.integers(0, n, m).astype(np.int32)
+        keep = u != v
+        u, v = u[keep], v[keep]
+        w = rng.integers(-3, 4, len(u)).astype(np.float64)
+        h = rng.integers(-2, 3, n).astype(np.float64)
+    elif kind == "er_small":
+        n, u, v, w = instances.erdos_renyi(500, 0.02, seed=3, pm=True)
+        h = None
+    else:
+        raise KeyError(kind)
+    return ptx.Problem.from_coo(n, u, v, w, h), oracle.Graph.from_coo(n, u, v, w, h)
+
+
+def compare(eng, oe, inst=0, what=""):
+    sp = eng.spins(inst)
+    osp = oe.spins()
+    assert np.array_equal(sp, osp), f"{what}: spins differ in {np.sum(sp != osp)} places"
+    assert np.array_equal(eng.energies(inst), oe.energies()), what
+    assert np.array_equal(eng.slot2rep(inst), oe.slot2rep()), what
+    assert np.array_equal(eng.flips(inst), oe.flips()), what
+    be, bs, bst = eng.best(inst)
+    obe, obs, obst = oe.best()
+    assert np.array_equal(be, obe) and np.array_equal(bs, obs), what
+    assert np.array_equal(bst, obst), what
+    a, s, n = eng.swap_stats(inst)
+    oa, os_, on = oe.swap_stats()
+    assert np.array_equal(a, oa) and np.array_equal(s, os_) and n == on, what
+
+
+# ------------------------------------------------------------- primitives ---
+def test_device_philox_known_answers(ptx, oracle):
+    ctr = np.array([[0, 0, 0, 0], [0xFFFFFFFF] * 4, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344]], np.uint32)
+    keys = [[0, 0], [0xFFFFFFFF] * 2, [0xA4093822, 0x299F31D0]]
+    for i in range(3):
+        got = ptx.device_philox(ctr[i:i + 1], keys[i])[0]
+        assert list(got) == list(oracle.philox(ctr[i], keys[i]))
+    rng = np.random.default_rng(0)
+    c = rng.integers(0, 2 ** 32, (4096, 4), dtype=np.uint64).astype(np.uint32)
+    got = ptx.device_philox(c, [123, 456])
+    for i in range(0, 4096, 97):
+        assert list(got[i]) == list(oracle.philox(c[i], [123, 456]))
+
+
+def test_device_fp64_bitwise(ptx, oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(1)
+    x = np.concatenate([-rng.random(50000) * 60, -rng.random(5000) * 745, [0.0, -745.0, -708.3]])
+    e, _ = ptx.device_fp64(x)
+    bad = [i for i in range(len(x)) if e[i] != L.or_exp(float(x[i]))]
+    assert not bad, (len(bad), x[bad[:3]])
+    u = np.concatenate([rng.integers(1, 2 ** 32, 50000) * 2.0 ** -32, [1.0, 0.5, 2 ** -32]])
+    _, lg = ptx.device_fp64(u)
+    bad = [i for i in range(len(u)) if lg[i] != L.or_log(float(u[i]))]
+    assert not bad, (len(bad), u[bad[:3]])
+
+
+# ------------------------------------------------------------------ sweeps --
+SWEEP_CASES = ["lattice_2d_5.gset", "lattice_2d_20.gset", "pm_torus20", "G022", "field_random", "ea3d_6"]
+
+
+@pytest.mark.parametrize("kind", SWEEP_CASES)
+def test_init_and_sweep_parity(ptx, oracle, kind):
+    p, g = make_pair(ptx, oracle, kind)
+    nt = 6
+    betas = np.linspace(0.3, 2.0, nt)
+    say("case", kind, "n", p.n)
+    eng = ptx.Engine(p, betas, seed=17, flags=ptx.FLAG_NO_EXCHANGE)
+    oe = oracle.Engine(g, betas, chains=2, seed=17, flags=oracle.FLAG_NO_EXCHANGE)
+    compare(eng, oe, what="init")
+    for s in range(4):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"step {s}")
+
+
+@pytest.mark.parametrize("sweeps", [2, 3])
+def test_multiple_sweeps_per_step(ptx, oracle, sweeps):
+    p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
+    betas = np.linspace(0.4, 1.2, 5)
+    eng = ptx.Engine(p, betas, seed=2, sweeps=sweeps, flags=ptx.FLAG_NO_EXCHANGE)
+    oe = oracle.Engine(g, betas, chains=2, seed=2, sweeps=sweeps, flags=oracle.FLAG_NO_EXCHANGE)
+    eng.step(3)
+    oe.step(3)
+    compare(eng, oe, what="sweeps")
+
+
+# ------------------------------------------------------- full PT steps ------
+FULL_CASES = [("lattice_2d_20.gset", 8, 0.5, 3.0, 20), ("pm_torus20", 32, 0.5, 3.0, 12),
+              ("G022", 16, 0.6, 2.8, 6), ("field_random", 10, 0.5, 3.0, 10), ("er_small", 12, 0.4, 2.5, 10)]
+
+
+@pytest.mark.parametrize("kind,nt,T0,T1,steps", FULL_CASES)
+@pytest.mark.parametrize("icm", [False, True])
+def test_full_step_parity(ptx, oracle, kind, nt, T0, T1, steps, icm):
+    p, g = make_pair(ptx, oracle, kind)
+    betas = oracle.beta_linspace(T0, T1, nt, 0)
+    say("case", kind, "nt", nt, "icm", icm)
+    flags = ptx.FLAG_ICM if icm else 0
+    nc = -1 if kind != "field_random" else nt // 2
+    eng = ptx.Engine(p, betas, seed=5, flags=flags, nc=nc)
+    oe = oracle.Engine(g, betas, chains=2, seed=5, flags=(oracle.FLAG_ICM if icm else 0), nc=nc)
+    assert eng.instance_info().nc == oe.nc
+    for s in range(steps):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"{kind} step {s}")
+    # bulk stepping (no host reads in between) ends in the same state
+    eng.step(25)
+    oe.step(25)
+    compare(eng, oe, what=f"{kind} bulk")
+
+
+def test_icm_all_slots_parity(ptx, oracle):
+    """ICM on every slot (nc = nt), including global flips of replica 1."""
+    p, g = make_pair(ptx, oracle, "G039")
+    nt = 8
+    betas = oracle.beta_linspace(0.5, 2.0, nt, 0)
+    eng = ptx.Engine(p, betas, seed=33, flags=ptx.FLAG_ICM, nc=nt)
+    oe = oracle.Engine(g, betas, chains=2, seed=33, flags=oracle.FLAG_ICM, nc=nt)
+    for s in range(12):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"step {s}")
+
+
+@pytest.mark.parametrize("wpg", [1, 2, 4, 8])
+def test_launch_shape_independence(ptx, oracle, wpg):
+    p, g = make_pair(ptx, oracle, "pm_torus20")
+    betas = oracle.beta_linspace(0.5, 3.0, 16, 0)
+    eng = ptx.Engine(p, betas, seed=9, flags=ptx.FLAG_ICM, waves_per_group=wpg)
+    oe = oracle.Engine(g, betas, chains=2, seed=9, flags=oracle.FLAG_ICM)
+    eng.step(15)
+    oe.step(15)
+    compare(eng, oe, what=f"W={wpg}")
+
+
+def test_write_spins_recomputes_energy(ptx, oracle):
+    p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
+    betas = np.linspace(0.5, 1.5, 4)
+    eng = ptx.Engine(p, betas, seed=1)
+    rng = np.random.default_rng(3)
+    s = rng.integers(0, 2, (8, p.n)).astype(np.uint8)
+    eng.set_spins(s)
+    assert np.array_equal(eng.spins(), s)
+    assert np.array_equal(eng.energies(), [g.energy(x) for x in s])
+
+
+# ---------------------------------------------------------- pilot / hist ----
+def test_pilot_histograms_match_oracle_logs(ptx, oracle):
+    p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
+    nt, burn, ns = 10, 100, 400
+    hb, bs, be = ptx.pilot_betas(0.6, 2.8, nt)
+    eng = ptx.Engine(p, hb, seed=21, chains=1, flags=ptx.FLAG_PILOT | ptx.FLAG_HISTOGRAM, hist_start=burn)
+    eng.step(burn + ns)
+    counts, e_base = eng.histograms()
+    logs = oracle.gather_samples(g, hb, 21, burn, ns)
+    assert counts.sum() == ns * nt
+    for j in range(nt):
+        want = np.bincount((logs[:, j] - e_base) // 2, minlength=counts.shape[1])
+        assert np.array_equal(counts[j], want), j
+    lad = ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_EXACT)
+    olad, _, _ = oracle.get_beta_from_logs(logs, hb, 2.0, 0.1, 0.4, bs, be)
+    assert np.array_equal(lad, olad)
+
+
+# ---------------------------------------------------------------- batched ---
+def test_batched_instances_match_single_oracles(ptx, oracle):
+    kinds = ["lattice_2d_10.gset", "ea3d_6", "pm_torus20", "lattice_2d_5.gset", "er_small"]
+    pairs = [make_pair(ptx, oracle, k) for k in kinds]
+    nts = [4, 8, 6, 3, 5]
+    ladders = [oracle.beta_linspace(0.5, 3.0, nt, 0) for nt in nts]
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=77, flags=ptx.FLAG_ICM)
+    oes = [oracle.Engine(g, lad, chains=2, seed=77, instance=i, flags=oracle.FLAG_ICM)
+           for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
+    for s in range(6):
+        eng.step(1)
+        for oe in oes:
+            oe.step(1)
+        for i, oe in enumerate(oes):
+            compare(eng, oe, inst=i, what=f"inst {i} step {s}")
+
+
+def test_batched_3d_ea_invariants(ptx, oracle):
+    """A C3-shaped batch (uneven 3D EA instances, 32 temperatures, ICM):
+    tracked energies equal recomputed ones, permutations stay within chains."""
+    rng = np.random.default_rng(0)
+    Ls = rng.choice([6, 8, 10, 12], size=48)
+    probs = []
+    graphs = []
+    for i, L in enumerate(Ls):
+        n, u, v, w = instances.ea_3d(int(L), seed=1000 + i)
+        probs.append(ptx.Problem.from_coo(n, u, v, w))
+        graphs.append(oracle.Graph.from_coo(n, u, v, w))
+    lad = oracle.beta_linspace(0.2, 3.0, 32, 1)
+    eng = ptx.Engine(probs, [lad] * len(probs), seed=3, flags=ptx.FLAG_ICM)
+    eng.step(40)
+    for i in range(0, len(probs), 7):
+        sp = eng.spins(i)
+        assert np.array_equal(eng.energies(i), [graphs[i].energy(x) for x in sp])
+        s2r = eng.slot2rep(i)
+        assert set(s2r[:32].tolist()) == set(range(32)) and set(s2r[32:].tolist()) == set(range(32, 64))
+        a, s, nsw = eng.swap_stats(i)
+        assert a.sum() == 40 * 2 * 31 and s.sum() == nsw
+    # one instance checked bit for bit against the oracle
+    oe = oracle.Engine(graphs[5], lad, chains=2, seed=3, instance=5, flags=oracle.FLAG_ICM)
+    oe.step(40)
+    compare(eng, oe, inst=5, what="C3 instance 5")

@@ -1,0 +1,154 @@
+"""CPU tests of the product's host side through the C ABI (no GPU calls):
+exported symbols, reader/energy/colouring/ladder parity with the oracle, and
+error behaviour at the boundary."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO, graph_path
+
+GRAPHS = ["lattice_2d_5.gset", "lattice_2d_10.gset", "lattice_2d_20.gset", "G022", "G039"]
+
+
+def test_every_declared_symbol_is_exported(ptx):
+    hdr = open(os.path.join(REPO, "include", "ptx.h")).read()
+    declared = set(re.findall(r"^(?:ptx_status|void|int32_t|const char\*)\s+(ptx_[a-z0-9_]+)\s*\(", hdr, re.M))
+    L = ptx.lib()
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert declared <= set(ptx.exported_symbols()) | {"ptx_engine_config_default"}
+
+
+@pytest.mark.parametrize("name", GRAPHS)
+def test_reader_matches_oracle(ptx, oracle, name):
+    p = ptx.Problem.read(graph_path(name))
+    g = oracle.Graph.read(graph_path(name))
+    assert p.n == g.n and p.info.nnz == g.nnz and bool(p.info.sparse) == g.sparse
+    assert p.info.char_scale == g.char_scale
+    for a, b in zip(p.csr(), g.csr()):
+        assert np.array_equal(a, b)
+    k, c = g.colour()
+    p2o, cs = g.positions()
+    q2o, qs = p.colouring()
+    assert p.info.n_colours == k and np.array_equal(p2o, q2o) and np.array_equal(cs, qs)
+    assert p.info.fmax == g.fmax()
+    rng = np.random.default_rng(0)
+    for _ in range(4):
+        s = rng.integers(0, 2, p.n).astype(np.uint8)
+        assert p.energy(s) == g.energy(s)
+        assert p.maxcut(s) == g.maxcut(s)
+        d, e = p.delta_e(s)
+        assert e == g.energy(s) and np.array_equal(d, g.delta_e(s))
+
+
+def test_coo_field_and_dense(ptx, oracle):
+    rng = np.random.default_rng(4)
+    n = 40
+    u, v = np.triu_indices(n, 1)
+    keep = rng.random(len(u)) < 0.3
+    u, v = u[keep].astype(np.int32), v[keep].astype(np.int32)
+    w = rng.integers(-3, 4, len(u)).astype(np.float64)
+    h = rng.integers(-2, 3, n).astype(np.float64)
+    for dense in (False, True):
+        p = ptx.Problem.from_coo(n, u, v, w, h, dense=dense)
+        g = oracle.Graph.from_coo(n, u, v, w, h, dense=dense)
+        for a, b in zip(p.csr(), g.csr()):
+            assert np.array_equal(a, b)
+        s = rng.integers(0, 2, n).astype(np.uint8)
+        assert p.energy(s) == g.energy(s)
+        assert p.info.has_field == 1
+
+
+def test_ladders_match_oracle(ptx, oracle):
+    for T0, T1, n, inv in [(0.5, 3.0, 32, 0), (2, 0.25, 20, 0), (0.2, 3.0, 32, 1), (3.0, 0.2, 7, 1)]:
+        assert np.array_equal(ptx.beta_linspace(T0, T1, n, inv), oracle.beta_linspace(T0, T1, n, inv))
+        hb, bs, be = ptx.pilot_betas(T0, T1, n, inv)
+        hb2, bs2, be2 = oracle.hist_betas(T0, T1, n, inv)
+        assert np.array_equal(hb, hb2) and bs == bs2 and be == be2
+    b = np.linspace(0.2, 3.0, 29)
+    for cs in (0.1, 1.0, 2.9, 5.0):
+        assert ptx.icm_count(b, cs) == oracle.icm_count(b, cs)
+
+
+def test_portable_fp64_host_matches_oracle(ptx, oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    x = np.concatenate([-rng.random(20000) * 60, -rng.random(2000) * 745, [0.0, -1e-300, -745.0, -708.0]])
+    e, _ = ptx.host_fp64(x)
+    assert all(e[i] == L.or_exp(float(x[i])) for i in range(len(x)))
+    u = np.concatenate([rng.integers(1, 2 ** 32, 20000) * 2.0 ** -32, [1.0, 0.5, 2 ** -32, 1 - 2 ** -32]])
+    _, lg = ptx.host_fp64(u)
+    assert all(lg[i] == L.or_log(float(u[i])) for i in range(len(u)))
+
+
+def _pilot_logs(oracle, name, nt, ns, seed=0):
+    g = oracle.Graph.read(graph_path(name))
+    hb, bs, be = oracle.hist_betas(1.0, 3.0, nt, 0)
+    logs = oracle.gather_samples(g, hb, seed, 200, ns)
+    return g, hb, bs, be, logs
+
+
+def _logs_to_engine_hist(logs, e_base):
+    nt = logs.shape[1]
+    nb = int((logs.max() - e_base) // 2) + 1
+    h = np.zeros((nt, nb), np.uint32)
+    for j in range(nt):
+        b = (logs[:, j] - e_base) // 2
+        np.add.at(h[j], b, 1)
+    return h
+
+
+def test_ladder_from_histograms_matches_oracle_exactly(ptx, oracle):
+    g, hb, bs, be, logs = _pilot_logs(oracle, "lattice_2d_10.gset", 8, 3000)
+    want, logdos, logz = oracle.get_beta_from_logs(logs, hb, 2.0, 0.1, 0.4, bs, be)
+    e_base = -200
+    counts = _logs_to_engine_hist(logs, e_base)
+    got = ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_EXACT)
+    assert np.array_equal(got, want)
+    fast = ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_FAST)
+    assert len(fast) == len(want) and np.allclose(fast, want, rtol=1e-9)
+
+
+def test_fast_acceptance_equals_exact(ptx, oracle):
+    g, hb, bs, be, logs = _pilot_logs(oracle, "lattice_2d_20.gset", 10, 2000, seed=3)
+    e_base = -800
+    counts = _logs_to_engine_hist(logs, e_base)
+    lad = np.linspace(hb[0], hb[-1], 12)
+    r_exact = ptx.ladder_predict_rates(counts, e_base, hb, lad, mode=ptx.LADDER_EXACT)
+    r_fast = ptx.ladder_predict_rates(counts, e_base, hb, lad, mode=ptx.LADDER_FAST)
+    assert np.allclose(r_exact, r_fast, rtol=1e-10, atol=1e-14)
+    assert np.all((r_exact > 0) & (r_exact <= 1))
+
+
+# ------------------------------------------------------------- errors -------
+def test_errors_are_status_codes(ptx, tmp_path):
+    with pytest.raises(ptx.PtxError) as ei:
+        ptx.Problem.read(str(tmp_path / "missing.gset"))
+    assert ei.value.code == -2
+    bad = tmp_path / "bad.gset"
+    bad.write_text("3 2\n0 1 1\nx y z\n")
+    with pytest.raises(ptx.PtxError) as ei:
+        ptx.Problem.read(str(bad))
+    assert ei.value.code == -1
+    frac = tmp_path / "frac.gset"
+    frac.write_text("3 2\n0 1 0.5\n1 2 1\n")
+    with pytest.raises(ptx.PtxError) as ei:
+        ptx.Problem.read(str(frac))
+    assert ei.value.code == -3
+    trunc = tmp_path / "trunc.gset"
+    trunc.write_text("3 2\n0 7 1\n")
+    with pytest.raises(ptx.PtxError) as ei:
+        ptx.Problem.read(str(trunc))
+    assert ei.value.code == -1
+
+
+def test_engine_without_device_fails_loudly(ptx):
+    if ptx.lib().ptx_device_count() > 0:
+        pytest.skip("a GPU is present")
+    p = ptx.Problem.read(graph_path("lattice_2d_10.gset"))
+    with pytest.raises(ptx.PtxError) as ei:
+        ptx.Engine(p, ptx.beta_linspace(0.5, 3.0, 4))
+    assert ei.value.code == -4

@@ -1,0 +1,457 @@
+"""Pins the CPU oracle: known-answer vectors, closed forms, brute force and
+invariants chosen so that a dropped term, a wrong sign or a transposed index
+fails one of them.  Runs on the CPU only."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REPO, graph_path
+
+GOLD = os.path.join(REPO, "tests", "golden")
+
+
+# ---------------------------------------------------------------- Philox ----
+def test_philox_known_answers(oracle):
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert list(oracle.philox(ctr, key)) == want
+
+
+def test_philox_uniformity(oracle):
+    # words of consecutive counters look uniform (mean, top-bit balance)
+    xs = np.concatenate([oracle.philox([i, 7, 3, 2], [5, 9]) for i in range(4000)]).astype(np.float64)
+    assert abs(xs.mean() / 2 ** 32 - 0.5) < 0.01
+    assert abs((xs >= 2 ** 31).mean() - 0.5) < 0.02
+
+
+# ------------------------------------------------------------ exp / log -----
+def test_portable_exp_log_accuracy(oracle):
+    L = oracle.lib()
+    xs = np.linspace(-700, 0, 20001)
+    worst = max(abs(L.or_exp(x) - math.exp(x)) / math.exp(x) for x in xs)
+    assert worst < 1e-15
+    assert L.or_exp(0.0) == 1.0 and L.or_exp(-1000.0) == 0.0
+    rng = np.random.default_rng(1)
+    for r in list(rng.integers(1, 2 ** 32, 50000)) + [1, 2, 2 ** 31, 2 ** 32 - 1]:
+        u = int(r) * 2.0 ** -32
+        assert abs(L.or_uniform_log(int(r)) - math.log(u)) < 4e-15
+    assert L.or_log(1.0) == 0.0
+    for k in range(-40, 40):
+        assert abs(L.or_log(2.0 ** k) - k * math.log(2)) < 1e-13
+    assert L.or_uniform_log(0) == -math.inf
+
+
+# ------------------------------------------------------------- reader -------
+@pytest.mark.parametrize("name,n,sparse", [("lattice_2d_5.gset", 25, False), ("lattice_2d_10.gset", 100, True),
+                                           ("lattice_2d_20.gset", 400, True)])
+def test_lattice_closed_forms(oracle, name, n, sparse):
+    g = oracle.Graph.read(graph_path(name))
+    assert g.n == n and g.nnz == 4 * n and g.sparse == sparse
+    up = np.ones(n, np.uint8)
+    assert g.energy(up) == -2 * n  # ferromagnet, w = -1 on 2N bonds
+    assert g.energy(np.zeros(n, np.uint8)) == -2 * n
+    # checkerboard: every bond frustrated
+    L = int(round(math.sqrt(n)))
+    if L % 2 == 0:
+        cb = np.array([(i // L + i % L) % 2 for i in range(n)], np.uint8)
+        assert g.energy(cb) == 2 * n
+        assert g.colour()[0] == 2
+    assert g.maxcut(up) == 0
+    assert g.char_scale == pytest.approx(0.0)  # mean -1, variance 0 -> max = 0
+
+
+def test_g022_published_state(oracle):
+    fx = json.load(open(os.path.join(GOLD, "g022_published.json")))
+    g = oracle.Graph.read(graph_path("G022"))
+    assert g.n == 2000 and g.m_header == 19990 and g.nnz == 2 * 19990 and g.sparse
+    s = np.array(fx["best_state"], np.uint8)
+    e = g.energy(s)
+    assert e == fx["best_energy"] == -6720
+    assert g.maxcut(s) == fx["best_maxcut"] == 13355
+    assert (19990 - e) // 2 == 13355  # unit weights: cut = (M - E) / 2
+    assert g.char_scale == 1.0
+
+
+def test_g039_weights(oracle):
+    g = oracle.Graph.read(graph_path("G039"))
+    rp, col, w, h = g.csr()
+    assert g.n == 2000 and g.nnz == 2 * 11778
+    assert set(np.unique(w).tolist()) <= {-1, 1}
+    mean = w.mean()
+    var = (w.astype(float) ** 2).mean() - mean ** 2
+    assert g.char_scale == pytest.approx(max(mean, var), rel=1e-12)
+
+
+def test_one_and_zero_based_files_agree(oracle, tmp_path):
+    src = open(graph_path("lattice_2d_10.gset")).read().splitlines()
+    n, m = src[0].split()
+    one = [src[0]] + [" ".join([str(int(a) + 1), str(int(b) + 1), c]) for a, b, c in (l.split() for l in src[1:])]
+    p1 = tmp_path / "lat10.txt"
+    p1.write_text("\n".join(one) + "\n")
+    g0 = oracle.Graph.read(graph_path("lattice_2d_10.gset"))
+    g1 = oracle.Graph.read(str(p1))
+    for a, b in zip(g0.csr(), g1.csr()):
+        assert np.array_equal(a, b)
+
+
+def test_missing_weight_and_ising_linear_terms(oracle, tmp_path):
+    p = tmp_path / "tri.gset"
+    p.write_text("3 3\n0 1\n1 2 -2\n0 2 3\n")
+    g = oracle.Graph.read(str(p))
+    s = np.array([1, 0, 1], np.uint8)  # spins +1 -1 +1
+    assert g.energy(s) == 1 * (1 * -1) + (-2) * (-1 * 1) + 3 * (1 * 1)
+    q = tmp_path / "f.ising"
+    q.write_text("3 2 1\n0 5\n2 -1\n0 1 2\n")
+    g2 = oracle.Graph.read(str(q))
+    _, _, _, h = g2.csr()
+    assert list(h) == [5, 0, -1]
+    assert g2.energy(s) == 5 * 1 + (-1) * 1 + 2 * (1 * -1)
+
+
+def test_dense_duplicate_overwrite_vs_sparse_keep(oracle):
+    # dense: last weight wins; sparse: both entries count
+    u = np.array([0, 0, 1], np.int32)
+    v = np.array([1, 1, 2], np.int32)
+    w = np.array([2.0, 5.0, 1.0])
+    s = np.array([1, 1, 1], np.uint8)
+    gd = oracle.Graph.from_coo(3, u, v, w, dense=True)
+    gs = oracle.Graph.from_coo(3, u, v, w, dense=False)
+    assert gd.energy(s) == 5 + 1
+    assert gs.energy(s) == 2 + 5 + 1
+
+
+# ------------------------------------------------------ energy / dE ---------
+def _brute_energy(n, u, v, w, h, s):
+    sg = 2 * s.astype(np.int64) - 1
+    return int((h * sg).sum() + (w * sg[u] * sg[v]).sum())
+
+
+def test_energy_brute_force_random(oracle):
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = int(rng.integers(2, 30))
+        m = int(rng.integers(1, 60))
+        u = rng.integers(0, n, m).astype(np.int32)
+        v = rng.integers(0, n, m).astype(np.int32)
+        keep = u != v
+        u, v = u[keep], v[keep]
+        w = rng.integers(-3, 4, len(u)).astype(np.float64)
+        h = rng.integers(-2, 3, n).astype(np.float64)
+        g = oracle.Graph.from_coo(n, u, v, w, h)
+        for _ in range(5):
+            s = rng.integers(0, 2, n).astype(np.uint8)
+            assert g.energy(s) == _brute_energy(n, u, v, w.astype(np.int64), h.astype(np.int64), s)
+            cut = sum(int(c) for a, b, c in zip(u, v, w) if s[a] != s[b])
+            assert g.maxcut(s) == cut
+
+
+@pytest.mark.parametrize("name", ["lattice_2d_5.gset", "lattice_2d_10.gset", "G039"])
+def test_delta_e_invariant(oracle, name):
+    """src/problem.cc:64-100: dE_i = E(flip i) - E, and the incremental update
+    keeps every dE_j exact after accepting i."""
+    g = oracle.Graph.read(graph_path(name))
+    rng = np.random.default_rng(7)
+    s = rng.integers(0, 2, g.n).astype(np.uint8)
+    e0 = g.energy(s)
+    dE = g.delta_e(s)
+    idx = range(g.n) if g.n <= 100 else rng.integers(0, g.n, 60)
+    for i in idx:
+        s[i] ^= 1
+        assert g.energy(s) - e0 == dE[i]
+        s[i] ^= 1
+    for i in list(idx)[:20]:
+        d2 = dE.copy()
+        g.incremental_de(d2, s, i)
+        s2 = s.copy()
+        s2[i] ^= 1
+        assert np.array_equal(d2, g.delta_e(s2))
+
+
+def test_colouring_proper(oracle):
+    for name in ["lattice_2d_5.gset", "lattice_2d_20.gset", "G022", "G039"]:
+        g = oracle.Graph.read(graph_path(name))
+        k, c = g.colour()
+        rp, col, w, h = g.csr()
+        for i in range(g.n):
+            assert not np.any(c[col[rp[i]:rp[i + 1]]] == c[i])
+        p2o, cs = g.positions()
+        assert sorted(p2o.tolist()) == list(range(g.n))
+        assert cs[0] == 0 and cs[-1] == g.n
+        for cl in range(k):
+            mem = p2o[cs[cl]:cs[cl + 1]]
+            assert np.all(c[mem] == cl) and np.all(np.diff(mem) > 0)
+
+
+# -------------------------------------------------------------- ladders -----
+def test_beta_linspace_published(oracle):
+    fx = json.load(open(os.path.join(GOLD, "g022_published.json")))
+    b = oracle.beta_linspace(fx["T0"], fx["T1"], fx["replicas"], fx["inverse"])
+    assert np.allclose(b, fx["betas"], atol=5e-7)  # printed with 6 decimals
+    # C1: 32 temperatures T 0.5..3.0 -> beta linear in [1/3, 2], ascending
+    c1 = oracle.beta_linspace(0.5, 3.0, 32, 0)
+    assert c1[0] == pytest.approx(1 / 3) and c1[-1] == pytest.approx(2.0) and np.all(np.diff(c1) > 0)
+    inv = oracle.beta_linspace(0.2, 3.0, 32, 1)
+    assert inv[0] == 0.2 and inv[-1] == pytest.approx(3.0)
+
+
+def test_icm_count_rule(oracle):
+    b = np.linspace(0.2, 3.0, 29)  # ascending
+    assert oracle.icm_count(b, 1.0) == int((b >= 1.0).sum())
+    assert oracle.icm_count(b[::-1].copy(), 1.0) == 0  # descending ladder: hottest last -> all or nothing
+    assert oracle.icm_count(b[::-1].copy(), 0.1) == 29
+
+
+def test_threshold_table(oracle):
+    b = np.array([0.3, 1.0, 2.5])
+    t = oracle.threshold_table(b, 6)
+    for i, beta in enumerate(b):
+        for m in range(1, 7):
+            p = math.exp(-beta * 2 * m) * 2 ** 32
+            assert abs(int(t[i, m - 1]) - p) <= 1.0
+
+
+# ---------------------------------------------------------- Metropolis ------
+def _tiny_instance(oracle, seed, n=7, field=True):
+    rng = np.random.default_rng(seed)
+    u, v = np.triu_indices(n, 1)
+    keep = rng.random(len(u)) < 0.5
+    u, v = u[keep].astype(np.int32), v[keep].astype(np.int32)
+    w = rng.integers(-2, 3, len(u)).astype(np.float64)
+    h = rng.integers(-1, 2, n).astype(np.float64) if field else np.zeros(n)
+    return oracle.Graph.from_coo(n, u, v, w, h), (u, v, w, h)
+
+
+@pytest.mark.parametrize("flags", [0, 2])  # colour order, reference order
+def test_detailed_balance_exact_enumeration(oracle, flags):
+    g, (u, v, w, h) = _tiny_instance(oracle, 11)
+    n = g.n
+    beta = 0.3  # barriers must be crossed often for 25k samples to converge
+    nrep = 64
+    eng = oracle.Engine(g, np.full(nrep, beta), chains=1, seed=5, flags=flags | oracle.FLAG_NO_EXCHANGE)
+    states = np.array([[(x >> i) & 1 for i in range(n)] for x in range(2 ** n)], np.uint8)
+    energies = np.array([_brute_energy(n, u, v, w.astype(np.int64), h.astype(np.int64), s) for s in states])
+    p = np.exp(-beta * (energies - energies.min()))
+    p /= p.sum()
+    counts = np.zeros(2 ** n)
+    eng.step(50)
+    pw = 1 << np.arange(n)
+    # every sweep is counted: a spin with zero local field flips at every
+    # sweep (dE <= 0 is always accepted), so even-step samples alone are biased
+    for _ in range(800):
+        eng.step(1)
+        s = eng.spins()
+        counts += np.bincount((s.astype(np.int64) * pw).sum(1), minlength=2 ** n)
+    exp = p * counts.sum()
+    mask = exp > 20
+    chi2 = (((counts - exp) ** 2 / exp)[mask]).sum()
+    dof = mask.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof), (chi2, dof)
+    # energies the engine tracks match the recomputed energies
+    assert np.array_equal(eng.energies(), np.array([g.energy(x) for x in eng.spins()]))
+
+
+def test_ferromagnet_energy_matches_table(oracle):
+    """2D ferromagnet L=20: per-spin energy against the tabulated u(T) of
+    data/lattice_2d_20_u.csv (the reference's own validation data)."""
+    import csv
+    tab = {round(float(r["T"]), 2): float(r["u"]) for r in csv.DictReader(open(os.path.join(GOLD, "data",
+                                                                                        "lattice_2d_20_u.csv")))}
+    g = oracle.Graph.read(graph_path("lattice_2d_20.gset"))
+    temps = [1.5, 2.0, 2.5, 3.5]
+    eng = oracle.Engine(g, 1.0 / np.array(temps), chains=2, seed=3, flags=0)
+    eng.step(500)
+    acc = np.zeros(len(temps) * 2)
+    k = 0
+    for _ in range(1500):
+        eng.step(1)
+        e = eng.energies()
+        acc += e[eng.slot2rep()]
+        k += 1
+    u = (acc / k) / 400
+    for i, T in enumerate(temps):
+        for c in range(2):
+            assert abs(u[c * len(temps) + i] - tab[T]) < 0.03, (T, u[c * len(temps) + i], tab[T])
+
+
+# ------------------------------------------------------------ exchange ------
+def test_exchange_two_state_closed_form(oracle):
+    """With sweeps off and nt = 2, each chain alternates between an ordered
+    pair (accepted with p = e^{dB dE} < 1) and the swapped pair (accepted with
+    probability 1): the long-run success fraction is 2p / (1 + p)."""
+    g, _ = _tiny_instance(oracle, 2, n=6, field=False)
+    betas = np.array([0.4, 0.9])
+    eng = oracle.Engine(g, betas, chains=2, seed=9, sweeps=0)
+    s = np.zeros((4, 6), np.uint8)
+    s[0] = [1, 0, 1, 1, 0, 0]
+    s[1] = [0, 0, 0, 0, 0, 0]
+    s[2] = s[0]
+    s[3] = s[1]
+    eng.set_spins(s)
+    e = eng.energies()
+    assert e[0] != e[1]
+    steps = 20000
+    eng.step(steps)
+    att, succ, nsw = eng.swap_stats()
+    x = (betas[0] - betas[1]) * (e[0] - e[1])
+    p = min(1.0, math.exp(x)) if x < 0 else math.exp(-x)
+    want = 2 * p / (1 + p)
+    for k in (0, 2):
+        assert att[k] == steps
+        assert abs(succ[k] / steps - want) < 0.02
+    assert nsw == succ.sum()
+    assert sorted(eng.slot2rep().tolist()) == [0, 1, 2, 3]
+
+
+def test_exchange_quirk_order(oracle):
+    """Chain 1 reuses chain 0's order (replica_exchange.cc:11-15): per step
+    each chain makes nt-1 attempts, and over many steps every pair is tried."""
+    g = oracle.Graph.read(graph_path("lattice_2d_10.gset"))
+    nt = 9
+    eng = oracle.Engine(g, oracle.beta_linspace(0.5, 3.0, nt, 0), chains=2, seed=1)
+    eng.step(300)
+    att, succ, _ = eng.swap_stats()
+    assert att[: nt - 1].sum() == 300 * (nt - 1)
+    assert att[nt: 2 * nt - 1].sum() == 300 * (nt - 1)
+    assert att[nt - 1] == 0 and att[2 * nt - 1] == 0
+    assert np.all(att[: nt - 1] > 0)
+    s2r = eng.slot2rep()
+    assert sorted(s2r.tolist()) == list(range(2 * nt))
+    assert set(s2r[:nt].tolist()) == set(range(nt))  # replicas never change chain
+
+
+# ----------------------------------------------------------------- ICM ------
+def _components(n, rp, col, mask):
+    comp = -np.ones(n, np.int64)
+    c = 0
+    for s in range(n):
+        if not mask[s] or comp[s] >= 0:
+            continue
+        stack = [s]
+        comp[s] = c
+        while stack:
+            i = stack.pop()
+            for j in col[rp[i]:rp[i + 1]]:
+                if mask[j] and comp[j] < 0:
+                    comp[j] = c
+                    stack.append(j)
+        c += 1
+    return comp
+
+
+@pytest.mark.parametrize("name", ["lattice_2d_20.gset", "G039"])
+def test_icm_energy_bookkeeping(oracle, name):
+    """Full steps with ICM keep every tracked energy equal to the recomputed one."""
+    g = oracle.Graph.read(graph_path(name))
+    nt = 6
+    eng = oracle.Engine(g, np.linspace(1.0, 2.0, nt), chains=2, seed=4, flags=oracle.FLAG_ICM, nc=nt)
+    for it in range(8):
+        eng.step(1)
+        sp = eng.spins()
+        assert np.array_equal(eng.energies(), np.array([g.energy(x) for x in sp]))
+        seeds, sizes, gflip = eng.icm_trace()
+        assert np.all(sizes <= g.n // 2)
+
+
+def test_icm_move_properties(oracle):
+    """ICM alone (sweeps off, exchange off): E1+E2 conserved with h = 0; the
+    flipped set is exactly the D-component of the seed; nothing else moves."""
+    g = oracle.Graph.read(graph_path("G039"))
+    rp, col, w, h = g.csr()
+    nt = 4
+    rng = np.random.default_rng(0)
+    eng = oracle.Engine(g, np.linspace(1.0, 2.0, nt), chains=2, seed=8, sweeps=0,
+                        flags=oracle.FLAG_ICM | oracle.FLAG_NO_EXCHANGE, nc=nt)
+    for trial in range(8):
+        s = rng.integers(0, 2, (2 * nt, g.n)).astype(np.uint8)
+        if trial % 2:  # mostly-equal replicas to keep D below n/2 on half the trials
+            s[nt:] = s[:nt] ^ (rng.random((nt, g.n)) < 0.2)
+        eng.set_spins(s)
+        e0 = eng.energies()
+        eng.step(1)
+        after = eng.spins()
+        e1 = eng.energies()
+        seeds, sizes, gflip = eng.icm_trace()
+        p2o, cs = g.positions()
+        for t in range(nt):
+            a, b = t, t + nt
+            assert e1[a] + e1[b] == e0[a] + e0[b]
+            s1 = s[a] ^ (1 if gflip[t] else 0)
+            d = s1 != s[b]
+            assert gflip[t] == int((s[a] != s[b]).sum() > g.n // 2)
+            if not d.any():
+                continue
+            seed_orig = p2o[seeds[t]]
+            assert d[seed_orig]
+            comp = _components(g.n, rp, col, d)
+            cluster = comp == comp[seed_orig]
+            assert cluster.sum() == sizes[t]
+            assert np.array_equal(after[a], s1 ^ cluster)
+            assert np.array_equal(after[b], s[b] ^ cluster)
+
+
+# ----------------------------------------------------- DOS and ladder -------
+def _torus_dos(L):
+    """Exact density of states of the periodic L x L ferromagnet (w = -1) by enumeration."""
+    n = L * L
+    idx = np.arange(n).reshape(L, L)
+    bonds = np.concatenate([np.stack([idx.ravel(), np.roll(idx, -1, 1).ravel()], 1),
+                            np.stack([idx.ravel(), np.roll(idx, -1, 0).ravel()], 1)])
+    x = np.arange(2 ** n, dtype=np.int64)
+    s = ((x[:, None] >> np.arange(n)) & 1) * 2 - 1
+    e = -(s[:, bonds[:, 0]] * s[:, bonds[:, 1]]).sum(1)
+    vals, cnt = np.unique(e, return_counts=True)
+    return vals, cnt
+
+
+def test_acceptance_rate_direct_sum(oracle):
+    vals, cnt = _torus_dos(4)
+    emin = vals.min()
+    estep = 2.0
+    nb = int((vals.max() - emin) // 2) + 1
+    logdos = np.full(nb, -np.finfo(float).max)
+    logdos[((vals - emin) // 2).astype(int)] = np.log(cnt)
+    b1, b2 = 0.6, 0.45
+    z1 = oracle.partition_func(logdos, b1, emin, estep)
+    z2 = oracle.partition_func(logdos, b2, emin, estep)
+    e = emin + estep * np.arange(nb)
+    p1 = logdos - e * b1 - z1
+    p2 = logdos - e * b2 - z2
+    r = oracle.acceptance_rate(logdos, p1, p2, b1, b2, estep, emin)
+    P1 = np.exp(p1)
+    P2 = np.exp(p2)
+    direct = sum(P1[i] * P2[j] * min(1.0, math.exp((b1 - b2) * (e[i] - e[j]))) for i in range(nb) for j in range(nb))
+    assert r == pytest.approx(direct, rel=1e-10)
+    # partition function equals the exact sum
+    assert z1 == pytest.approx(math.log((cnt * np.exp(-b1 * vals)).sum()), rel=1e-12)
+
+
+def test_wham_recovers_exact_dos(oracle):
+    """Histograms sampled from the exact Boltzmann distributions of the 4x4
+    torus: the multihistogram DOS equals log g(E) up to a constant."""
+    vals, cnt = _torus_dos(4)
+    rng = np.random.default_rng(2)
+    nt, ns = 6, 200000
+    hb, bs, be = oracle.hist_betas(1.0, 3.0, nt, 0)
+    logs = np.zeros((ns, nt), np.int32)
+    for j, b in enumerate(hb):
+        p = cnt * np.exp(-b * (vals - vals.min()))
+        p /= p.sum()
+        logs[:, j] = rng.choice(vals, size=ns, p=p)
+    betas, logdos, logz = oracle.get_beta_from_logs(logs, hb, 2.0, 0.1, 0.4, bs, be)
+    emin = logs.min()
+    e = emin + 2 * np.arange(len(logdos))
+    ok = logdos > -1e300
+    exact = dict(zip(vals.tolist(), np.log(cnt).tolist()))
+    diff = np.array([logdos[i] - exact[int(e[i])] for i in np.nonzero(ok)[0] if int(e[i]) in exact])
+    # the well-sampled low-energy bins agree up to a constant
+    assert np.std(diff[:6]) < 0.05
+    assert len(betas) >= 2 and np.all(np.diff(betas) < 0)
