@@ -262,9 +262,9 @@ ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out) {
     const ptx::Engine& E = e->e;
     out->n_instances = E.ninst;
     out->steps_done = E.steps_done;
-    out->mode = PTX_MODE_SPLIT;
-    out->waves_per_group = E.W;
-    out->steps_per_launch = 1;
+    out->mode = E.mode;
+    out->waves_per_group = E.mode == PTX_MODE_FUSED ? E.Wf : E.W;
+    out->steps_per_launch = E.mode == PTX_MODE_FUSED ? E.K : 1;
     out->total_replicas = E.total_rep;
     out->spin_updates_per_step = E.updates_per_step;
     out->lds_bytes_per_step = E.lds_bytes_per_step;

@@ -84,6 +84,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     ladders.resize(ninst);
     std::vector<int32_t> rowptr, cls, hv, p2o;
     std::vector<uint32_t> adj, thr;
+    std::vector<uint16_t> adj16;
+    bool unit_ok = true;
     std::vector<double> beta;
     int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0, adj_total = 0;
     const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
@@ -117,6 +119,9 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             for (int32_t q = P.rowptr[o]; q < P.rowptr[o + 1]; ++q) {
                 const uint32_t pos = (uint32_t)P.orig2pos[P.col[q]];
                 adj.push_back(pos | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
+                const bool unit = (P.w[q] == 1 || P.w[q] == -1) && pos < 32768u;
+                unit_ok = unit_ok && unit;
+                adj16.push_back((uint16_t)(pos | (P.w[q] < 0 ? 0x8000u : 0u)));
                 ++adj_total;
             }
         }
@@ -177,8 +182,37 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     for (int32_t i = 0; i < ninst; ++i)
         for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
 
+    // ---- execution mode ----
+    fmt = unit_ok ? kFmtUnit : kFmtWide;
+    mode = PTX_MODE_SPLIT;
+    if (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO) {
+        Wf = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
+        if (Wf != 4 && Wf != 8 && Wf != 16) Wf = Wf < 6 ? 4 : (Wf < 12 ? 8 : 16);
+        int lds_max = 0;
+        for (int32_t i = 0; i < ninst; ++i) {
+            const Problem& P = *problems[i];
+            const FusedLayout L = fused_layout(P.n, (int)P.nnz(), desc[i].nrep, desc[i].nt, desc[i].fmax, Wf, fmt);
+            lds_max = std::max(lds_max, L.total);
+        }
+        const int lds_limit = 160 * 1024;
+        if (lds_max <= lds_limit) {
+            mode = PTX_MODE_FUSED;
+            fused_lds = lds_max;
+        } else if (cfg.mode == PTX_MODE_FUSED) {
+            return Status::err(PTX_ERR_UNSUPPORTED, "instance does not fit the fused kernel's LDS (" +
+                                                        std::to_string(lds_max) + " bytes); use split mode");
+        }
+        K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 16;
+    }
+    std::vector<int32_t> fused_order(order.begin(), order.end());
+
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
+    if (mode == PTX_MODE_FUSED) {
+        if (fmt == kFmtUnit) PTX_TRY(d_adj16.upload(adj16, stream));
+        PTX_TRY(d_order.upload(fused_order, stream));
+    }
+    PTX_TRY(d_inst_step.alloc((size_t)ninst));
     PTX_TRY(d_rowptr.upload(rowptr, stream));
     PTX_TRY(d_adj.upload(adj, stream));
     PTX_TRY(d_cls.upload(cls, stream));
@@ -229,6 +263,8 @@ static DevArgs build_args(Engine& e) {
     a.hist = e.d_hist.p;
     a.items = e.d_items.p;
     a.pairs = e.d_pairs.p;
+    a.order = e.d_order.p;
+    a.inst_step = e.d_inst_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
     a.sweeps = e.cfg.sweeps;
@@ -285,7 +321,7 @@ Status Engine::launch_step(uint32_t s) {
         }));
     }
     {
-        const size_t lds = (size_t)nrep_max * 40 + 64;
+        const size_t lds = (size_t)nrep_max * 48 + 64;
         PTX_TRY(timed(1, [&] {
             hipLaunchKernelGGL(k_exchange, dim3((unsigned)ninst), dim3(kWave), lds, stream, a, s, nrep_max);
         }));
@@ -299,8 +335,57 @@ Status Engine::launch_step(uint32_t s) {
     return Status::ok();
 }
 
+Status Engine::launch_fused(int nsteps) {
+    DevArgs a = build_args(*this);
+    const void* adjp = fmt == kFmtUnit ? (const void*)d_adj16.p : (const void*)d_adj.p;
+    const dim3 grid((unsigned)ninst), block((unsigned)(Wf * kWave));
+    const size_t lds = (size_t)fused_lds;
+    auto go = [&](auto kern) -> Status {
+        if (lds > 64 * 1024)
+            PTX_TRY(hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                              "hipFuncSetAttribute"));
+        Timed t;
+        if (timing) {
+            t.kind = 3;
+            PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
+            PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
+            PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
+        }
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, a, adjp, nsteps);
+        PTX_TRY(hip_check(hipGetLastError(), "k_fused launch"));
+        if (timing) {
+            PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
+            pending.push_back(t);
+        }
+        return Status::ok();
+    };
+    if (fmt == kFmtUnit) {
+        switch (Wf) {
+            case 4: return go(k_fused<kFmtUnit, 4>);
+            case 8: return go(k_fused<kFmtUnit, 8>);
+            default: return go(k_fused<kFmtUnit, 16>);
+        }
+    }
+    switch (Wf) {
+        case 4: return go(k_fused<kFmtWide, 4>);
+        case 8: return go(k_fused<kFmtWide, 8>);
+        default: return go(k_fused<kFmtWide, 16>);
+    }
+}
+
 Status Engine::step(int64_t n) {
     if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
+    if (steps_done + n > 0xFFFFFFFFll) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^32 PT steps");
+    if (mode == PTX_MODE_FUSED) {
+        while (n > 0) {
+            const int k = (int)std::min<int64_t>(n, K);
+            PTX_TRY(launch_fused(k));
+            steps_done += k;
+            n -= k;
+        }
+        return Status::ok();
+    }
     for (int64_t i = 0; i < n; ++i) {
         PTX_TRY(launch_step((uint32_t)steps_done));
         ++steps_done;
@@ -471,6 +556,7 @@ Status selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint
 }
 
 template struct DevBuf<uint32_t>;
+template struct DevBuf<uint16_t>;
 template struct DevBuf<int32_t>;
 template struct DevBuf<double>;
 

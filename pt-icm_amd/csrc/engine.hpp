@@ -42,7 +42,12 @@ struct Engine {
     std::vector<std::vector<double>> ladders;  // per instance, nt entries
     int64_t total_rep = 0;
     int64_t steps_done = 0;
-    int32_t W = 1;               // waves per sweep workgroup
+    int32_t W = 1;               // waves per sweep workgroup (split mode)
+    int32_t mode = PTX_MODE_SPLIT;
+    int32_t fmt = 0;             // adjacency record format of the fused kernel
+    int32_t Wf = 8;              // waves per workgroup (fused mode)
+    int32_t K = 16;              // PT steps per fused launch
+    int32_t fused_lds = 0;       // bytes of dynamic LDS per fused workgroup
     int32_t nrep_max = 0, nw_max = 0, n_max = 0, fmax_max = 1;
     int32_t spin_stride = 0, thr_stride = 1;
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;
@@ -66,6 +71,9 @@ struct Engine {
     DevBuf<unsigned long long> d_flips, d_swaps;
     DevBuf<uint32_t> d_att, d_succ, d_hist;
     DevBuf<int2> d_items, d_pairs;
+    DevBuf<uint16_t> d_adj16;
+    DevBuf<int32_t> d_order;
+    DevBuf<uint32_t> d_inst_step;
 
     ~Engine();
     Status create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -86,6 +94,7 @@ struct Engine {
   private:
     Status launch_init(int mode);
     Status launch_step(uint32_t s);
+    Status launch_fused(int nsteps);
     Status collect_timing();
 };
 

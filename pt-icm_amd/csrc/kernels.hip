@@ -1,12 +1,15 @@
-// kernels.hip -- the gfx950 kernels of one PT step (split form) plus state
-// initialisation and device self-tests.
+// kernels.hip -- the gfx950 kernels.
 //
-//   k_init      random Bernoulli(1/2) states (initialization.cc:63-76), energies
+// Split form (one launch per phase of a PT step; any instance size):
+//   k_init      Bernoulli(1/2) initial states (initialization.cc:63-76) and energies
 //   k_sweep     Metropolis sweeps, one wavefront per replica (replica.cc:5-38)
-//   k_exchange  histogram of slot energies + (NN)_a exchange as a temperature
-//               permutation, one wavefront per instance (replica_exchange.cc:5-100,
-//               cae_estimator.cc:122-139 in pilot mode)
+//   k_exchange  slot-energy histogram + (NN)_a exchange, one wavefront per
+//               instance (replica_exchange.cc:5-100; cae_estimator.cc:122-139
+//               in pilot mode)
 //   k_icm       Houdayer cluster move, one wavefront per slot pair (icm.cc:8-215)
+// Fused form (batched instances): k_fused runs whole PT steps -- sweeps of
+// every replica, the exchange and the cluster moves -- for one instance per
+// workgroup with the adjacency staged in LDS, several steps per launch.
 #include "kernels.hpp"
 
 namespace ptx {
@@ -32,7 +35,8 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
             uint8_t b = 0;
             if (p < d.n) {
                 const uint32_t i = (uint32_t)a.p2o[d.p2o_off + p];
-                const Philox4 x = philox4x32_10(i >> 2, (uint32_t)r, 0u, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
+                const Philox4 x =
+                    philox4x32_10(i >> 2, (uint32_t)r, 0u, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
                 b = (uint8_t)(pick(x, i) >> 31);
             }
             sp[p] = b;
@@ -44,7 +48,8 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
         expand_bits(words, d.nw, sp, lane);
         wave_sync();
     }
-    const int32_t E = energy_bytes(a, d, sp, lane);
+    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
+    const int32_t E = energy_bytes(csr, d.n, a.hfield + d.h_off, d.has_field != 0, sp, lane);
     if (lane == 0) {
         a.E[gr] = E;
         if (mode == 0) {
@@ -85,9 +90,12 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
     wave_sync();
+    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
+    const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     for (int sw = 0; sw < a.sweeps; ++sw) {
         const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
-        sweep_replica(a, d, r, gsweep, sp, thr_row, best_words, st, lane);
+        sweep_replica(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.h_off, d.has_field != 0, d.nw, c3, a.seed, r,
+                      gsweep, sp, thr_row, best_words, st, lane);
     }
     pack_bits(sp, d.nw, words, lane);
     if (lane == 0) {
@@ -98,26 +106,36 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
     }
 }
 
+// Histogram of the slot energies after the sweeps (the point where
+// gather_samples logs them, cae_estimator.cc:117-121): energy = e_base + 2*bin.
+__device__ __forceinline__ void record_histogram(const DevArgs& a, const InstDesc& d, uint32_t step, const int32_t* E,
+                                                 const int32_t* s2r, int lane) {
+    if (!a.hist_on || (int64_t)step < a.hist_start) return;
+    for (int s = lane; s < d.nrep; s += kWave) {
+        const int bin = (E[s2r[s]] - d.e_base) >> 1;
+        atomicAdd(&a.hist[d.hist_off + (int64_t)s * d.hist_bins + bin], 1u);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// k_exchange: one wavefront per instance.  Lane-parallel: histogram update,
-// pair orders (random sort keys), log-uniforms of every attempt.  Lane 0 then
-// walks the attempts in order; each reads the permutation the previous ones
-// left (replica_exchange.cc:11-14,74).
+// k_exchange: one wavefront per instance (LDS: 48 bytes per slot).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, int nrep_max) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const InstDesc d = a.desc[blockIdx.x];
-    const int nrep = d.nrep, nt = d.nt, np = nt - 1;
-    // LDS: 40 bytes per slot (2(nt-1) attempts and two orders fit in nrep slots)
+    const int nrep = d.nrep;
     double* beta = reinterpret_cast<double*>(smem);
-    double* logu = beta + nrep_max;
-    int32_t* E = reinterpret_cast<int32_t*>(logu + nrep_max);
+    ExchScratch x;
+    x.logu = beta + nrep_max;
+    int32_t* E = reinterpret_cast<int32_t*>(x.logu + nrep_max);
     int32_t* s2r = E + nrep_max;
     uint32_t* att = reinterpret_cast<uint32_t*>(s2r + nrep_max);
     uint32_t* succ = att + nrep_max;
-    uint32_t* key = succ + nrep_max;
-    int32_t* perm = reinterpret_cast<int32_t*>(key + nrep_max);  // [2][np]
+    x.key = succ + nrep_max;
+    x.perm = reinterpret_cast<int32_t*>(x.key + nrep_max);
+    x.tpos = x.perm + nrep_max;
+    x.level = x.tpos + nrep_max;
     const int64_t ro = d.rep_off;
     for (int s = lane; s < nrep; s += kWave) {
         beta[s] = a.beta[d.beta_off + s];
@@ -126,60 +144,14 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, in
         att[s] = 0;
         succ[s] = 0;
     }
-    __syncthreads();
-    if (a.hist_on && (int64_t)step >= a.hist_start) {
-        for (int s = lane; s < nrep; s += kWave) {
-            const int e = E[s2r[s]];
-            const int bin = (e - d.e_base) >> 1;
-            atomicAdd(&a.hist[d.hist_off + (int64_t)s * d.hist_bins + bin], 1u);
-        }
-    }
-    if (a.no_exchange || np <= 0) return;
-    const int nhalf = a.pilot ? 1 : 2;
-    const int nattempt = np * nhalf;
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    const uint32_t c3s = ctr3(kTagShuf | a.tag_base, d.inst_id);
-    for (int h = 0; h < nhalf; ++h) {
-        for (int j = lane; j < np; j += kWave) {
-            const Philox4 x = philox4x32_10((uint32_t)j >> 2, (uint32_t)h, step, c3s, k0, k1);
-            key[j] = pick(x, (uint32_t)j);
-        }
-        __syncthreads();
-        for (int j = lane; j < np; j += kWave) {
-            const uint32_t kj = key[j];
-            int rank = 0;
-            for (int i = 0; i < np; ++i) {
-                const uint32_t ki = key[i];
-                rank += (ki < kj) || (ki == kj && i < j);
-            }
-            perm[h * np + rank] = j;
-        }
-        __syncthreads();
-    }
-    const uint32_t c3e = ctr3(kTagExch | a.tag_base, d.inst_id);
-    for (int t = lane; t < nattempt; t += kWave) {
-        const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, c3e, k0, k1);
-        logu[t] = uniform_log(pick(x, (uint32_t)t));
-    }
-    __syncthreads();
-    if (lane == 0) {
-        unsigned long long nsw = 0;
-        for (int t = 0; t < nattempt; ++t) {
-            const int chain = a.pilot ? 0 : (t & 1);
-            const int k = perm[(t / np) * np + (t % np)] + chain * nt;
-            att[k] += 1u;
-            const int r1 = s2r[k], r2 = s2r[k + 1];
-            const double lb = (beta[k] - beta[k + 1]) * (double)(E[r1] - E[r2]);
-            if (logu[t] < lb) {
-                s2r[k] = r2;
-                s2r[k + 1] = r1;
-                succ[k] += 1u;
-                ++nsw;
-            }
-        }
-        a.swaps[blockIdx.x] += nsw;
-    }
-    __syncthreads();
+    wave_sync();
+    record_histogram(a, d, step, E, s2r, lane);
+    if (a.no_exchange) return;
+    const unsigned long long nsw = exchange_wave(d.nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta,
+                                                 E, s2r, att, succ, x, lane);
+    const unsigned long long total = (unsigned long long)wave_sum_i64((long long)nsw);
+    if (lane == 0) a.swaps[blockIdx.x] += total;
+    wave_sync();
     for (int s = lane; s < nrep; s += kWave) {
         a.slot2rep[ro + s] = s2r[s];
         a.rep2slot[ro + s2r[s]] = s;
@@ -190,10 +162,6 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, in
 
 // ---------------------------------------------------------------------------
 // k_icm: one wavefront per (instance, slot t) pair, t in [nt-nc, nt).
-// LDS: both states, the disagreement set D, the visited set V (bit words)
-// and a queue of up to n/2 positions.  The cluster is grown breadth-first
-// from a uniform seed in D; the reference grows it depth-first
-// (icm.cc:37-72), which gives the same component.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_max) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -201,153 +169,219 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
     const int2 pr = a.pairs[blockIdx.x];
     const InstDesc d = a.desc[pr.x];
     const int t = pr.y;
-    const int n = d.n, nw = d.nw;
-    uint32_t* S1 = reinterpret_cast<uint32_t*>(smem);
-    uint32_t* S2 = S1 + nw_max;
-    uint32_t* D = S2 + nw_max;
-    uint32_t* V = D + nw_max;
-    int32_t* tail = reinterpret_cast<int32_t*>(V + nw_max);
-    uint16_t* Q = reinterpret_cast<uint16_t*>(tail + 4);
+    IcmScratch sc;
+    sc.S1 = reinterpret_cast<uint32_t*>(smem);
+    sc.S2 = sc.S1 + nw_max;
+    sc.D = sc.S2 + nw_max;
+    sc.V = sc.D + nw_max;
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw_max);
+    sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
     const int64_t ro = d.rep_off;
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
-    uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * nw;
-    uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * nw;
-    const uint32_t lastmask = (n & 31) ? ((1u << (n & 31)) - 1u) : 0xFFFFFFFFu;
-    int cnt = 0;
-    for (int w = lane; w < nw; w += kWave) {
-        const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
-        const uint32_t x1 = g1[w] & m, x2 = g2[w] & m;
-        S1[w] = x1;
-        S2[w] = x2;
-        D[w] = x1 ^ x2;
-        V[w] = 0u;
-        cnt += __popc(x1 ^ x2);
-    }
-    cnt = wave_sum_i32(cnt);
-    const int32_t* __restrict__ rp = a.rowptr + d.rowptr_off;
-    const int32_t* __restrict__ hf = a.hfield + d.h_off;
+    uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * d.nw;
+    uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
+    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
     long long dE1 = 0, dE2 = 0;
-    __syncthreads();
-    if (cnt > n / 2) {  // icm.cc:20-28: flip replica 1 globally
-        if (d.has_field) {
-            for (int p = lane; p < n; p += kWave) {
-                const int s1 = (S1[p >> 5] >> (p & 31)) & 1u;
-                dE1 += -2ll * (s1 ? 1 : -1) * hf[p];
-            }
-        }
-        __syncthreads();
-        for (int w = lane; w < nw; w += kWave) {
-            const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
-            S1[w] = ~S1[w] & m;
-            D[w] = ~D[w] & m;
-        }
-        cnt = n - cnt;
-        __syncthreads();
-    }
-    if (cnt > 0) {
-        const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, ctr3(kTagIcm | a.tag_base, d.inst_id),
-                                        (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
-        const int k = (int)(((unsigned long long)pick(x, (uint32_t)t) * (unsigned long long)cnt) >> 32);
-        // k-th element of D in position order: per-64-word chunks, inclusive scan
-        int seed_pos = -1, base = 0;
-        for (int w0 = 0; w0 < nw && seed_pos < 0; w0 += kWave) {
-            const int w = w0 + lane;
-            const uint32_t dw = w < nw ? D[w] : 0u;
-            const int c = __popc(dw);
-            int incl = c;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int y = __shfl_up(incl, o, kWave);
-                if (lane >= o) incl += y;
-            }
-            const int total = __shfl(incl, kWave - 1, kWave);
-            if (k < base + total) {
-                const bool mine = (k < base + incl) && (k >= base + incl - c);
-                const unsigned long long ball = __ballot(mine);
-                const int owner = __ffsll((long long)ball) - 1;
-                int sel = -1;
-                if (lane == owner) {
-                    int kk = k - (base + incl - c);
-                    uint32_t v = dw;
-                    for (int b = 0; b < 32; ++b) {
-                        if ((v >> b) & 1u) {
-                            if (kk == 0) { sel = 32 * w + b; break; }
-                            --kk;
-                        }
-                    }
-                }
-                seed_pos = __shfl(sel, owner, kWave);
-            }
-            base += total;
-        }
-        if (lane == 0) {
-            V[seed_pos >> 5] |= 1u << (seed_pos & 31);
-            Q[0] = (uint16_t)seed_pos;
-            tail[0] = 1;
-        }
-        __syncthreads();
-        int head = 0;
-        while (true) {
-            const int tl = tail[0];
-            __syncthreads();
-            if (head >= tl) break;
-            const int idx = head + lane;
-            if (idx < tl) {
-                const int p = Q[idx];
-                for (int q = rp[p]; q < rp[p + 1]; ++q) {
-                    const int j = (int)(a.adj[q] & 0xFFFFu);
-                    const uint32_t bit = 1u << (j & 31);
-                    if ((D[j >> 5] & bit) && !(V[j >> 5] & bit)) {
-                        const uint32_t old = atomicOr(&V[j >> 5], bit);
-                        if (!(old & bit)) {
-                            const int pos = atomicAdd(&tail[0], 1);
-                            Q[pos] = (uint16_t)j;
-                        }
-                    }
-                }
-            }
-            head = (tl - head > kWave) ? head + kWave : tl;
-            __syncthreads();
-        }
-        const int csize = tail[0];
-        // energy change of flipping the cluster C in each replica: boundary
-        // edges (p in C, j not in C) and the field terms of C
-        for (int idx = lane; idx < csize; idx += kWave) {
-            const int p = Q[idx];
-            const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
-            const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
-            long long b1 = 0, b2 = 0;
-            for (int q = rp[p]; q < rp[p + 1]; ++q) {
-                const uint32_t rec = a.adj[q];
-                const int j = (int)(rec & 0xFFFFu);
-                if ((V[j >> 5] >> (j & 31)) & 1u) continue;
-                const int w = (int32_t)rec >> 16;
-                b1 += (long long)w * (((S1[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
-                b2 += (long long)w * (((S2[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
-            }
-            dE1 += -2ll * s1 * b1;
-            dE2 += -2ll * s2 * b2;
-            if (d.has_field) {
-                dE1 += -2ll * s1 * hf[p];
-                dE2 += -2ll * s2 * hf[p];
-            }
-        }
-        __syncthreads();
-        for (int w = lane; w < nw; w += kWave) {
-            S1[w] ^= V[w];
-            S2[w] ^= V[w];
-        }
-        __syncthreads();
-    }
-    dE1 = wave_sum_i64(dE1);
-    dE2 = wave_sum_i64(dE2);
-    for (int w = lane; w < nw; w += kWave) {
-        g1[w] = S1[w];
-        g2[w] = S2[w];
-    }
+    icm_wave(csr, a.hfield + d.h_off, d.has_field != 0, d.n, d.nw, t, step, a.seed,
+             ctr3(kTagIcm | a.tag_base, d.inst_id), g1, g2, sc, dE1, dE2, lane);
     if (lane == 0) {
         a.E[ro + r1] += (int32_t)dE1;
         a.E[ro + r2] += (int32_t)dE2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused per-instance kernel: one workgroup of W waves owns one instance for
+// nsteps PT steps.  LDS holds the adjacency (16- or 32-bit records), the
+// threshold table, betas, energies, the permutation, the exchange scratch and
+// one scratch area per wave (spin bytes during sweeps, ICM sets during
+// cluster moves).  Spin words stay in global memory (L2-resident).
+// ---------------------------------------------------------------------------
+struct FusedLayout {
+    int rp, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, scratch, scratch_stride, total;
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
+
+__host__ __device__ inline FusedLayout fused_layout(int n, int nnz, int nrep, int nt, int fmax, int W, int fmt) {
+    FusedLayout L;
+    int o = 0;
+    L.rp = o;
+    o += align16(4 * (n + 1));
+    L.adj = o;
+    o += align16((fmt == kFmtUnit ? 2 : 4) * (nnz > 0 ? nnz : 1));
+    L.thr = o;
+    o += align16(4 * nt * (fmax > 0 ? fmax : 1));
+    L.beta = o;
+    o += align16(8 * nrep);
+    L.logu = o;
+    o += align16(8 * nrep);
+    L.E = o;
+    o += align16(4 * nrep);
+    L.s2r = o;
+    o += align16(4 * nrep);
+    L.r2s = o;
+    o += align16(4 * nrep);
+    L.att = o;
+    o += align16(4 * nrep);
+    L.succ = o;
+    o += align16(4 * nrep);
+    L.key = o;
+    o += align16(4 * nrep);
+    L.perm = o;
+    o += align16(4 * nrep);
+    L.tpos = o;
+    o += align16(4 * nrep);
+    L.level = o;
+    o += align16(4 * nrep);
+    const int nw = (n + 31) / 32;
+    const int sweep_bytes = 32 * nw;
+    const int icm_bytes = icm_scratch_bytes(n);
+    L.scratch_stride = align16(sweep_bytes > icm_bytes ? sweep_bytes : icm_bytes);
+    L.scratch = o;
+    o += W * L.scratch_stride;
+    L.total = o;
+    return L;
+}
+
+template <int Fmt, int W>
+__global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, const void* __restrict__ adj_fmt, int nsteps) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tid = threadIdx.x;
+    const int inst = a.order[blockIdx.x];
+    const InstDesc d = a.desc[inst];
+    const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
+    const int32_t* grp = a.rowptr + d.rowptr_off;
+    const int q0 = grp[0];
+    const int nnz = grp[n] - q0;
+    const FusedLayout L = fused_layout(n, nnz, nrep, nt, d.fmax, W, Fmt);
+    int32_t* rp = reinterpret_cast<int32_t*>(smem + L.rp);
+    uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
+    double* beta = reinterpret_cast<double*>(smem + L.beta);
+    int32_t* E = reinterpret_cast<int32_t*>(smem + L.E);
+    int32_t* s2r = reinterpret_cast<int32_t*>(smem + L.s2r);
+    int32_t* r2s = reinterpret_cast<int32_t*>(smem + L.r2s);
+    uint32_t* att = reinterpret_cast<uint32_t*>(smem + L.att);
+    uint32_t* succ = reinterpret_cast<uint32_t*>(smem + L.succ);
+    ExchScratch xs;
+    xs.logu = reinterpret_cast<double*>(smem + L.logu);
+    xs.key = reinterpret_cast<uint32_t*>(smem + L.key);
+    xs.perm = reinterpret_cast<int32_t*>(smem + L.perm);
+    xs.tpos = reinterpret_cast<int32_t*>(smem + L.tpos);
+    xs.level = reinterpret_cast<int32_t*>(smem + L.level);
+    uint8_t* scratch = smem + L.scratch + wave * L.scratch_stride;
+
+    // ---- stage the instance ----
+    for (int p = tid; p <= n; p += W * kWave) rp[p] = grp[p] - q0;
+    if (Fmt == kFmtUnit) {
+        const uint16_t* src = static_cast<const uint16_t*>(adj_fmt) + q0;
+        uint16_t* dst = reinterpret_cast<uint16_t*>(smem + L.adj);
+        for (int q = tid; q < nnz; q += W * kWave) dst[q] = src[q];
+    } else {
+        const uint32_t* src = a.adj + q0;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(smem + L.adj);
+        for (int q = tid; q < nnz; q += W * kWave) dst[q] = src[q];
+    }
+    for (int m = tid; m < nt * d.fmax; m += W * kWave) thr[m] = a.thr[d.thr_off + m];
+    const int64_t ro = d.rep_off;
+    for (int s = tid; s < nrep; s += W * kWave) {
+        beta[s] = a.beta[d.beta_off + s];
+        E[s] = a.E[ro + s];
+        s2r[s] = a.slot2rep[ro + s];
+        r2s[s] = a.rep2slot[ro + s];
+        att[s] = 0;
+        succ[s] = 0;
+    }
+    __syncthreads();
+    const Csr<Fmt> csr{rp, smem + L.adj};
+    const int32_t* hf = a.hfield + d.h_off;
+    const int32_t* cls = a.cls + d.cls_off;
+    const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
+    const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
+    const uint32_t step0 = a.inst_step[inst];
+    unsigned long long nsw_total = 0;
+    IcmScratch sc;
+    sc.S1 = reinterpret_cast<uint32_t*>(scratch);
+    sc.S2 = sc.S1 + nw;
+    sc.D = sc.S2 + nw;
+    sc.V = sc.D + nw;
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw);
+    sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
+
+    for (int it = 0; it < nsteps; ++it) {
+        const uint32_t step = step0 + (uint32_t)it;
+        // ---- sweeps: replica r = wave, wave + W, ... ----
+        for (int r = wave; r < nrep; r += W) {
+            const int64_t gr = ro + r;
+            uint32_t* words = a.spins + d.spin_off + (int64_t)r * nw;
+            uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r * nw;
+            const uint32_t* thr_row = thr + (r2s[r] % nt) * d.fmax;
+            expand_bits(words, nw, scratch, lane);
+            SweepState st;
+            st.E = E[r];
+            st.bestE = a.bestE[gr];
+            st.best_sweep = a.best_sweep[gr];
+            st.flips = 0;
+            wave_sync();
+            for (int sw = 0; sw < a.sweeps; ++sw) {
+                const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
+                sweep_replica(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, scratch, thr_row,
+                              best_words, st, lane);
+            }
+            pack_bits(scratch, nw, words, lane);
+            if (lane == 0) {
+                E[r] = st.E;
+                a.bestE[gr] = st.bestE;
+                a.best_sweep[gr] = st.best_sweep;
+                a.flips[gr] += st.flips;
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        // ---- histogram + exchange (wave 0) ----
+        if (wave == 0) {
+            record_histogram(a, d, step, E, s2r, lane);
+            if (!a.no_exchange) {
+                nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta, E, s2r,
+                                           att, succ, xs, lane);
+                wave_sync();
+                for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
+            }
+        }
+        __syncthreads();
+        // ---- cluster moves: slot t = nt-1-j, j = wave, wave + W, ... < nc ----
+        if (d.nc > 0) {
+            for (int j = wave; j < d.nc; j += W) {
+                const int t = nt - 1 - j;
+                const int r1 = s2r[t], r2 = s2r[t + nt];
+                long long dE1 = 0, dE2 = 0;
+                icm_wave(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                         a.spins + d.spin_off + (int64_t)r1 * nw, a.spins + d.spin_off + (int64_t)r2 * nw, sc, dE1,
+                         dE2, lane);
+                if (lane == 0) {
+                    E[r1] += (int32_t)dE1;
+                    E[r2] += (int32_t)dE2;
+                }
+                wave_sync();
+            }
+            __syncthreads();
+        }
+    }
+    // ---- write back ----
+    for (int s = tid; s < nrep; s += W * kWave) {
+        a.E[ro + s] = E[s];
+        a.slot2rep[ro + s] = s2r[s];
+        a.rep2slot[ro + s] = r2s[s];
+        a.swap_att[ro + s] += att[s];
+        a.swap_succ[ro + s] += succ[s];
+    }
+    if (wave == 0) {
+        const unsigned long long tot = (unsigned long long)wave_sum_i64((long long)nsw_total);
+        if (lane == 0) {
+            a.swaps[inst] += tot;
+            a.inst_step[inst] = step0 + (uint32_t)nsteps;
+        }
     }
 }
 
@@ -382,5 +416,11 @@ template __global__ void k_sweep<1>(DevArgs, uint32_t);
 template __global__ void k_sweep<2>(DevArgs, uint32_t);
 template __global__ void k_sweep<4>(DevArgs, uint32_t);
 template __global__ void k_sweep<8>(DevArgs, uint32_t);
+template __global__ void k_fused<kFmtWide, 4>(DevArgs, const void*, int);
+template __global__ void k_fused<kFmtWide, 8>(DevArgs, const void*, int);
+template __global__ void k_fused<kFmtWide, 16>(DevArgs, const void*, int);
+template __global__ void k_fused<kFmtUnit, 4>(DevArgs, const void*, int);
+template __global__ void k_fused<kFmtUnit, 8>(DevArgs, const void*, int);
+template __global__ void k_fused<kFmtUnit, 16>(DevArgs, const void*, int);
 
 }  // namespace ptx

@@ -1,9 +1,10 @@
 // kernels.hpp -- device-side building blocks shared by the split kernels
-// (one launch per phase) and the fused per-instance kernel.  One wavefront
-// (64 lanes) owns one replica during a sweep; spins of that replica are
-// expanded to one byte per spin in LDS so neighbour reads are single
-// ds_read_u8 and flips are plain byte stores (no atomics: inside a colour
-// class no two updated sites are adjacent).
+// (one launch per phase) and the fused per-instance kernel.
+//
+// A wavefront (64 lanes) owns one replica during a sweep; the replica's spins
+// are expanded to one byte per spin in LDS so a neighbour read is a single
+// ds_read_u8 and a flip a plain byte store (inside a colour class no two
+// updated sites are adjacent, so no atomics are needed).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,11 +15,15 @@ namespace ptx {
 
 constexpr int kWave = 64;
 
+// Adjacency record formats.  Wide: u32 = position | (int16 weight << 16).
+// Unit: u16 = position | (weight < 0) << 15, for |w| == 1 and n <= 32768.
+enum : int { kFmtWide = 0, kFmtUnit = 1 };
+
 // Arguments common to every kernel (device pointers of the engine).
 struct DevArgs {
     const InstDesc* desc;
     const int32_t* rowptr;   // absolute offsets into adj
-    const uint32_t* adj;     // (neighbour position) | (weight << 16)
+    const uint32_t* adj;     // wide records
     const int32_t* cls;      // class_start
     const int32_t* hfield;   // h by position
     const int32_t* p2o;      // pos2orig
@@ -36,12 +41,14 @@ struct DevArgs {
     uint32_t* swap_succ;
     unsigned long long* swaps;  // per instance
     uint32_t* hist;
+    uint32_t* inst_step;     // per-instance PT step counter (fused mode)
     const int2* items;       // (instance, first replica) per sweep workgroup
     const int2* pairs;       // (instance, slot t) per ICM wavefront
+    const int32_t* order;    // fused mode: instance per workgroup
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
     int32_t sweeps;
-    int32_t spin_stride;     // LDS bytes per wave for spin bytes
+    int32_t spin_stride;     // LDS bytes per wave for spin bytes / ICM scratch
     int32_t thr_stride;      // LDS words per wave for the threshold row
     int32_t pilot;
     int32_t no_exchange;
@@ -65,11 +72,67 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
 }
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Adjacency access: rowptr entries are offsets into adj; adj holds records of
+// the chosen format (in LDS for the fused kernel, in global memory otherwise).
+// ---------------------------------------------------------------------------
+template <int Fmt>
+struct Csr {
+    const int32_t* rp;
+    const void* adj;
+    __device__ __forceinline__ int begin(int p) const { return rp[p]; }
+    __device__ __forceinline__ int end(int p) const { return rp[p + 1]; }
+    __device__ __forceinline__ int col(int q) const {
+        if (Fmt == kFmtUnit) return (int)(static_cast<const uint16_t*>(adj)[q] & 0x7FFFu);
+        return (int)(static_cast<const uint32_t*>(adj)[q] & 0xFFFFu);
+    }
+    __device__ __forceinline__ void edge(int q, int& c, int& w) const {
+        if (Fmt == kFmtUnit) {
+            const uint32_t r = static_cast<const uint16_t*>(adj)[q];
+            c = (int)(r & 0x7FFFu);
+            w = (r & 0x8000u) ? -1 : 1;
+        } else {
+            const uint32_t r = static_cast<const uint32_t*>(adj)[q];
+            c = (int)(r & 0xFFFFu);
+            w = (int32_t)r >> 16;
+        }
+    }
+    // sum_j w_pj s_j with s = 2b - 1, from byte spins
+    __device__ __forceinline__ int field(int p, const uint8_t* sp) const {
+        const int q0 = rp[p], q1 = rp[p + 1];
+        int f = 0;
+        if (Fmt == kFmtUnit) {
+            const uint16_t* A = static_cast<const uint16_t*>(adj);
+            int agree = 0;  // number of neighbours with s_j * sign(w) = +1
+            for (int q = q0; q < q1; ++q) {
+                const uint32_t r = A[q];
+                agree += (int)(sp[r & 0x7FFFu] ^ (r >> 15));
+            }
+            f = 2 * agree - (q1 - q0);
+        } else {
+            const uint32_t* A = static_cast<const uint32_t*>(adj);
+            for (int q = q0; q < q1; ++q) {
+                const uint32_t r = A[q];
+                const int w = (int32_t)r >> 16;
+                f += sp[r & 0xFFFFu] ? w : -w;
+            }
+        }
+        return f;
+    }
+};
 
 // Bytes <-> bits.  A state word covers 32 consecutive positions; the byte
-// buffer has nw*32 entries (padding bytes stay 0).
-__device__ __forceinline__ void expand_bits(const uint32_t* __restrict__ words, int nw, uint8_t* sp, int lane) {
-    for (int w = lane; w < nw; w += kWave) {
+// buffer has nw*32 entries (padding bytes stay 0).  lane/stride let several
+// waves share the work.
+__device__ __forceinline__ void expand_bits(const uint32_t* __restrict__ words, int nw, uint8_t* sp, int lane,
+                                            int stride = kWave) {
+    for (int w = lane; w < nw; w += stride) {
         const uint32_t v = words[w];
         uint32_t* dst = reinterpret_cast<uint32_t*>(sp + 32 * w);
 #pragma unroll
@@ -79,8 +142,9 @@ __device__ __forceinline__ void expand_bits(const uint32_t* __restrict__ words, 
         }
     }
 }
-__device__ __forceinline__ void pack_bits(const uint8_t* sp, int nw, uint32_t* __restrict__ words, int lane) {
-    for (int w = lane; w < nw; w += kWave) {
+__device__ __forceinline__ void pack_bits(const uint8_t* sp, int nw, uint32_t* __restrict__ words, int lane,
+                                          int stride = kWave) {
+    for (int w = lane; w < nw; w += stride) {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(sp + 32 * w);
         uint32_t v = 0;
 #pragma unroll
@@ -93,29 +157,27 @@ __device__ __forceinline__ void pack_bits(const uint8_t* sp, int nw, uint32_t* _
     }
 }
 
-// One Metropolis sweep of one replica in colour-class order
-// (src/replica.cc:5-38 restated for class-parallel updates).  Each lane takes
-// aligned groups of 4 positions, draws one Philox block per group (word k for
-// position 4g+k), computes the local fields from the byte spins in LDS and
-// accepts dE <= 0 always, dE > 0 iff draw < thr[dE/2 - 1].  After every class
-// that accepted a flip the replica's best energy is updated
-// (replica.cc:23-27 at class granularity).
+// ---------------------------------------------------------------------------
+// Metropolis sweep of one replica in colour-class order (src/replica.cc:5-38
+// restated for class-parallel updates).  Each lane takes aligned groups of 4
+// positions and draws one Philox block per group (word k for position 4g+k);
+// dE <= 0 is accepted, dE > 0 iff draw < thr[dE/2 - 1].  After every class
+// that accepted a flip the best energy is updated (replica.cc:23-27 at class
+// granularity).
+// ---------------------------------------------------------------------------
 struct SweepState {
     int32_t E, bestE, best_sweep;
     unsigned long long flips;
-    bool best_dirty;
 };
 
-__device__ __forceinline__ void sweep_replica(const DevArgs& a, const InstDesc& d, int r, uint32_t gsweep, uint8_t* sp,
+template <int Fmt>
+__device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
+                                              const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
+                                              uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
                                               const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
                                               int lane) {
-    const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
-    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
-    const int32_t* __restrict__ cls = a.cls + d.cls_off;
-    const int32_t* __restrict__ rp = a.rowptr + d.rowptr_off;
-    const int32_t* __restrict__ hf = a.hfield + d.h_off;
-    const uint32_t* __restrict__ adj = a.adj;
-    for (int c = 0; c < d.ncls; ++c) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int c = 0; c < ncls; ++c) {
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
         for (int g = (cs >> 2) + lane; 4 * g < ce; g += kWave) {
@@ -127,13 +189,7 @@ __device__ __forceinline__ void sweep_replica(const DevArgs& a, const InstDesc& 
                 const int p = base + k;
                 dE[k] = 0;
                 if (p >= cs && p < ce) {
-                    int f = d.has_field ? hf[p] : 0;
-                    const int q0 = rp[p], q1 = rp[p + 1];
-                    for (int q = q0; q < q1; ++q) {
-                        const uint32_t rec = adj[q];
-                        const int w = (int32_t)rec >> 16;
-                        f += sp[rec & 0xFFFFu] ? w : -w;
-                    }
+                    const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
                     dE[k] = sp[p] ? -2 * f : 2 * f;
                     any_up |= dE[k] > 0;
                 }
@@ -161,29 +217,299 @@ __device__ __forceinline__ void sweep_replica(const DevArgs& a, const InstDesc& 
         if (nacc > 0 && st.E < st.bestE) {
             st.bestE = st.E;
             st.best_sweep = (int32_t)gsweep;
-            pack_bits(sp, d.nw, best_words, lane);
+            pack_bits(sp, nw, best_words, lane);
         }
     }
 }
 
-// Energy of the byte state in LDS: E = sum_p h_p s_p + 1/2 sum_p s_p sum_q w s_q
+// Energy of the byte state: E = sum_p h_p s_p + 1/2 sum_p s_p sum_q w s_q
 // (src/problem.cc:121-142).
-__device__ __forceinline__ int32_t energy_bytes(const DevArgs& a, const InstDesc& d, const uint8_t* sp, int lane) {
-    const int32_t* __restrict__ rp = a.rowptr + d.rowptr_off;
-    const int32_t* __restrict__ hf = a.hfield + d.h_off;
+template <int Fmt>
+__device__ __forceinline__ int32_t energy_bytes(const Csr<Fmt>& csr, int n, const int32_t* hf, bool has_field,
+                                                const uint8_t* sp, int lane) {
     long long acc2 = 0;
-    for (int p = lane; p < d.n; p += kWave) {
+    for (int p = lane; p < n; p += kWave) {
         const int sgn = sp[p] ? 1 : -1;
-        int f = 0;
-        for (int q = rp[p]; q < rp[p + 1]; ++q) {
-            const uint32_t rec = a.adj[q];
-            const int w = (int32_t)rec >> 16;
-            f += sp[rec & 0xFFFFu] ? w : -w;
-        }
-        acc2 += (long long)sgn * f + (d.has_field ? 2ll * sgn * hf[p] : 0ll);
+        acc2 += (long long)sgn * csr.field(p, sp) + (has_field ? 2ll * sgn * hf[p] : 0ll);
     }
     acc2 = wave_sum_i64(acc2);
     return (int32_t)(acc2 / 2);
+}
+
+// ---------------------------------------------------------------------------
+// (NN)_a exchange of one instance, executed by one wavefront on LDS state.
+//
+// The reference walks 2(Nt-1) attempts in a fixed order, each reading the
+// permutation the earlier ones left (replica_exchange.cc:5-100 driven by
+// solver.cc:81-89; gather_samples' loop in pilot mode, cae_estimator.cc:
+// 122-139).  An attempt on pair k reads slots k and k+1, so it depends only on
+// the latest earlier attempts on pairs k-1, k and k+1 of the same chain.  The
+// attempts of one half (one pair order) are distinct pairs, so they are
+// grouped into dependency levels and every level runs in parallel across
+// lanes; the result is the sequential one bit for bit.
+//
+// LDS inputs: beta[nrep] (slot betas), E[nrep] (by replica), s2r[nrep];
+// outputs: s2r, att/succ (+=).  Scratch (ExchScratch) sized by nrep.
+// ---------------------------------------------------------------------------
+struct ExchScratch {
+    double* logu;    // [nrep]  log-uniform of attempt a
+    uint32_t* key;   // [nrep]  sort keys of one half
+    int32_t* perm;   // [nrep]  pair order of the current half
+    int32_t* tpos;   // [nrep]  attempt index of pair k in this half, -1 if none
+    int32_t* level;  // [nrep]
+};
+
+__device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bool pilot, uint64_t seed,
+                                                            uint32_t tag_base, uint32_t inst_id, uint32_t step,
+                                                            const double* beta, const int32_t* E, int32_t* s2r,
+                                                            uint32_t* att, uint32_t* succ, ExchScratch x, int lane) {
+    const int np = nt - 1;
+    if (np <= 0) return 0ull;
+    const int nhalf = pilot ? 1 : 2;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t c3s = ctr3(kTagShuf | tag_base, inst_id);
+    const uint32_t c3e = ctr3(kTagExch | tag_base, inst_id);
+    unsigned long long nsw = 0;
+    for (int h = 0; h < nhalf; ++h) {
+        // pair order of this half: indices sorted by (key, index)
+        for (int j = lane; j < np; j += kWave) {
+            x.key[j] = pick(philox4x32_10((uint32_t)j >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)j);
+        }
+        for (int k = lane; k < nrep; k += kWave) {
+            x.tpos[k] = -1;
+            x.level[k] = 0;
+        }
+        wave_sync();
+        for (int j = lane; j < np; j += kWave) {
+            const uint32_t kj = x.key[j];
+            int rank = 0;
+            for (int i = 0; i < np; ++i) {
+                const uint32_t ki = x.key[i];
+                rank += (ki < kj) || (ki == kj && i < j);
+            }
+            x.perm[rank] = j;
+        }
+        // log-uniform of every attempt of this half (attempt index a is global in the step)
+        for (int i = lane; i < np; i += kWave) {
+            const int a = h * np + i;
+            x.logu[a] = uniform_log(pick(philox4x32_10((uint32_t)a >> 2, 0u, step, c3e, k0, k1), (uint32_t)a));
+        }
+        wave_sync();
+        for (int i = lane; i < np; i += kWave) {
+            const int a = h * np + i;
+            const int chain = pilot ? 0 : (a & 1);
+            x.tpos[x.perm[i] + chain * nt] = a;
+        }
+        wave_sync();
+        // dependency levels: level(k) = 1 + max level of an earlier attempt on k-1 or k+1
+        bool changed = true;
+        while (changed) {
+            bool mine = false;
+            for (int k = lane; k < nrep; k += kWave) {
+                const int tk = x.tpos[k];
+                if (tk < 0) continue;
+                const int kc = k % nt;  // index inside the chain; pairs are 0..nt-2
+                int lv = 1;
+                if (kc > 0) {
+                    const int tl = x.tpos[k - 1];
+                    if (tl >= 0 && tl < tk) lv = max(lv, x.level[k - 1] + 1);
+                }
+                if (kc + 1 < np) {
+                    const int tr = x.tpos[k + 1];
+                    if (tr >= 0 && tr < tk) lv = max(lv, x.level[k + 1] + 1);
+                }
+                if (lv != x.level[k]) {
+                    x.level[k] = lv;
+                    mine = true;
+                }
+            }
+            wave_sync();
+            changed = __ballot(mine) != 0ull;
+        }
+        int maxlv = 0;
+        for (int k = lane; k < nrep; k += kWave) maxlv = max(maxlv, x.level[k]);
+        maxlv = wave_max_i32(maxlv);
+        for (int L = 1; L <= maxlv; ++L) {
+            for (int k = lane; k < nrep; k += kWave) {
+                if (x.level[k] != L) continue;
+                const int r1 = s2r[k], r2 = s2r[k + 1];
+                const double lb = (beta[k] - beta[k + 1]) * (double)(E[r1] - E[r2]);
+                att[k] += 1u;
+                if (x.logu[x.tpos[k]] < lb) {
+                    s2r[k] = r2;
+                    s2r[k + 1] = r1;
+                    succ[k] += 1u;
+                    ++nsw;
+                }
+            }
+            wave_sync();
+        }
+    }
+    return nsw;
+}
+
+// ---------------------------------------------------------------------------
+// Houdayer cluster move on one slot pair by one wavefront (icm.cc:8-215).
+// S1/S2 hold both states (bit words, copied in from global), D the
+// disagreement set, V the cluster; Q is a queue of up to n/2 positions.  The
+// cluster is grown breadth-first from a uniform seed in D; the reference grows
+// it depth-first (icm.cc:37-72), which gives the same component.  Returns the
+// energy changes of both replicas; the new states are written to g1/g2.
+// ---------------------------------------------------------------------------
+struct IcmScratch {
+    uint32_t* S1;
+    uint32_t* S2;
+    uint32_t* D;
+    uint32_t* V;
+    int32_t* tail;
+    uint16_t* Q;
+};
+
+template <int Fmt>
+__device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf, bool has_field, int n, int nw,
+                                         int t, uint32_t step, uint64_t seed, uint32_t c3, uint32_t* g1, uint32_t* g2,
+                                         IcmScratch sc, long long& dE1_out, long long& dE2_out, int lane) {
+    uint32_t* S1 = sc.S1;
+    uint32_t* S2 = sc.S2;
+    uint32_t* D = sc.D;
+    uint32_t* V = sc.V;
+    const uint32_t lastmask = (n & 31) ? ((1u << (n & 31)) - 1u) : 0xFFFFFFFFu;
+    int cnt = 0;
+    for (int w = lane; w < nw; w += kWave) {
+        const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
+        const uint32_t x1 = g1[w] & m, x2 = g2[w] & m;
+        S1[w] = x1;
+        S2[w] = x2;
+        D[w] = x1 ^ x2;
+        V[w] = 0u;
+        cnt += __popc(x1 ^ x2);
+    }
+    cnt = wave_sum_i32(cnt);
+    long long dE1 = 0, dE2 = 0;
+    wave_sync();
+    if (cnt > n / 2) {  // icm.cc:20-28: flip replica 1 globally
+        if (has_field) {
+            for (int p = lane; p < n; p += kWave) {
+                const int s1 = (S1[p >> 5] >> (p & 31)) & 1u;
+                dE1 += -2ll * (s1 ? 1 : -1) * hf[p];
+            }
+        }
+        wave_sync();
+        for (int w = lane; w < nw; w += kWave) {
+            const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
+            S1[w] = ~S1[w] & m;
+            D[w] = ~D[w] & m;
+        }
+        cnt = n - cnt;
+        wave_sync();
+    }
+    if (cnt > 0) {
+        const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const int k = (int)(((unsigned long long)pick(x, (uint32_t)t) * (unsigned long long)cnt) >> 32);
+        // k-th element of D in position order: per-64-word chunks, inclusive scan
+        int seed_pos = -1, base = 0;
+        for (int w0 = 0; w0 < nw && seed_pos < 0; w0 += kWave) {
+            const int w = w0 + lane;
+            const uint32_t dw = w < nw ? D[w] : 0u;
+            const int c = __popc(dw);
+            int incl = c;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int y = __shfl_up(incl, o, kWave);
+                if (lane >= o) incl += y;
+            }
+            const int total = __shfl(incl, kWave - 1, kWave);
+            if (k < base + total) {
+                const bool mine = (k < base + incl) && (k >= base + incl - c);
+                const unsigned long long ball = __ballot(mine);
+                const int owner = __ffsll((long long)ball) - 1;
+                int sel = -1;
+                if (lane == owner) {
+                    int kk = k - (base + incl - c);
+                    for (int b = 0; b < 32; ++b) {
+                        if ((dw >> b) & 1u) {
+                            if (kk == 0) {
+                                sel = 32 * w + b;
+                                break;
+                            }
+                            --kk;
+                        }
+                    }
+                }
+                seed_pos = __shfl(sel, owner, kWave);
+            }
+            base += total;
+        }
+        if (lane == 0) {
+            V[seed_pos >> 5] |= 1u << (seed_pos & 31);
+            sc.Q[0] = (uint16_t)seed_pos;
+            sc.tail[0] = 1;
+        }
+        wave_sync();
+        int head = 0;
+        while (true) {
+            const int tl = sc.tail[0];
+            wave_sync();
+            if (head >= tl) break;
+            const int idx = head + lane;
+            if (idx < tl) {
+                const int p = sc.Q[idx];
+                for (int q = csr.begin(p); q < csr.end(p); ++q) {
+                    const int j = csr.col(q);
+                    const uint32_t bit = 1u << (j & 31);
+                    if ((D[j >> 5] & bit) && !(V[j >> 5] & bit)) {
+                        const uint32_t old = atomicOr(&V[j >> 5], bit);
+                        if (!(old & bit)) {
+                            const int pos = atomicAdd(&sc.tail[0], 1);
+                            sc.Q[pos] = (uint16_t)j;
+                        }
+                    }
+                }
+            }
+            head = (tl - head > kWave) ? head + kWave : tl;
+            wave_sync();
+        }
+        const int csize = sc.tail[0];
+        // energy change of flipping the cluster C in each replica: boundary
+        // edges (p in C, j not in C) and the field terms of C
+        for (int idx = lane; idx < csize; idx += kWave) {
+            const int p = sc.Q[idx];
+            const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
+            const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
+            long long b1 = 0, b2 = 0;
+            for (int q = csr.begin(p); q < csr.end(p); ++q) {
+                int j, w;
+                csr.edge(q, j, w);
+                if ((V[j >> 5] >> (j & 31)) & 1u) continue;
+                b1 += (long long)w * (((S1[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
+                b2 += (long long)w * (((S2[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
+            }
+            dE1 += -2ll * s1 * b1;
+            dE2 += -2ll * s2 * b2;
+            if (has_field) {
+                dE1 += -2ll * s1 * hf[p];
+                dE2 += -2ll * s2 * hf[p];
+            }
+        }
+        wave_sync();
+        for (int w = lane; w < nw; w += kWave) {
+            S1[w] ^= V[w];
+            S2[w] ^= V[w];
+        }
+        wave_sync();
+    }
+    dE1_out = wave_sum_i64(dE1);
+    dE2_out = wave_sum_i64(dE2);
+    for (int w = lane; w < nw; w += kWave) {
+        g1[w] = S1[w];
+        g2[w] = S2[w];
+    }
+}
+
+// LDS bytes of the ICM scratch for n spins (S1,S2,D,V words, tail, queue of n/2+2).
+__host__ __device__ inline int icm_scratch_bytes(int n) {
+    const int nw = (n + 31) / 32;
+    return 16 * nw + 16 + 2 * (n / 2 + 2);
 }
 
 }  // namespace ptx

@@ -95,13 +95,18 @@ def test_device_fp64_bitwise(ptx, oracle):
 SWEEP_CASES = ["lattice_2d_5.gset", "lattice_2d_20.gset", "pm_torus20", "G022", "field_random", "ea3d_6"]
 
 
+MODES = [1, 2]  # split kernels, fused per-instance kernel
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("kind", SWEEP_CASES)
-def test_init_and_sweep_parity(ptx, oracle, kind):
+def test_init_and_sweep_parity(ptx, oracle, kind, mode):
     p, g = make_pair(ptx, oracle, kind)
     nt = 6
     betas = np.linspace(0.3, 2.0, nt)
-    say("case", kind, "n", p.n)
-    eng = ptx.Engine(p, betas, seed=17, flags=ptx.FLAG_NO_EXCHANGE)
+    say("case", kind, "n", p.n, "mode", mode)
+    eng = ptx.Engine(p, betas, seed=17, flags=ptx.FLAG_NO_EXCHANGE, mode=mode)
+    assert eng.info().mode == mode
     oe = oracle.Engine(g, betas, chains=2, seed=17, flags=oracle.FLAG_NO_EXCHANGE)
     compare(eng, oe, what="init")
     for s in range(4):
@@ -110,11 +115,12 @@ def test_init_and_sweep_parity(ptx, oracle, kind):
         compare(eng, oe, what=f"step {s}")
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("sweeps", [2, 3])
-def test_multiple_sweeps_per_step(ptx, oracle, sweeps):
+def test_multiple_sweeps_per_step(ptx, oracle, sweeps, mode):
     p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
     betas = np.linspace(0.4, 1.2, 5)
-    eng = ptx.Engine(p, betas, seed=2, sweeps=sweeps, flags=ptx.FLAG_NO_EXCHANGE)
+    eng = ptx.Engine(p, betas, seed=2, sweeps=sweeps, flags=ptx.FLAG_NO_EXCHANGE, mode=mode)
     oe = oracle.Engine(g, betas, chains=2, seed=2, sweeps=sweeps, flags=oracle.FLAG_NO_EXCHANGE)
     eng.step(3)
     oe.step(3)
@@ -126,15 +132,16 @@ FULL_CASES = [("lattice_2d_20.gset", 8, 0.5, 3.0, 20), ("pm_torus20", 32, 0.5, 3
               ("G022", 16, 0.6, 2.8, 6), ("field_random", 10, 0.5, 3.0, 10), ("er_small", 12, 0.4, 2.5, 10)]
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("kind,nt,T0,T1,steps", FULL_CASES)
 @pytest.mark.parametrize("icm", [False, True])
-def test_full_step_parity(ptx, oracle, kind, nt, T0, T1, steps, icm):
+def test_full_step_parity(ptx, oracle, kind, nt, T0, T1, steps, icm, mode):
     p, g = make_pair(ptx, oracle, kind)
     betas = oracle.beta_linspace(T0, T1, nt, 0)
-    say("case", kind, "nt", nt, "icm", icm)
+    say("case", kind, "nt", nt, "icm", icm, "mode", mode)
     flags = ptx.FLAG_ICM if icm else 0
     nc = -1 if kind != "field_random" else nt // 2
-    eng = ptx.Engine(p, betas, seed=5, flags=flags, nc=nc)
+    eng = ptx.Engine(p, betas, seed=5, flags=flags, nc=nc, mode=mode, steps_per_launch=3)
     oe = oracle.Engine(g, betas, chains=2, seed=5, flags=(oracle.FLAG_ICM if icm else 0), nc=nc)
     if icm:
         assert eng.instance_info().nc == oe.nc
@@ -148,12 +155,13 @@ def test_full_step_parity(ptx, oracle, kind, nt, T0, T1, steps, icm):
     compare(eng, oe, what=f"{kind} bulk")
 
 
-def test_icm_all_slots_parity(ptx, oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_icm_all_slots_parity(ptx, oracle, mode):
     """ICM on every slot (nc = nt), including global flips of replica 1."""
     p, g = make_pair(ptx, oracle, "G039")
     nt = 8
     betas = oracle.beta_linspace(0.5, 2.0, nt, 0)
-    eng = ptx.Engine(p, betas, seed=33, flags=ptx.FLAG_ICM, nc=nt)
+    eng = ptx.Engine(p, betas, seed=33, flags=ptx.FLAG_ICM, nc=nt, mode=mode)
     oe = oracle.Engine(g, betas, chains=2, seed=33, flags=oracle.FLAG_ICM, nc=nt)
     for s in range(12):
         eng.step(1)
@@ -161,11 +169,12 @@ def test_icm_all_slots_parity(ptx, oracle):
         compare(eng, oe, what=f"step {s}")
 
 
-@pytest.mark.parametrize("wpg", [1, 2, 4, 8])
-def test_launch_shape_independence(ptx, oracle, wpg):
+@pytest.mark.parametrize("mode,wpg,spl", [(1, 1, 0), (1, 2, 0), (1, 4, 0), (1, 8, 0), (2, 4, 1), (2, 8, 7),
+                                          (2, 16, 16)])
+def test_launch_shape_independence(ptx, oracle, mode, wpg, spl):
     p, g = make_pair(ptx, oracle, "pm_torus20")
     betas = oracle.beta_linspace(0.5, 3.0, 16, 0)
-    eng = ptx.Engine(p, betas, seed=9, flags=ptx.FLAG_ICM, waves_per_group=wpg)
+    eng = ptx.Engine(p, betas, seed=9, flags=ptx.FLAG_ICM, waves_per_group=wpg, mode=mode, steps_per_launch=spl)
     oe = oracle.Engine(g, betas, chains=2, seed=9, flags=oracle.FLAG_ICM)
     eng.step(15)
     oe.step(15)
@@ -184,11 +193,12 @@ def test_write_spins_recomputes_energy(ptx, oracle):
 
 
 # ---------------------------------------------------------- pilot / hist ----
-def test_pilot_histograms_match_oracle_logs(ptx, oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_pilot_histograms_match_oracle_logs(ptx, oracle, mode):
     p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
     nt, burn, ns = 10, 100, 400
     hb, bs, be = ptx.pilot_betas(0.6, 2.8, nt)
-    eng = ptx.Engine(p, hb, seed=21, chains=1, flags=ptx.FLAG_PILOT | ptx.FLAG_HISTOGRAM, hist_start=burn)
+    eng = ptx.Engine(p, hb, seed=21, chains=1, flags=ptx.FLAG_PILOT | ptx.FLAG_HISTOGRAM, hist_start=burn, mode=mode)
     eng.step(burn + ns)
     counts, e_base = eng.histograms()
     logs = oracle.gather_samples(g, hb, 21, burn, ns)
@@ -202,12 +212,16 @@ def test_pilot_histograms_match_oracle_logs(ptx, oracle):
 
 
 # ---------------------------------------------------------------- batched ---
-def test_batched_instances_match_single_oracles(ptx, oracle):
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("with_field", [False, True])
+def test_batched_instances_match_single_oracles(ptx, oracle, mode, with_field):
     kinds = ["lattice_2d_10.gset", "ea3d_6", "pm_torus20", "lattice_2d_5.gset", "er_small"]
+    if with_field:
+        kinds.append("field_random")  # forces the 32-bit adjacency format in the fused kernel
     pairs = [make_pair(ptx, oracle, k) for k in kinds]
-    nts = [4, 8, 6, 3, 5]
+    nts = [4, 8, 6, 3, 5, 7][: len(kinds)]
     ladders = [oracle.beta_linspace(0.5, 3.0, nt, 0) for nt in nts]
-    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=77, flags=ptx.FLAG_ICM)
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=77, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=2)
     oes = [oracle.Engine(g, lad, chains=2, seed=77, instance=i, flags=oracle.FLAG_ICM)
            for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
     for s in range(6):
@@ -231,6 +245,7 @@ def test_batched_3d_ea_invariants(ptx, oracle):
         graphs.append(oracle.Graph.from_coo(n, u, v, w))
     lad = oracle.beta_linspace(0.2, 3.0, 32, 1)
     eng = ptx.Engine(probs, [lad] * len(probs), seed=3, flags=ptx.FLAG_ICM)
+    assert eng.info().mode == 2  # batches run the fused kernel
     eng.step(40)
     for i in range(0, len(probs), 7):
         sp = eng.spins(i)
