@@ -82,14 +82,22 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // ---- host layout ----
     desc.resize(ninst);
     ladders.resize(ninst);
-    std::vector<int32_t> rowptr, cls, hv, p2o;
-    std::vector<uint32_t> adj, thr;
-    std::vector<uint16_t> adj16;
-    bool unit_ok = true;
+    std::vector<int32_t> cls, hv, p2o;
+    std::vector<uint32_t> rowinfo, thr;
+    std::vector<uint32_t> adjw;  // padded rows as 32-bit words (both formats)
     std::vector<double> beta;
-    int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0, adj_total = 0;
+    int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0;
     const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
     int64_t t_off = 0;
+    // record format: 16-bit when every instance has |w| = 1 and n <= 32768
+    bool unit_ok = true;
+    for (const auto& pp : problems) {
+        if (pp->max_deg > kMaxDegree)
+            return Status::err(PTX_ERR_UNSUPPORTED, "a vertex has more than 511 neighbours");
+        if (pp->n > 32768) unit_ok = false;
+        for (int32_t wv : pp->w) unit_ok = unit_ok && (wv == 1 || wv == -1);
+    }
+    fmt = unit_ok ? kFmtUnit : kFmtWide;
     for (int32_t i = 0; i < ninst; ++i) {
         const Problem& P = *problems[i];
         const int32_t nt = n_temps[i];
@@ -111,26 +119,40 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         }
         d.nc = nc;
         d.inst_id = (uint32_t)i;
-        // adjacency by position
-        d.rowptr_off = (int64_t)rowptr.size();
+        // adjacency rows by position, each padded to 16 bytes
+        d.pos_off = (int64_t)rowinfo.size();
+        d.adj_off16 = (int64_t)adjw.size() / 4;
+        int64_t cur16 = 0;
         for (int32_t p = 0; p < P.n; ++p) {
-            rowptr.push_back((int32_t)adj_total);
             const int32_t o = P.pos2orig[p];
-            for (int32_t q = P.rowptr[o]; q < P.rowptr[o + 1]; ++q) {
-                const uint32_t pos = (uint32_t)P.orig2pos[P.col[q]];
-                adj.push_back(pos | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
-                const bool unit = (P.w[q] == 1 || P.w[q] == -1) && pos < 32768u;
-                unit_ok = unit_ok && unit;
-                adj16.push_back((uint16_t)(pos | (P.w[q] < 0 ? 0x8000u : 0u)));
-                ++adj_total;
+            const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
+            if (cur16 >= (1ll << 23)) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
+            rowinfo.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
+            if (fmt == kFmtUnit) {
+                const int32_t chunks = (dg + 7) / 8;
+                std::vector<uint16_t> rec((size_t)chunks * 8, 0);
+                for (int32_t j = 0; j < dg; ++j) {
+                    const int32_t q = P.rowptr[o] + j;
+                    rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 0x8000u : 0u));
+                }
+                for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
+                cur16 += chunks;
+            } else {
+                const int32_t chunks = (dg + 3) / 4;
+                for (int32_t j = 0; j < chunks * 4; ++j) {
+                    if (j < dg) {
+                        const int32_t q = P.rowptr[o] + j;
+                        adjw.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
+                    } else {
+                        adjw.push_back(0u);  // weight 0: contributes nothing
+                    }
+                }
+                cur16 += chunks;
             }
         }
-        rowptr.push_back((int32_t)adj_total);
-        if (adj_total > 0x7FFFFFFF) return Status::err(PTX_ERR_UNSUPPORTED, "adjacency larger than 2^31 entries");
+        d.adj_len16 = cur16;
         d.cls_off = (int64_t)cls.size();
         cls.insert(cls.end(), P.class_start.begin(), P.class_start.end());
-        d.h_off = (int64_t)hv.size();
-        d.p2o_off = (int64_t)p2o.size();
         for (int32_t p = 0; p < P.n; ++p) {
             hv.push_back(P.h[P.pos2orig[p]]);
             p2o.push_back(P.pos2orig[p]);
@@ -164,7 +186,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // ---- launch shape ----
     W = cfg.waves_per_group;
     if (W <= 0) W = (total_rep < 4 * 256) ? 1 : 4;  // spread few replicas over CUs, group many
-    if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8");
+    if (W > 8) W = 8;                                 // split kernels take at most 8 waves
+    if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
     spin_stride = (int32_t)align_up((int64_t)nw_max * 32, 16);
     thr_stride = (int32_t)align_up(fmax_max, 4);
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
@@ -183,15 +206,14 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
 
     // ---- execution mode ----
-    fmt = unit_ok ? kFmtUnit : kFmtWide;
     mode = PTX_MODE_SPLIT;
     if (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO) {
         Wf = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
         if (Wf != 4 && Wf != 8 && Wf != 16) Wf = Wf < 6 ? 4 : (Wf < 12 ? 8 : 16);
         int lds_max = 0;
         for (int32_t i = 0; i < ninst; ++i) {
-            const Problem& P = *problems[i];
-            const FusedLayout L = fused_layout(P.n, (int)P.nnz(), desc[i].nrep, desc[i].nt, desc[i].fmax, Wf, fmt);
+            const FusedLayout L =
+                fused_layout(desc[i].n, (int)desc[i].adj_len16, desc[i].nrep, desc[i].nt, desc[i].fmax, Wf);
             lds_max = std::max(lds_max, L.total);
         }
         const int lds_limit = 160 * 1024;
@@ -208,13 +230,10 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
 
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
-    if (mode == PTX_MODE_FUSED) {
-        if (fmt == kFmtUnit) PTX_TRY(d_adj16.upload(adj16, stream));
-        PTX_TRY(d_order.upload(fused_order, stream));
-    }
+    if (mode == PTX_MODE_FUSED) PTX_TRY(d_order.upload(fused_order, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
-    PTX_TRY(d_rowptr.upload(rowptr, stream));
-    PTX_TRY(d_adj.upload(adj, stream));
+    PTX_TRY(d_rowinfo.upload(rowinfo, stream));
+    PTX_TRY(d_adj.upload(adjw, stream));
     PTX_TRY(d_cls.upload(cls, stream));
     PTX_TRY(d_h.upload(hv, stream));
     PTX_TRY(d_p2o.upload(p2o, stream));
@@ -242,8 +261,9 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
 static DevArgs build_args(Engine& e) {
     DevArgs a{};
     a.desc = e.d_desc.p;
-    a.rowptr = e.d_rowptr.p;
-    a.adj = e.d_adj.p;
+    a.rowinfo = e.d_rowinfo.p;
+    a.adj = reinterpret_cast<const uint8_t*>(e.d_adj.p);
+    a.fmt = e.fmt;
     a.cls = e.d_cls.p;
     a.hfield = e.d_h.p;
     a.p2o = e.d_p2o.p;
@@ -281,11 +301,21 @@ Status Engine::launch_init(int mode) {
     DevArgs a = build_args(*this);
     const size_t lds = (size_t)W * spin_stride;
     const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
-    switch (W) {
-        case 1: hipLaunchKernelGGL(k_init<1>, grid, block, lds, stream, a, mode); break;
-        case 2: hipLaunchKernelGGL(k_init<2>, grid, block, lds, stream, a, mode); break;
-        case 4: hipLaunchKernelGGL(k_init<4>, grid, block, lds, stream, a, mode); break;
-        default: hipLaunchKernelGGL(k_init<8>, grid, block, lds, stream, a, mode); break;
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, mode); };
+    if (fmt == kFmtUnit) {
+        switch (W) {
+            case 1: go(k_init<kFmtUnit, 1>); break;
+            case 2: go(k_init<kFmtUnit, 2>); break;
+            case 4: go(k_init<kFmtUnit, 4>); break;
+            default: go(k_init<kFmtUnit, 8>); break;
+        }
+    } else {
+        switch (W) {
+            case 1: go(k_init<kFmtWide, 1>); break;
+            case 2: go(k_init<kFmtWide, 2>); break;
+            case 4: go(k_init<kFmtWide, 4>); break;
+            default: go(k_init<kFmtWide, 8>); break;
+        }
     }
     return hip_check(hipGetLastError(), "k_init launch");
 }
@@ -312,11 +342,21 @@ Status Engine::launch_step(uint32_t s) {
         const size_t lds = (size_t)W * (spin_stride + 4 * (size_t)thr_stride);
         const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
         PTX_TRY(timed(0, [&] {
-            switch (W) {
-                case 1: hipLaunchKernelGGL(k_sweep<1>, grid, block, lds, stream, a, s); break;
-                case 2: hipLaunchKernelGGL(k_sweep<2>, grid, block, lds, stream, a, s); break;
-                case 4: hipLaunchKernelGGL(k_sweep<4>, grid, block, lds, stream, a, s); break;
-                default: hipLaunchKernelGGL(k_sweep<8>, grid, block, lds, stream, a, s); break;
+            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, s); };
+            if (fmt == kFmtUnit) {
+                switch (W) {
+                    case 1: go(k_sweep<kFmtUnit, 1>); break;
+                    case 2: go(k_sweep<kFmtUnit, 2>); break;
+                    case 4: go(k_sweep<kFmtUnit, 4>); break;
+                    default: go(k_sweep<kFmtUnit, 8>); break;
+                }
+            } else {
+                switch (W) {
+                    case 1: go(k_sweep<kFmtWide, 1>); break;
+                    case 2: go(k_sweep<kFmtWide, 2>); break;
+                    case 4: go(k_sweep<kFmtWide, 4>); break;
+                    default: go(k_sweep<kFmtWide, 8>); break;
+                }
             }
         }));
     }
@@ -329,7 +369,10 @@ Status Engine::launch_step(uint32_t s) {
     if (!pairs.empty()) {
         const size_t lds = (size_t)nw_max * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
         PTX_TRY(timed(2, [&] {
-            hipLaunchKernelGGL(k_icm, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
+            if (fmt == kFmtUnit)
+                hipLaunchKernelGGL(k_icm<kFmtUnit>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
+            else
+                hipLaunchKernelGGL(k_icm<kFmtWide>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
         }));
     }
     return Status::ok();
@@ -337,7 +380,6 @@ Status Engine::launch_step(uint32_t s) {
 
 Status Engine::launch_fused(int nsteps) {
     DevArgs a = build_args(*this);
-    const void* adjp = fmt == kFmtUnit ? (const void*)d_adj16.p : (const void*)d_adj.p;
     const dim3 grid((unsigned)ninst), block((unsigned)(Wf * kWave));
     const size_t lds = (size_t)fused_lds;
     auto go = [&](auto kern) -> Status {
@@ -352,7 +394,7 @@ Status Engine::launch_fused(int nsteps) {
             PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
             PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
         }
-        hipLaunchKernelGGL(kern, grid, block, lds, stream, a, adjp, nsteps);
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, a, nsteps);
         PTX_TRY(hip_check(hipGetLastError(), "k_fused launch"));
         if (timing) {
             PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));

@@ -63,15 +63,14 @@ struct Engine {
     int64_t launches[4] = {0, 0, 0, 0};
 
     DevBuf<InstDesc> d_desc;
-    DevBuf<int32_t> d_rowptr, d_cls, d_h, d_p2o;
-    DevBuf<uint32_t> d_adj, d_thr;
+    DevBuf<int32_t> d_cls, d_h, d_p2o;
+    DevBuf<uint32_t> d_rowinfo, d_adj, d_thr;
     DevBuf<double> d_beta;
     DevBuf<uint32_t> d_spins, d_best_spins;
     DevBuf<int32_t> d_E, d_bestE, d_best_sweep, d_slot2rep, d_rep2slot;
     DevBuf<unsigned long long> d_flips, d_swaps;
     DevBuf<uint32_t> d_att, d_succ, d_hist;
     DevBuf<int2> d_items, d_pairs;
-    DevBuf<uint16_t> d_adj16;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
 

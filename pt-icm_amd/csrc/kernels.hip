@@ -18,7 +18,7 @@ namespace ptx {
 // k_init: one wave per replica.  mode 0 draws the initial state; mode 1 keeps
 // the stored state and only recomputes its energy.
 // ---------------------------------------------------------------------------
-template <int W>
+template <int Fmt, int W>
 __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
         for (int p = lane; p < d.nw * 32; p += kWave) {
             uint8_t b = 0;
             if (p < d.n) {
-                const uint32_t i = (uint32_t)a.p2o[d.p2o_off + p];
+                const uint32_t i = (uint32_t)a.p2o[d.pos_off + p];
                 const Philox4 x =
                     philox4x32_10(i >> 2, (uint32_t)r, 0u, c3, (uint32_t)a.seed, (uint32_t)(a.seed >> 32));
                 b = (uint8_t)(pick(x, i) >> 31);
@@ -48,8 +48,8 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
         expand_bits(words, d.nw, sp, lane);
         wave_sync();
     }
-    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
-    const int32_t E = energy_bytes(csr, d.n, a.hfield + d.h_off, d.has_field != 0, sp, lane);
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
+    const int32_t E = energy_bytes(csr, d.n, a.hfield + d.pos_off, d.has_field != 0, sp, lane);
     if (lane == 0) {
         a.E[gr] = E;
         if (mode == 0) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
 // k_sweep: W wavefronts per workgroup, each sweeping one replica of the
 // workgroup's instance.  LDS per wave: spin bytes + the replica's threshold row.
 // ---------------------------------------------------------------------------
-template <int W>
+template <int Fmt, int W>
 __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -90,11 +90,11 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
     wave_sync();
-    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
     const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     for (int sw = 0; sw < a.sweeps; ++sw) {
         const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
-        sweep_replica(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.h_off, d.has_field != 0, d.nw, c3, a.seed, r,
+        sweep_replica(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw, c3, a.seed, r,
                       gsweep, sp, thr_row, best_words, st, lane);
     }
     pack_bits(sp, d.nw, words, lane);
@@ -163,6 +163,7 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, in
 // ---------------------------------------------------------------------------
 // k_icm: one wavefront per (instance, slot t) pair, t in [nt-nc, nt).
 // ---------------------------------------------------------------------------
+template <int Fmt>
 __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_max) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
@@ -180,9 +181,9 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
     uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * d.nw;
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
-    const Csr<kFmtWide> csr{a.rowptr + d.rowptr_off, a.adj};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
     long long dE1 = 0, dE2 = 0;
-    icm_wave(csr, a.hfield + d.h_off, d.has_field != 0, d.n, d.nw, t, step, a.seed,
+    icm_wave(csr, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed,
              ctr3(kTagIcm | a.tag_base, d.inst_id), g1, g2, sc, dE1, dE2, lane);
     if (lane == 0) {
         a.E[ro + r1] += (int32_t)dE1;
@@ -198,18 +199,18 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
 // cluster moves).  Spin words stay in global memory (L2-resident).
 // ---------------------------------------------------------------------------
 struct FusedLayout {
-    int rp, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, scratch, scratch_stride, total;
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, scratch, scratch_stride, total;
 };
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
-__host__ __device__ inline FusedLayout fused_layout(int n, int nnz, int nrep, int nt, int fmax, int W, int fmt) {
+__host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nrep, int nt, int fmax, int W) {
     FusedLayout L;
     int o = 0;
-    L.rp = o;
-    o += align16(4 * (n + 1));
+    L.info = o;
+    o += align16(4 * n);
     L.adj = o;
-    o += align16((fmt == kFmtUnit ? 2 : 4) * (nnz > 0 ? nnz : 1));
+    o += 16 * (adj_len16 > 0 ? adj_len16 : 1);
     L.thr = o;
     o += align16(4 * nt * (fmax > 0 ? fmax : 1));
     L.beta = o;
@@ -245,18 +246,16 @@ __host__ __device__ inline FusedLayout fused_layout(int n, int nnz, int nrep, in
 }
 
 template <int Fmt, int W>
-__global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, const void* __restrict__ adj_fmt, int nsteps) {
+__global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tid = threadIdx.x;
     const int inst = a.order[blockIdx.x];
     const InstDesc d = a.desc[inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
-    const int32_t* grp = a.rowptr + d.rowptr_off;
-    const int q0 = grp[0];
-    const int nnz = grp[n] - q0;
-    const FusedLayout L = fused_layout(n, nnz, nrep, nt, d.fmax, W, Fmt);
-    int32_t* rp = reinterpret_cast<int32_t*>(smem + L.rp);
+    const int adj16 = (int)d.adj_len16;
+    const FusedLayout L = fused_layout(n, adj16, nrep, nt, d.fmax, W);
+    uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
     int32_t* E = reinterpret_cast<int32_t*>(smem + L.E);
@@ -273,15 +272,11 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, const void* __res
     uint8_t* scratch = smem + L.scratch + wave * L.scratch_stride;
 
     // ---- stage the instance ----
-    for (int p = tid; p <= n; p += W * kWave) rp[p] = grp[p] - q0;
-    if (Fmt == kFmtUnit) {
-        const uint16_t* src = static_cast<const uint16_t*>(adj_fmt) + q0;
-        uint16_t* dst = reinterpret_cast<uint16_t*>(smem + L.adj);
-        for (int q = tid; q < nnz; q += W * kWave) dst[q] = src[q];
-    } else {
-        const uint32_t* src = a.adj + q0;
-        uint32_t* dst = reinterpret_cast<uint32_t*>(smem + L.adj);
-        for (int q = tid; q < nnz; q += W * kWave) dst[q] = src[q];
+    for (int p = tid; p < n; p += W * kWave) info[p] = a.rowinfo[d.pos_off + p];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
+        uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
+        for (int q = tid; q < adj16; q += W * kWave) dst[q] = src[q];
     }
     for (int m = tid; m < nt * d.fmax; m += W * kWave) thr[m] = a.thr[d.thr_off + m];
     const int64_t ro = d.rep_off;
@@ -294,8 +289,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, const void* __res
         succ[s] = 0;
     }
     __syncthreads();
-    const Csr<Fmt> csr{rp, smem + L.adj};
-    const int32_t* hf = a.hfield + d.h_off;
+    const Csr<Fmt> csr{info, smem + L.adj};
+    const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
@@ -408,19 +403,20 @@ __global__ void k_selftest_philox(const uint32_t* ctr, int64_t n, uint32_t k0, u
 }
 
 // explicit instantiations used by the host launcher
-template __global__ void k_init<1>(DevArgs, int);
-template __global__ void k_init<2>(DevArgs, int);
-template __global__ void k_init<4>(DevArgs, int);
-template __global__ void k_init<8>(DevArgs, int);
-template __global__ void k_sweep<1>(DevArgs, uint32_t);
-template __global__ void k_sweep<2>(DevArgs, uint32_t);
-template __global__ void k_sweep<4>(DevArgs, uint32_t);
-template __global__ void k_sweep<8>(DevArgs, uint32_t);
-template __global__ void k_fused<kFmtWide, 4>(DevArgs, const void*, int);
-template __global__ void k_fused<kFmtWide, 8>(DevArgs, const void*, int);
-template __global__ void k_fused<kFmtWide, 16>(DevArgs, const void*, int);
-template __global__ void k_fused<kFmtUnit, 4>(DevArgs, const void*, int);
-template __global__ void k_fused<kFmtUnit, 8>(DevArgs, const void*, int);
-template __global__ void k_fused<kFmtUnit, 16>(DevArgs, const void*, int);
+#define PTX_SPLIT_INST(F)                                          \
+    template __global__ void k_init<F, 1>(DevArgs, int);          \
+    template __global__ void k_init<F, 2>(DevArgs, int);          \
+    template __global__ void k_init<F, 4>(DevArgs, int);          \
+    template __global__ void k_init<F, 8>(DevArgs, int);          \
+    template __global__ void k_sweep<F, 1>(DevArgs, uint32_t);    \
+    template __global__ void k_sweep<F, 2>(DevArgs, uint32_t);    \
+    template __global__ void k_sweep<F, 4>(DevArgs, uint32_t);    \
+    template __global__ void k_sweep<F, 8>(DevArgs, uint32_t);    \
+    template __global__ void k_icm<F>(DevArgs, uint32_t, int);    \
+    template __global__ void k_fused<F, 4>(DevArgs, int);         \
+    template __global__ void k_fused<F, 8>(DevArgs, int);         \
+    template __global__ void k_fused<F, 16>(DevArgs, int);
+PTX_SPLIT_INST(kFmtWide)
+PTX_SPLIT_INST(kFmtUnit)
 
 }  // namespace ptx

@@ -15,15 +15,17 @@ namespace ptx {
 
 constexpr int kWave = 64;
 
-// Adjacency record formats.  Wide: u32 = position | (int16 weight << 16).
-// Unit: u16 = position | (weight < 0) << 15, for |w| == 1 and n <= 32768.
+// Adjacency record formats.  Wide: u32 = position | (int16 weight << 16),
+// padding records have weight 0.  Unit: u16 = position | (weight < 0) << 15,
+// for |w| == 1 and n <= 32768.  Rows start on 16-byte boundaries.
 enum : int { kFmtWide = 0, kFmtUnit = 1 };
+constexpr int kMaxDegree = 511;
 
 // Arguments common to every kernel (device pointers of the engine).
 struct DevArgs {
     const InstDesc* desc;
-    const int32_t* rowptr;   // absolute offsets into adj
-    const uint32_t* adj;     // wide records
+    const uint32_t* rowinfo; // per position: row start (16-byte units) << 9 | degree
+    const uint8_t* adj;      // padded adjacency rows of every instance
     const int32_t* cls;      // class_start
     const int32_t* hfield;   // h by position
     const int32_t* p2o;      // pos2orig
@@ -45,6 +47,7 @@ struct DevArgs {
     const int2* items;       // (instance, first replica) per sweep workgroup
     const int2* pairs;       // (instance, slot t) per ICM wavefront
     const int32_t* order;    // fused mode: instance per workgroup
+    int32_t fmt;             // adjacency record format
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
     int32_t sweeps;
@@ -79,48 +82,55 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 }
 
 // ---------------------------------------------------------------------------
-// Adjacency access: rowptr entries are offsets into adj; adj holds records of
-// the chosen format (in LDS for the fused kernel, in global memory otherwise).
+// Adjacency access.  info[p] = (row start in 16-byte units) << 9 | degree,
+// relative to base (LDS for the fused kernel, global memory otherwise).  The
+// field of a site reads its row with 128-bit loads, so 64 lanes reading their
+// own rows touch consecutive banks instead of one bank per row stride.
 // ---------------------------------------------------------------------------
 template <int Fmt>
 struct Csr {
-    const int32_t* rp;
-    const void* adj;
-    __device__ __forceinline__ int begin(int p) const { return rp[p]; }
-    __device__ __forceinline__ int end(int p) const { return rp[p + 1]; }
-    __device__ __forceinline__ int col(int q) const {
-        if (Fmt == kFmtUnit) return (int)(static_cast<const uint16_t*>(adj)[q] & 0x7FFFu);
-        return (int)(static_cast<const uint32_t*>(adj)[q] & 0xFFFFu);
-    }
-    __device__ __forceinline__ void edge(int q, int& c, int& w) const {
+    const uint32_t* info;
+    const uint8_t* base;
+    static constexpr int kPerChunk = Fmt == kFmtUnit ? 8 : 4;
+    __device__ __forceinline__ int deg(int p) const { return (int)(info[p] & (uint32_t)kMaxDegree); }
+    __device__ __forceinline__ const uint8_t* row(int p) const { return base + (size_t)(info[p] >> 9) * 16u; }
+    __device__ __forceinline__ void entry(const uint8_t* row, int j, int& c, int& w) const {
         if (Fmt == kFmtUnit) {
-            const uint32_t r = static_cast<const uint16_t*>(adj)[q];
+            const uint32_t r = reinterpret_cast<const uint16_t*>(row)[j];
             c = (int)(r & 0x7FFFu);
             w = (r & 0x8000u) ? -1 : 1;
         } else {
-            const uint32_t r = static_cast<const uint32_t*>(adj)[q];
+            const uint32_t r = reinterpret_cast<const uint32_t*>(row)[j];
             c = (int)(r & 0xFFFFu);
             w = (int32_t)r >> 16;
         }
     }
     // sum_j w_pj s_j with s = 2b - 1, from byte spins
     __device__ __forceinline__ int field(int p, const uint8_t* sp) const {
-        const int q0 = rp[p], q1 = rp[p + 1];
-        int f = 0;
+        const uint32_t ri = info[p];
+        const int d = (int)(ri & (uint32_t)kMaxDegree);
+        const uint4* r = reinterpret_cast<const uint4*>(base + (size_t)(ri >> 9) * 16u);
         if (Fmt == kFmtUnit) {
-            const uint16_t* A = static_cast<const uint16_t*>(adj);
-            int agree = 0;  // number of neighbours with s_j * sign(w) = +1
-            for (int q = q0; q < q1; ++q) {
-                const uint32_t r = A[q];
-                agree += (int)(sp[r & 0x7FFFu] ^ (r >> 15));
+            int agree = 0;  // neighbours with s_j * sign(w_pj) = +1
+            for (int c = 0; 8 * c < d; ++c) {
+                const uint4 v = r[c];
+                const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    const uint32_t e = (wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
+                    if (8 * c + h < d) agree += (int)(sp[e & 0x7FFFu] ^ (e >> 15));
+                }
             }
-            f = 2 * agree - (q1 - q0);
-        } else {
-            const uint32_t* A = static_cast<const uint32_t*>(adj);
-            for (int q = q0; q < q1; ++q) {
-                const uint32_t r = A[q];
-                const int w = (int32_t)r >> 16;
-                f += sp[r & 0xFFFFu] ? w : -w;
+            return 2 * agree - d;
+        }
+        int f = 0;
+        for (int c = 0; 4 * c < d; ++c) {
+            const uint4 v = r[c];
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int w = (int32_t)wv[h] >> 16;  // 0 on padding records
+                f += sp[wv[h] & 0xFFFFu] ? w : -w;
             }
         }
         return f;
@@ -454,8 +464,11 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
             const int idx = head + lane;
             if (idx < tl) {
                 const int p = sc.Q[idx];
-                for (int q = csr.begin(p); q < csr.end(p); ++q) {
-                    const int j = csr.col(q);
+                const uint8_t* row = csr.row(p);
+                const int dg = csr.deg(p);
+                for (int q = 0; q < dg; ++q) {
+                    int j, w;
+                    csr.entry(row, q, j, w);
                     const uint32_t bit = 1u << (j & 31);
                     if ((D[j >> 5] & bit) && !(V[j >> 5] & bit)) {
                         const uint32_t old = atomicOr(&V[j >> 5], bit);
@@ -477,9 +490,11 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
             const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
             const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
             long long b1 = 0, b2 = 0;
-            for (int q = csr.begin(p); q < csr.end(p); ++q) {
+            const uint8_t* row = csr.row(p);
+            const int dg = csr.deg(p);
+            for (int q = 0; q < dg; ++q) {
                 int j, w;
-                csr.edge(q, j, w);
+                csr.entry(row, q, j, w);
                 if ((V[j >> 5] >> (j & 31)) & 1u) continue;
                 b1 += (long long)w * (((S1[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
                 b2 += (long long)w * (((S2[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);

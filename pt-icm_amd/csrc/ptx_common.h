@@ -190,8 +190,9 @@ PTX_HD double uniform_log(uint32_t r) {
 
 // ---------------------------------------------------------------------------
 // Device data layout.  Positions p are the colour-sorted vertex order; all
-// device arrays are indexed by position.  Adjacency records are 32-bit:
-// bits 0..15 = neighbour position, bits 16..31 = signed weight.
+// device arrays are indexed by position.  The adjacency of position p is a
+// row of records padded to 16 bytes (one 128-bit load reads 8 16-bit or 4
+// 32-bit records); rowinfo[p] = (row start in 16-byte units) << 9 | degree.
 // ---------------------------------------------------------------------------
 struct InstDesc {
     int32_t n;         // spins
@@ -202,10 +203,10 @@ struct InstDesc {
     int32_t fmax;      // threshold-table width (max |dE|/2)
     int32_t nc;        // ICM slots
     int32_t has_field; // any h != 0
-    int64_t rowptr_off;  // -> rowptr[n+1] (absolute offsets into adj)
+    int64_t pos_off;     // -> rowinfo[n], h[n], pos2orig[n]
+    int64_t adj_off16;   // -> first 16-byte unit of the instance's adjacency rows
+    int64_t adj_len16;   // 16-byte units of adjacency rows
     int64_t cls_off;     // -> class_start[ncls+1]
-    int64_t h_off;       // -> h[n]
-    int64_t p2o_off;     // -> pos2orig[n]
     int64_t thr_off;     // -> thr[nt][fmax]
     int64_t beta_off;    // -> beta[nrep] (slot betas, FP64)
     int64_t rep_off;     // -> per-replica arrays (E, best, flips, rep2slot, slot2rep, swap stats)
