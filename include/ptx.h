@@ -198,6 +198,11 @@ typedef struct {
 ptx_status ptx_engine_timing_enable(ptx_engine* e, int32_t on);
 ptx_status ptx_engine_timing_get(ptx_engine* e, ptx_engine_timing* out);
 
+/* Fused mode: wall-clock microseconds that workgroup 0 spent in the sweep,
+ * exchange and cluster-move phases since profiling was switched on (on >= 0
+ * resets and switches; on < 0 only reads).  us may be NULL. */
+ptx_status ptx_engine_phase_profile(ptx_engine* e, int32_t on, double* us);
+
 /* Device self-test of the FP64 helpers the exchange uses (exp/log portable
  * forms), for bitwise comparison against the host forms. */
 ptx_status ptx_selftest_fp64(const double* x, int64_t n, double* exp_out, double* log_out);

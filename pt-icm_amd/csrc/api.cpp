@@ -336,6 +336,10 @@ ptx_status ptx_engine_timing_get(ptx_engine* e, ptx_engine_timing* out) {
     PTX_ENGINE_CALL(e->e.timing_get(out));
 }
 
+ptx_status ptx_engine_phase_profile(ptx_engine* e, int32_t on, double* us) {
+    PTX_ENGINE_CALL(e->e.phase_profile(on, us));
+}
+
 ptx_status ptx_selftest_fp64(const double* x, int64_t n, double* exp_out, double* log_out) {
     if (!x || !exp_out || !log_out || n < 0) return fail(PTX_ERR_INVALID, "null buffer");
     return guarded([&]() { return done(ptx::selftest_fp64(x, n, exp_out, log_out)); });

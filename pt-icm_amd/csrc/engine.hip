@@ -284,6 +284,7 @@ static DevArgs build_args(Engine& e) {
     a.items = e.d_items.p;
     a.pairs = e.d_pairs.p;
     a.order = e.d_order.p;
+    a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.inst_step = e.d_inst_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
@@ -464,6 +465,23 @@ Status Engine::timing_get(ptx_engine_timing* out) {
     return Status::ok();
 }
 
+Status Engine::phase_profile(int on, double* us) {
+    if (on >= 0) {
+        if (!d_phase.p) PTX_TRY(d_phase.alloc(4));
+        PTX_TRY(sync());
+        PTX_TRY(hip_check(hipMemset(d_phase.p, 0, 4 * sizeof(unsigned long long)), "hipMemset"));
+        phase_on = on != 0;
+    }
+    if (us) {
+        unsigned long long t[4] = {0, 0, 0, 0};
+        if (d_phase.p) PTX_TRY(d_phase.download(t, 4, 0, stream));
+        int rate_khz = 0;
+        PTX_TRY(hip_check(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0), "wall clock rate"));
+        for (int k = 0; k < 3; ++k) us[k] = rate_khz > 0 ? (double)t[k] * 1e3 / rate_khz : 0.0;
+    }
+    return Status::ok();
+}
+
 static Status check_inst(const Engine& e, int32_t inst) {
     if (inst < 0 || inst >= e.ninst) return Status::err(PTX_ERR_INVALID, "instance index out of range");
     return Status::ok();
@@ -599,6 +617,7 @@ Status selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint
 
 template struct DevBuf<uint32_t>;
 template struct DevBuf<uint16_t>;
+template struct DevBuf<unsigned long long>;
 template struct DevBuf<int32_t>;
 template struct DevBuf<double>;
 

@@ -73,6 +73,8 @@ struct Engine {
     DevBuf<int2> d_items, d_pairs;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
+    DevBuf<unsigned long long> d_phase;
+    bool phase_on = false;
 
     ~Engine();
     Status create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -89,6 +91,7 @@ struct Engine {
     Status read_histograms(int32_t inst, uint32_t* out);
     Status read_min_energies(int64_t* out);
     Status timing_get(ptx_engine_timing* out);
+    Status phase_profile(int on, double* us);
 
   private:
     Status launch_init(int mode);

@@ -304,6 +304,15 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     sc.tail = reinterpret_cast<int32_t*>(sc.V + nw);
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
 
+    const bool prof = a.phase != nullptr && blockIdx.x == 0 && tid == 0;
+    unsigned long long t_mark = prof ? wall_clock64() : 0ull;
+    auto phase_done = [&](int k) {
+        if (prof) {
+            const unsigned long long t = wall_clock64();
+            a.phase[k] += t - t_mark;
+            t_mark = t;
+        }
+    };
     for (int it = 0; it < nsteps; ++it) {
         const uint32_t step = step0 + (uint32_t)it;
         // ---- sweeps: replica r = wave, wave + W, ... ----
@@ -334,6 +343,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
             wave_sync();
         }
         __syncthreads();
+        phase_done(0);
         // ---- histogram + exchange (wave 0) ----
         if (wave == 0) {
             record_histogram(a, d, step, E, s2r, lane);
@@ -345,6 +355,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
             }
         }
         __syncthreads();
+        phase_done(1);
         // ---- cluster moves: slot t = nt-1-j, j = wave, wave + W, ... < nc ----
         if (d.nc > 0) {
             for (int j = wave; j < d.nc; j += W) {
@@ -362,6 +373,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
             }
             __syncthreads();
         }
+        phase_done(2);
     }
     // ---- write back ----
     for (int s = tid; s < nrep; s += W * kWave) {

@@ -48,6 +48,7 @@ struct DevArgs {
     const int2* pairs;       // (instance, slot t) per ICM wavefront
     const int32_t* order;    // fused mode: instance per workgroup
     int32_t fmt;             // adjacency record format
+    unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
     int32_t sweeps;
@@ -111,14 +112,21 @@ struct Csr {
         const int d = (int)(ri & (uint32_t)kMaxDegree);
         const uint4* r = reinterpret_cast<const uint4*>(base + (size_t)(ri >> 9) * 16u);
         if (Fmt == kFmtUnit) {
+            // All 8 gathers of a chunk are issued unconditionally (padding
+            // records read position 0) and masked afterwards, so they overlap
+            // instead of each waiting on the previous one.
             int agree = 0;  // neighbours with s_j * sign(w_pj) = +1
             for (int c = 0; 8 * c < d; ++c) {
                 const uint4 v = r[c];
                 const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+                uint32_t b[8];
+#pragma unroll
+                for (int h = 0; h < 8; ++h) b[h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                const int live = d - 8 * c;
 #pragma unroll
                 for (int h = 0; h < 8; ++h) {
-                    const uint32_t e = (wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
-                    if (8 * c + h < d) agree += (int)(sp[e & 0x7FFFu] ^ (e >> 15));
+                    const uint32_t sgn = (wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u;
+                    agree += (h < live) ? (int)(b[h] ^ sgn) : 0;
                 }
             }
             return 2 * agree - d;

@@ -103,6 +103,7 @@ _SIGS = {
     "ptx_engine_read_min_energies": (C.c_int32, [_vp, _P(C.c_int64)]),
     "ptx_engine_timing_enable": (C.c_int32, [_vp, C.c_int32]),
     "ptx_engine_timing_get": (C.c_int32, [_vp, _P(Timing)]),
+    "ptx_engine_phase_profile": (C.c_int32, [_vp, C.c_int32, _P(C.c_double)]),
     "ptx_selftest_fp64": (C.c_int32, [_P(C.c_double), C.c_int64, _P(C.c_double), _P(C.c_double)]),
     "ptx_host_fp64": (C.c_int32, [_P(C.c_double), C.c_int64, _P(C.c_double), _P(C.c_double)]),
     "ptx_selftest_philox": (C.c_int32, [_P(C.c_uint32), C.c_int64, _P(C.c_uint32), _P(C.c_uint32)]),
@@ -361,6 +362,12 @@ class Engine:
         out = np.zeros(self.ninst, np.int64)
         check(lib().ptx_engine_read_min_energies(self.h, _ptr(out, C.c_int64)))
         return out
+
+    def phase_profile(self, on=None):
+        """on=True/False resets and switches; None reads [sweep, exchange, icm] microseconds (workgroup 0)."""
+        us = np.zeros(3, np.float64)
+        check(lib().ptx_engine_phase_profile(self.h, -1 if on is None else int(on), _ptr(us, C.c_double)))
+        return us
 
     def timing(self, on=None):
         if on is not None:
