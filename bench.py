@@ -70,6 +70,38 @@ def cpu_baseline(problem_path, betas, seconds, seed=0):
                       f"{steps} PT steps in {dt:.1f} s on 1 host core"}
 
 
+def batched_ea(ptx, steps, seed, n_inst=4096):
+    """The batched workload: 4096 independent 3D Edwards-Anderson +-J instances,
+    L uniform in {6, 8, 10, 12}, 32 inverse temperatures 0.2..3.0 x 2 chains,
+    ICM by the slot rule, fused per-instance kernel."""
+    sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+    import instances
+    rng = np.random.default_rng(seed)
+    Ls = rng.choice([6, 8, 10, 12], size=n_inst)
+    probs = [ptx.Problem.from_coo(*instances.ea_3d(int(L), seed=1000 + i)) for i, L in enumerate(Ls)]
+    lad = ptx.beta_linspace(0.2, 3.0, 32, True)
+    eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
+    eng.step(16)
+    eng.sync()
+    eng.timing(True)
+    eng.timing()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    tm = eng.timing()
+    info = eng.info()
+    avg_ms = tm.fused_ms / max(tm.fused_launches, 1)
+    bpl = info.lds_bytes_per_step * info.steps_per_launch
+    return {"workload": "4096 x 3D EA +-J, L in {6,8,10,12}, 32 betas 0.2..3.0, 2 chains, ICM",
+            "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
+            "ms_per_step": dt * 1e3 / steps, "steps": steps, "instances_per_hour_at_10k_steps":
+                n_inst * 3600.0 / (dt / steps * 10000), "waves_per_group": info.waves_per_group,
+            "roofline": {"bound": "lds", "achieved": bpl / (avg_ms * 1e-3) / 1e9, "peak": LDS_PEAK_GBS,
+                         "unit": "GB/s", "frac": bpl / (avg_ms * 1e-3) / 1e9 / LDS_PEAK_GBS,
+                         "kernel": "k_fused", "avg_launch_ms": avg_ms}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +110,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", type=int, default=1, help="1 split kernels, 2 fused per-instance kernel")
+    ap.add_argument("--batched-steps", type=int, default=64,
+                    help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -91,7 +126,7 @@ def main():
     t_pre = time.perf_counter() - t0
     nt = len(betas)
     log(f"preprocess: {nt} temperatures in {t_pre:.2f} s; beta {betas[0]:.4f} .. {betas[-1]:.4f}")
-    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM)
+    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM, mode=args.mode)
     nc = eng.instance_info().nc
     eng.step(args.warmup)
     eng.sync()
@@ -105,9 +140,13 @@ def main():
     info = eng.info()
     updates = info.spin_updates_per_step * args.steps
     value = updates / dt
-    sweep_avg_ms = tm.sweep_ms / max(tm.sweep_launches, 1)
-    bytes_per_launch = info.lds_bytes_per_step
-    achieved = bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
+    if info.mode == 2:  # fused kernel: one launch covers steps_per_launch PT steps
+        kern, avg_ms = "k_fused", tm.fused_ms / max(tm.fused_launches, 1)
+        bytes_per_launch = info.lds_bytes_per_step * info.steps_per_launch
+    else:
+        kern, avg_ms = "k_sweep", tm.sweep_ms / max(tm.sweep_launches, 1)
+        bytes_per_launch = info.lds_bytes_per_step
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     be, bs, _ = eng.best(0, with_state=False)
     best = int(be.min())
     out = {
@@ -127,12 +166,16 @@ def main():
                                f"{nt} chosen), 2 chains, ICM slots {nc}, 1 sweep/step",
                    "n": p.n, "replicas": 2 * nt, "icm_slots": nc, "preprocess_s": round(t_pre, 3),
                    "best_energy": best, "best_cut": (19990 - best) // 2},
-        "kernel_ms": {"sweep": sweep_avg_ms, "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
-                      "icm": tm.icm_ms / max(tm.icm_launches, 1)},
+        "kernel_ms": {"sweep": tm.sweep_ms / max(tm.sweep_launches, 1),
+                      "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
+                      "icm": tm.icm_ms / max(tm.icm_launches, 1), "fused": tm.fused_ms / max(tm.fused_launches, 1)},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep", "bytes_per_launch": bytes_per_launch},
+                     "kernel": kern, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
     }
+    del eng
+    if args.batched_steps > 0:
+        out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
     print(json.dumps(out))

@@ -125,6 +125,7 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
 #define PTX_MODE_AUTO 0
 #define PTX_MODE_SPLIT 1 /* sweep / exchange / ICM kernels per step */
 #define PTX_MODE_FUSED 2 /* one workgroup per instance runs whole PT steps */
+#define PTX_MODE_COOP 3  /* one instance: one workgroup per replica, grid barrier per step */
 
 typedef struct {
     int32_t chains;          /* 2 for the main run (initialization.cc:166), 1 for a pilot */
@@ -136,6 +137,7 @@ typedef struct {
     int32_t waves_per_group; /* replicas per workgroup (0 = autotuned) */
     int32_t steps_per_launch;/* fused mode: PT steps per launch (0 = auto) */
     int32_t use_graph;       /* capture step sequences in a hipGraph */
+    int32_t spins_per_lane;  /* cooperative mode: 1 or 4 positions per lane (0 = auto) */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 

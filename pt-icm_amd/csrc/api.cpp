@@ -223,6 +223,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->waves_per_group = 0;
     c->steps_per_launch = 0;
     c->use_graph = 0;
+    c->spins_per_lane = 0;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -263,8 +264,8 @@ ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out) {
     out->n_instances = E.ninst;
     out->steps_done = E.steps_done;
     out->mode = E.mode;
-    out->waves_per_group = E.mode == PTX_MODE_FUSED ? E.Wf : E.W;
-    out->steps_per_launch = E.mode == PTX_MODE_FUSED ? E.K : 1;
+    out->waves_per_group = E.mode == PTX_MODE_SPLIT ? E.W : E.Wf;
+    out->steps_per_launch = E.mode == PTX_MODE_SPLIT ? 1 : E.K;
     out->total_replicas = E.total_rep;
     out->spin_updates_per_step = E.updates_per_step;
     out->lds_bytes_per_step = E.lds_bytes_per_step;

@@ -1,0 +1,322 @@
+// coop.hip -- persistent single-instance kernel: one workgroup per replica,
+// all replicas co-resident (cooperative launch), several PT steps per launch.
+//
+// Per PT step each workgroup sweeps its replica with all of its waves, then
+// publishes its spin words and energy to a double-buffered area, then waits
+// at one grid barrier.  After the barrier every workgroup computes the whole
+// (NN)_a exchange itself from the published energies (the computation is
+// deterministic, so every copy of the permutation is identical), and the two
+// workgroups of each ICM slot pair both grow the same cluster from the
+// published states and each keeps its own replica's half.  No second barrier
+// is needed: a buffer is rewritten only two steps later, after every
+// workgroup has passed the next barrier.
+#include "kernels.hpp"
+
+namespace ptx {
+
+struct CoopArgs {
+    uint32_t* pub_spins;        // [2][nrep][nw]
+    int32_t* pub_E;             // [2][nrep]
+    unsigned long long* bar;    // grid barrier counter (monotonic)
+    uint32_t* err;              // set to 1 when a barrier wait times out
+    unsigned long long bar_base;  // barriers completed before this launch
+    int32_t nblocks;
+    int32_t inst;
+};
+
+// Metropolis sweep of one replica by a whole workgroup (W waves).  Lane gl of
+// nthr takes SPL positions per step of the class loop: SPL = 1 -> position
+// cs + gl + k*nthr; SPL = 4 -> the aligned groups of 4 used by the wave-level
+// sweep.  Every position draws word p&3 of the Philox block p>>2, so the
+// result is the same for any SPL and workgroup size.  Class sums go through
+// three rotating LDS accumulators (one barrier per class).
+template <int Fmt, int SPL>
+__device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
+                                                 const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
+                                                 uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
+                                                 const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
+                                                 int gl, int nthr, int* acc) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const int lane = gl % kWave;
+    for (int c = 0; c < ncls; ++c) {
+        if (gl == 0) {  // clear the accumulator of class c+1 (last read before the previous barrier)
+            acc[2 * ((c + 1) % 3)] = 0;
+            acc[2 * ((c + 1) % 3) + 1] = 0;
+        }
+        const int cs = cls[c], ce = cls[c + 1];
+        int dsum = 0, nacc = 0;
+        if (SPL == 1) {
+            for (int p = cs + gl; p < ce; p += nthr) {
+                const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
+                const int dE = sp[p] ? -2 * f : 2 * f;
+                bool accept = dE <= 0;
+                if (!accept) {
+                    const Philox4 rnd = philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1);
+                    accept = pick(rnd, (uint32_t)p) < thr_row[(dE >> 1) - 1];
+                }
+                if (accept) {
+                    sp[p] ^= 1u;
+                    dsum += dE;
+                    ++nacc;
+                }
+            }
+        } else {
+            for (int g = (cs >> 2) + gl; 4 * g < ce; g += nthr) {
+                const int base = 4 * g;
+                int dE[4];
+                bool any_up = false;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int p = base + k;
+                    dE[k] = 0;
+                    if (p >= cs && p < ce) {
+                        const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
+                        dE[k] = sp[p] ? -2 * f : 2 * f;
+                        any_up |= dE[k] > 0;
+                    }
+                }
+                Philox4 rnd = {{0u, 0u, 0u, 0u}};
+                if (any_up) rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int p = base + k;
+                    if (p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1])) {
+                        sp[p] ^= 1u;
+                        dsum += dE[k];
+                        ++nacc;
+                    }
+                }
+            }
+        }
+        dsum = wave_sum_i32(dsum);
+        nacc = wave_sum_i32(nacc);
+        int* a = acc + 2 * (c % 3);
+        if (lane == 0 && (dsum != 0 || nacc != 0)) {
+            atomicAdd(&a[0], dsum);
+            atomicAdd(&a[1], nacc);
+        }
+        __syncthreads();
+        const int ds = a[0], na = a[1];
+        st.E += ds;
+        st.flips += (unsigned long long)na;
+        if (na > 0 && st.E < st.bestE) {
+            st.bestE = st.E;
+            st.best_sweep = (int32_t)gsweep;
+            pack_bits(sp, nw, best_words, gl, nthr);
+        }
+    }
+}
+
+// Grid barrier over all workgroups of the launch.  Returns false when the
+// wait exceeded its bound (the error flag is then set and every workgroup
+// leaves the step loop).
+__device__ __forceinline__ bool grid_barrier(unsigned long long* bar, uint32_t* err, unsigned long long target,
+                                             int* lds_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        atomicAdd(bar, 1ull);
+        unsigned long long spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicOr(err, 1u);
+                ok = 0;
+                break;
+            }
+        }
+        __threadfence();
+        *lds_flag = ok;
+    }
+    __syncthreads();
+    return *lds_flag != 0;
+}
+
+struct CoopLayout {
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, sp, icm, acc, flag, total;
+};
+
+__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax) {
+    CoopLayout L;
+    int o = 0;
+    auto take = [&](int bytes) {
+        const int at = o;
+        o += (bytes + 15) & ~15;
+        return at;
+    };
+    L.info = take(4 * n);
+    L.adj = take(16 * (adj_len16 > 0 ? adj_len16 : 1));
+    L.thr = take(4 * nt * (fmax > 0 ? fmax : 1));
+    L.beta = take(8 * nrep);
+    L.logu = take(8 * nrep);
+    L.E = take(4 * nrep);
+    L.s2r = take(4 * nrep);
+    L.r2s = take(4 * nrep);
+    L.att = take(4 * nrep);
+    L.succ = take(4 * nrep);
+    L.key = take(4 * nrep);
+    L.perm = take(4 * nrep);
+    L.tpos = take(4 * nrep);
+    L.level = take(4 * nrep);
+    const int nw = (n + 31) / 32;
+    L.sp = take(32 * nw);
+    L.icm = take(icm_scratch_bytes(n));
+    L.acc = take(4 * 6);
+    L.flag = take(4);
+    L.total = o;
+    return L;
+}
+
+template <int Fmt, int W, int SPL>
+__global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int nsteps) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int nthr = W * kWave;
+    const InstDesc d = a.desc[c.inst];
+    const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
+    const int r = blockIdx.x;  // this workgroup's replica
+    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax);
+    uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
+    uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
+    double* beta = reinterpret_cast<double*>(smem + L.beta);
+    int32_t* E = reinterpret_cast<int32_t*>(smem + L.E);
+    int32_t* s2r = reinterpret_cast<int32_t*>(smem + L.s2r);
+    int32_t* r2s = reinterpret_cast<int32_t*>(smem + L.r2s);
+    uint32_t* att = reinterpret_cast<uint32_t*>(smem + L.att);
+    uint32_t* succ = reinterpret_cast<uint32_t*>(smem + L.succ);
+    ExchScratch xs;
+    xs.logu = reinterpret_cast<double*>(smem + L.logu);
+    xs.key = reinterpret_cast<uint32_t*>(smem + L.key);
+    xs.perm = reinterpret_cast<int32_t*>(smem + L.perm);
+    xs.tpos = reinterpret_cast<int32_t*>(smem + L.tpos);
+    xs.level = reinterpret_cast<int32_t*>(smem + L.level);
+    uint8_t* sp = smem + L.sp;
+    int* acc = reinterpret_cast<int*>(smem + L.acc);
+    int* flag = reinterpret_cast<int*>(smem + L.flag);
+    IcmScratch sc;
+    sc.S1 = reinterpret_cast<uint32_t*>(smem + L.icm);
+    sc.S2 = sc.S1 + nw;
+    sc.D = sc.S2 + nw;
+    sc.V = sc.D + nw;
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw);
+    sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
+
+    // ---- stage the instance and this replica ----
+    for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
+        uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
+        for (int q = tid; q < (int)d.adj_len16; q += nthr) dst[q] = src[q];
+    }
+    for (int m = tid; m < nt * d.fmax; m += nthr) thr[m] = a.thr[d.thr_off + m];
+    const int64_t ro = d.rep_off;
+    for (int s = tid; s < nrep; s += nthr) {
+        beta[s] = a.beta[d.beta_off + s];
+        s2r[s] = a.slot2rep[ro + s];
+        r2s[s] = a.rep2slot[ro + s];
+        att[s] = 0;
+        succ[s] = 0;
+    }
+    if (tid < 6) acc[tid] = 0;
+    expand_bits(a.spins + d.spin_off + (int64_t)r * nw, nw, sp, tid, nthr);
+    __syncthreads();
+    const Csr<Fmt> csr{info, smem + L.adj};
+    const int32_t* hf = a.hfield + d.pos_off;
+    const int32_t* cls = a.cls + d.cls_off;
+    const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
+    const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
+    const uint32_t step0 = a.inst_step[c.inst];
+    const int64_t gr = ro + r;
+    uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r * nw;
+    SweepState st;
+    st.E = a.E[gr];
+    st.bestE = a.bestE[gr];
+    st.best_sweep = a.best_sweep[gr];
+    st.flips = 0;
+    unsigned long long nsw_total = 0;
+    int it = 0;
+    for (; it < nsteps; ++it) {
+        const uint32_t step = step0 + (uint32_t)it;
+        const int par = (int)(step & 1u);
+        // ---- sweep ----
+        const uint32_t* thr_row = thr + (r2s[r] % nt) * d.fmax;
+        for (int sw = 0; sw < a.sweeps; ++sw) {
+            const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
+            sweep_replica_wg<Fmt, SPL>(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, sp,
+                                       thr_row, best_words, st, tid, nthr, acc);
+        }
+        // ---- publish and wait ----
+        pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
+        if (tid == 0) c.pub_E[par * nrep + r] = st.E;
+        if (!grid_barrier(c.bar, c.err, (c.bar_base + (unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
+                          flag))
+            break;
+        for (int s = tid; s < nrep; s += nthr) E[s] = c.pub_E[par * nrep + s];
+        __syncthreads();
+        // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0) ----
+        if (wave == 0) {
+            if (r == 0) record_histogram(a, d, step, E, s2r, lane);
+            if (!a.no_exchange) {
+                nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta, E,
+                                           s2r, att, succ, xs, lane);
+                wave_sync();
+                for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
+            }
+        }
+        __syncthreads();
+        // ---- cluster move of this replica's slot pair (both partners compute it) ----
+        if (d.nc > 0) {
+            const int t = r2s[r] % nt;
+            if (t >= nt - d.nc) {
+                const int r1 = s2r[t], r2 = s2r[t + nt];
+                if (wave == 0) {
+                    long long dE1 = 0, dE2 = 0;
+                    icm_wave(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                             c.pub_spins + ((int64_t)par * nrep + r1) * nw,
+                             c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, false);
+                    if (lane == 0) st.E += (int32_t)(r == r1 ? dE1 : dE2);
+                }
+                __syncthreads();
+                expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
+                if (tid == 0) flag[0] = st.E;  // share the new energy with every wave
+                __syncthreads();
+                st.E = flag[0];
+                __syncthreads();
+            }
+        }
+    }
+    // ---- write back ----
+    pack_bits(sp, nw, a.spins + d.spin_off + (int64_t)r * nw, tid, nthr);
+    if (tid == 0) {
+        a.E[gr] = st.E;
+        a.bestE[gr] = st.bestE;
+        a.best_sweep[gr] = st.best_sweep;
+        a.flips[gr] += st.flips;
+    }
+    if (r == 0) {
+        for (int s = tid; s < nrep; s += nthr) {
+            a.slot2rep[ro + s] = s2r[s];
+            a.rep2slot[ro + s] = r2s[s];
+            a.swap_att[ro + s] += att[s];
+            a.swap_succ[ro + s] += succ[s];
+        }
+        if (wave == 0) {
+            const unsigned long long tot = (unsigned long long)wave_sum_i64((long long)nsw_total);
+            if (lane == 0) {
+                a.swaps[c.inst] += tot;
+                a.inst_step[c.inst] = step0 + (uint32_t)it;
+            }
+        }
+    }
+}
+
+#define PTX_COOP_INST(F)                                         \
+    template __global__ void k_coop<F, 4, 1>(DevArgs, CoopArgs, int);   \
+    template __global__ void k_coop<F, 8, 1>(DevArgs, CoopArgs, int);   \
+    template __global__ void k_coop<F, 4, 4>(DevArgs, CoopArgs, int);   \
+    template __global__ void k_coop<F, 8, 4>(DevArgs, CoopArgs, int);
+PTX_COOP_INST(kFmtWide)
+PTX_COOP_INST(kFmtUnit)
+
+}  // namespace ptx

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "kernels.hip"
+#include "coop.hip"
 #include "ladder.hpp"
 
 namespace ptx {
@@ -207,7 +208,27 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
 
     // ---- execution mode ----
     mode = PTX_MODE_SPLIT;
-    if (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO) {
+    if ((cfg.mode == PTX_MODE_COOP || cfg.mode == PTX_MODE_AUTO) && ninst == 1) {
+        int dev = 0, ncu = 0, coopok = 0;
+        PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+        PTX_TRY(hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
+        PTX_TRY(hip_check(hipDeviceGetAttribute(&coopok, hipDeviceAttributeCooperativeLaunch, dev), "coop attr"));
+        const CoopLayout L = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax);
+        const bool fits = coopok && L.total <= 160 * 1024 && desc[0].nrep <= ncu;
+        if (fits) {
+            mode = PTX_MODE_COOP;
+            coop_lds = L.total;
+            Wf = cfg.waves_per_group == 8 ? 8 : 4;
+            spl = cfg.spins_per_lane == 4 ? 4 : 1;
+            K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 256;
+        } else if (cfg.mode == PTX_MODE_COOP) {
+            return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode needs one instance with at most one replica "
+                                                    "per CU and its adjacency within 160 KiB of LDS");
+        }
+    } else if (cfg.mode == PTX_MODE_COOP) {
+        return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode runs a single instance");
+    }
+    if (mode == PTX_MODE_SPLIT && (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO)) {
         Wf = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
         if (Wf != 4 && Wf != 8 && Wf != 16) Wf = Wf < 6 ? 4 : (Wf < 12 ? 8 : 16);
         int lds_max = 0;
@@ -232,6 +253,12 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_desc.upload(desc, stream));
     if (mode == PTX_MODE_FUSED) PTX_TRY(d_order.upload(fused_order, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
+    if (mode == PTX_MODE_COOP) {
+        PTX_TRY(d_pub_spins.alloc((size_t)2 * desc[0].nrep * desc[0].nw));
+        PTX_TRY(d_pub_E.alloc((size_t)2 * desc[0].nrep));
+        PTX_TRY(d_bar.alloc(1));
+        PTX_TRY(d_err.alloc(1));
+    }
     PTX_TRY(d_rowinfo.upload(rowinfo, stream));
     PTX_TRY(d_adj.upload(adjw, stream));
     PTX_TRY(d_cls.upload(cls, stream));
@@ -417,9 +444,62 @@ Status Engine::launch_fused(int nsteps) {
     }
 }
 
+Status Engine::launch_coop(int nsteps) {
+    DevArgs a = build_args(*this);
+    CoopArgs c{};
+    c.pub_spins = d_pub_spins.p;
+    c.pub_E = d_pub_E.p;
+    c.bar = d_bar.p;
+    c.err = d_err.p;
+    c.bar_base = barriers_done;
+    c.nblocks = desc[0].nrep;
+    c.inst = 0;
+    const dim3 grid((unsigned)desc[0].nrep), block((unsigned)(Wf * kWave));
+    const unsigned lds = (unsigned)coop_lds;
+    auto go = [&](auto kern) -> Status {
+        const void* f = reinterpret_cast<const void*>(kern);
+        if (lds > 64 * 1024)
+            PTX_TRY(hip_check(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                              "hipFuncSetAttribute"));
+        void* args[] = {&a, &c, &nsteps};
+        Timed t;
+        if (timing) {
+            t.kind = 3;
+            PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
+            PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
+            PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
+        }
+        PTX_TRY(hip_check(hipLaunchCooperativeKernel(f, grid, block, args, lds, stream), "k_coop launch"));
+        if (timing) {
+            PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
+            pending.push_back(t);
+        }
+        return Status::ok();
+    };
+    Status st;
+    if (fmt == kFmtUnit) {
+        if (Wf == 8) st = spl == 4 ? go(k_coop<kFmtUnit, 8, 4>) : go(k_coop<kFmtUnit, 8, 1>);
+        else st = spl == 4 ? go(k_coop<kFmtUnit, 4, 4>) : go(k_coop<kFmtUnit, 4, 1>);
+    } else {
+        if (Wf == 8) st = spl == 4 ? go(k_coop<kFmtWide, 8, 4>) : go(k_coop<kFmtWide, 8, 1>);
+        else st = spl == 4 ? go(k_coop<kFmtWide, 4, 4>) : go(k_coop<kFmtWide, 4, 1>);
+    }
+    if (st) barriers_done += (unsigned long long)nsteps;
+    return st;
+}
+
 Status Engine::step(int64_t n) {
     if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
     if (steps_done + n > 0xFFFFFFFFll) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^32 PT steps");
+    if (mode == PTX_MODE_COOP) {
+        while (n > 0) {
+            const int k = (int)std::min<int64_t>(n, K);
+            PTX_TRY(launch_coop(k));
+            steps_done += k;
+            n -= k;
+        }
+        return Status::ok();
+    }
     if (mode == PTX_MODE_FUSED) {
         while (n > 0) {
             const int k = (int)std::min<int64_t>(n, K);
@@ -436,7 +516,15 @@ Status Engine::step(int64_t n) {
     return Status::ok();
 }
 
-Status Engine::sync() { return hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+Status Engine::sync() {
+    PTX_TRY(hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"));
+    if (mode == PTX_MODE_COOP && d_err.p) {
+        uint32_t err = 0;
+        PTX_TRY(hip_check(hipMemcpy(&err, d_err.p, sizeof(err), hipMemcpyDeviceToHost), "read error flag"));
+        if (err) return Status::err(PTX_ERR_DEVICE, "cooperative grid barrier timed out");
+    }
+    return Status::ok();
+}
 
 Status Engine::collect_timing() {
     PTX_TRY(sync());

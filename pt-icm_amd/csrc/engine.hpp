@@ -74,6 +74,13 @@ struct Engine {
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
     DevBuf<unsigned long long> d_phase;
+    // cooperative mode
+    DevBuf<uint32_t> d_pub_spins, d_err;
+    DevBuf<int32_t> d_pub_E;
+    DevBuf<unsigned long long> d_bar;
+    unsigned long long barriers_done = 0;
+    int32_t spl = 1;             // positions per lane (cooperative mode)
+    int32_t coop_lds = 0;
     bool phase_on = false;
 
     ~Engine();
@@ -97,6 +104,7 @@ struct Engine {
     Status launch_init(int mode);
     Status launch_step(uint32_t s);
     Status launch_fused(int nsteps);
+    Status launch_coop(int nsteps);
     Status collect_timing();
 };
 

@@ -372,7 +372,8 @@ __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bo
 // disagreement set, V the cluster; Q is a queue of up to n/2 positions.  The
 // cluster is grown breadth-first from a uniform seed in D; the reference grows
 // it depth-first (icm.cc:37-72), which gives the same component.  Returns the
-// energy changes of both replicas; the new states are written to g1/g2.
+// energy changes of both replicas; the new states stay in S1/S2 and are
+// written to g1/g2 unless write_back is false.
 // ---------------------------------------------------------------------------
 struct IcmScratch {
     uint32_t* S1;
@@ -385,8 +386,12 @@ struct IcmScratch {
 
 template <int Fmt>
 __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf, bool has_field, int n, int nw,
-                                         int t, uint32_t step, uint64_t seed, uint32_t c3, uint32_t* g1, uint32_t* g2,
-                                         IcmScratch sc, long long& dE1_out, long long& dE2_out, int lane) {
+                                         int t, uint32_t step, uint64_t seed, uint32_t c3, const uint32_t* g1_in,
+                                         const uint32_t* g2_in, IcmScratch sc, long long& dE1_out, long long& dE2_out,
+                                         int lane, bool write_back = true) {
+    // the new states are left in sc.S1/sc.S2 and, with write_back, stored to g1/g2
+    uint32_t* g1 = const_cast<uint32_t*>(g1_in);
+    uint32_t* g2 = const_cast<uint32_t*>(g2_in);
     uint32_t* S1 = sc.S1;
     uint32_t* S2 = sc.S2;
     uint32_t* D = sc.D;
@@ -523,9 +528,11 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
     }
     dE1_out = wave_sum_i64(dE1);
     dE2_out = wave_sum_i64(dE2);
-    for (int w = lane; w < nw; w += kWave) {
-        g1[w] = S1[w];
-        g2[w] = S2[w];
+    if (write_back) {
+        for (int w = lane; w < nw; w += kWave) {
+            g1[w] = S1[w];
+            g2[w] = S2[w];
+        }
     }
 }
 

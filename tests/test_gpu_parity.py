@@ -95,7 +95,8 @@ def test_device_fp64_bitwise(ptx, oracle):
 SWEEP_CASES = ["lattice_2d_5.gset", "lattice_2d_20.gset", "pm_torus20", "G022", "field_random", "ea3d_6"]
 
 
-MODES = [1, 2]  # split kernels, fused per-instance kernel
+MODES = [1, 2, 3]  # split kernels, fused per-instance kernel, cooperative per-replica kernel
+BATCH_MODES = [1, 2]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -169,12 +170,15 @@ def test_icm_all_slots_parity(ptx, oracle, mode):
         compare(eng, oe, what=f"step {s}")
 
 
-@pytest.mark.parametrize("mode,wpg,spl", [(1, 1, 0), (1, 2, 0), (1, 4, 0), (1, 8, 0), (2, 4, 1), (2, 8, 7),
-                                          (2, 16, 16)])
-def test_launch_shape_independence(ptx, oracle, mode, wpg, spl):
+@pytest.mark.parametrize("mode,wpg,spl,lanes", [(1, 1, 0, 0), (1, 2, 0, 0), (1, 4, 0, 0), (1, 8, 0, 0), (2, 4, 1, 0),
+                                                (2, 8, 7, 0), (2, 16, 16, 0), (3, 4, 5, 1), (3, 8, 16, 1),
+                                                (3, 4, 1, 4), (3, 8, 4, 4)])
+def test_launch_shape_independence(ptx, oracle, mode, wpg, spl, lanes):
     p, g = make_pair(ptx, oracle, "pm_torus20")
     betas = oracle.beta_linspace(0.5, 3.0, 16, 0)
-    eng = ptx.Engine(p, betas, seed=9, flags=ptx.FLAG_ICM, waves_per_group=wpg, mode=mode, steps_per_launch=spl)
+    eng = ptx.Engine(p, betas, seed=9, flags=ptx.FLAG_ICM, waves_per_group=wpg, mode=mode, steps_per_launch=spl,
+                     spins_per_lane=lanes)
+    assert eng.info().mode == mode
     oe = oracle.Engine(g, betas, chains=2, seed=9, flags=oracle.FLAG_ICM)
     eng.step(15)
     oe.step(15)
@@ -212,7 +216,7 @@ def test_pilot_histograms_match_oracle_logs(ptx, oracle, mode):
 
 
 # ---------------------------------------------------------------- batched ---
-@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("mode", BATCH_MODES)
 @pytest.mark.parametrize("with_field", [False, True])
 def test_batched_instances_match_single_oracles(ptx, oracle, mode, with_field):
     kinds = ["lattice_2d_10.gset", "ea3d_6", "pm_torus20", "lattice_2d_5.gset", "er_small"]
