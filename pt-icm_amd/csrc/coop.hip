@@ -35,13 +35,16 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
                                                  const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
                                                  uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
                                                  const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
-                                                 int gl, int nthr, int* acc) {
+                                                 int gl, int nthr, int* acc, uint32_t& cc) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int lane = gl % kWave;
-    for (int c = 0; c < ncls; ++c) {
-        if (gl == 0) {  // clear the accumulator of class c+1 (last read before the previous barrier)
-            acc[2 * ((c + 1) % 3)] = 0;
-            acc[2 * ((c + 1) % 3) + 1] = 0;
+    for (int c = 0; c < ncls; ++c, ++cc) {
+        // cc counts classes across sweeps and steps: class cc sums into
+        // acc[cc % 3] and clears acc[(cc + 1) % 3], which was last read before
+        // the previous class's barrier and is next written after this one's.
+        if (gl == 0) {
+            acc[2 * ((cc + 1) % 3)] = 0;
+            acc[2 * ((cc + 1) % 3) + 1] = 0;
         }
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
@@ -90,7 +93,7 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
         }
         dsum = wave_sum_i32(dsum);
         nacc = wave_sum_i32(nacc);
-        int* a = acc + 2 * (c % 3);
+        int* a = acc + 2 * (cc % 3);
         if (lane == 0 && (dsum != 0 || nacc != 0)) {
             atomicAdd(&a[0], dsum);
             atomicAdd(&a[1], nacc);
@@ -235,6 +238,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
     unsigned long long nsw_total = 0;
+    uint32_t cc = 0;  // running class counter of the accumulator rotation (acc[] is zero at start)
     int it = 0;
     for (; it < nsteps; ++it) {
         const uint32_t step = step0 + (uint32_t)it;
@@ -244,7 +248,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         for (int sw = 0; sw < a.sweeps; ++sw) {
             const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
             sweep_replica_wg<Fmt, SPL>(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, sp,
-                                       thr_row, best_words, st, tid, nthr, acc);
+                                       thr_row, best_words, st, tid, nthr, acc, cc);
         }
         // ---- publish and wait ----
         pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
