@@ -106,6 +106,25 @@ struct Csr {
             w = (int32_t)r >> 16;
         }
     }
+    // Records of 16-byte chunk ch of a row (padding records read position 0).
+    __device__ __forceinline__ void chunk(const uint8_t* row, int ch, int* c, int* w) const {
+        const uint4 v = reinterpret_cast<const uint4*>(row)[ch];
+        const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+        if (Fmt == kFmtUnit) {
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t e = (wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
+                c[h] = (int)(e & 0x7FFFu);
+                w[h] = (e & 0x8000u) ? -1 : 1;
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                c[h] = (int)(wv[h] & 0xFFFFu);
+                w[h] = (int32_t)wv[h] >> 16;
+            }
+        }
+    }
     // sum_j w_pj s_j with s = 2b - 1, from byte spins
     __device__ __forceinline__ int field(int p, const uint8_t* sp) const {
         const uint32_t ri = info[p];
@@ -466,51 +485,90 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
         if (lane == 0) {
             V[seed_pos >> 5] |= 1u << (seed_pos & 31);
             sc.Q[0] = (uint16_t)seed_pos;
-            sc.tail[0] = 1;
         }
         wave_sync();
-        int head = 0;
-        while (true) {
-            const int tl = sc.tail[0];
-            wave_sync();
-            if (head >= tl) break;
+        // Breadth-first growth.  Each lane takes one queued site and walks its
+        // row a 16-byte chunk at a time: the D and V words of the chunk's
+        // neighbours are read together, candidates are claimed with atomicOr
+        // on V, and the newly claimed sites are appended at wave-scanned
+        // offsets (the queue tail is a register: only this wave appends).
+        constexpr int K = Csr<Fmt>::kPerChunk;
+        int head = 0, tail = 1;
+        while (head < tail) {
+            const int tl = tail;
             const int idx = head + lane;
-            if (idx < tl) {
-                const int p = sc.Q[idx];
-                const uint8_t* row = csr.row(p);
-                const int dg = csr.deg(p);
-                for (int q = 0; q < dg; ++q) {
-                    int j, w;
-                    csr.entry(row, q, j, w);
-                    const uint32_t bit = 1u << (j & 31);
-                    if ((D[j >> 5] & bit) && !(V[j >> 5] & bit)) {
-                        const uint32_t old = atomicOr(&V[j >> 5], bit);
-                        if (!(old & bit)) {
-                            const int pos = atomicAdd(&sc.tail[0], 1);
-                            sc.Q[pos] = (uint16_t)j;
-                        }
+            const bool act = idx < tl;
+            const int p = act ? (int)sc.Q[idx] : 0;
+            const int dg = act ? csr.deg(p) : 0;
+            const uint8_t* row = csr.row(p);
+            const int nch = (dg + K - 1) / K;
+            const int maxch = wave_max_i32(nch);
+            for (int ch = 0; ch < maxch; ++ch) {
+                const int live = ch < nch ? min(K, dg - ch * K) : 0;
+                int col[K];
+                uint32_t cand = 0;
+                if (live > 0) {
+                    int wgt[K];
+                    csr.chunk(row, ch, col, wgt);
+#pragma unroll
+                    for (int h = 0; h < K; ++h) {
+                        const int j = col[h];
+                        const uint32_t bit = 1u << (j & 31);
+                        const bool c = h < live && (D[j >> 5] & bit) && !(V[j >> 5] & bit);
+                        cand |= (c ? 1u : 0u) << h;
                     }
                 }
+                uint32_t claimed = 0;
+#pragma unroll
+                for (int h = 0; h < K; ++h) {
+                    if ((cand >> h) & 1u) {
+                        const int j = col[h];
+                        const uint32_t bit = 1u << (j & 31);
+                        if (!(atomicOr(&V[j >> 5], bit) & bit)) claimed |= 1u << h;
+                    }
+                }
+                const int cnt_new = __popc(claimed);
+                int incl = cnt_new;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const int y = __shfl_up(incl, o, kWave);
+                    if (lane >= o) incl += y;
+                }
+                const int total = __shfl(incl, kWave - 1, kWave);
+                int pos = tail + incl - cnt_new;
+#pragma unroll
+                for (int h = 0; h < K; ++h)
+                    if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
+                tail += total;
             }
             head = (tl - head > kWave) ? head + kWave : tl;
             wave_sync();
         }
-        const int csize = sc.tail[0];
+        const int csize = tail;
         // energy change of flipping the cluster C in each replica: boundary
         // edges (p in C, j not in C) and the field terms of C
         for (int idx = lane; idx < csize; idx += kWave) {
             const int p = sc.Q[idx];
             const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
             const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
-            long long b1 = 0, b2 = 0;
+            int b1 = 0, b2 = 0;
             const uint8_t* row = csr.row(p);
             const int dg = csr.deg(p);
-            for (int q = 0; q < dg; ++q) {
-                int j, w;
-                csr.entry(row, q, j, w);
-                if ((V[j >> 5] >> (j & 31)) & 1u) continue;
-                b1 += (long long)w * (((S1[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
-                b2 += (long long)w * (((S2[j >> 5] >> (j & 31)) & 1u) ? 1 : -1);
+            for (int ch = 0; K * ch < dg; ++ch) {
+                int col[K], wgt[K];
+                csr.chunk(row, ch, col, wgt);
+                const int live = min(K, dg - ch * K);
+#pragma unroll
+                for (int h = 0; h < K; ++h) {
+                    const int j = col[h];
+                    const uint32_t sh = (uint32_t)(j & 31);
+                    const uint32_t vin = (V[j >> 5] >> sh) & 1u;
+                    const int a1 = ((S1[j >> 5] >> sh) & 1u) ? wgt[h] : -wgt[h];
+                    const int a2 = ((S2[j >> 5] >> sh) & 1u) ? wgt[h] : -wgt[h];
+                    const bool use = h < live && !vin;
+                    b1 += use ? a1 : 0;
+                    b2 += use ? a2 : 0;
+                }
             }
             dE1 += -2ll * s1 * b1;
             dE2 += -2ll * s2 * b2;
