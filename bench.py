@@ -110,7 +110,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", type=int, default=1, help="1 split kernels, 2 fused per-instance kernel")
+    ap.add_argument("--mode", type=int, default=0,
+                    help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
     args = ap.parse_args()
@@ -126,7 +127,7 @@ def main():
     t_pre = time.perf_counter() - t0
     nt = len(betas)
     log(f"preprocess: {nt} temperatures in {t_pre:.2f} s; beta {betas[0]:.4f} .. {betas[-1]:.4f}")
-    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM, mode=args.mode)
+    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM, mode=args.mode, steps_per_launch=500)
     nc = eng.instance_info().nc
     eng.step(args.warmup)
     eng.sync()
@@ -140,13 +141,15 @@ def main():
     info = eng.info()
     updates = info.spin_updates_per_step * args.steps
     value = updates / dt
-    if info.mode == 2:  # fused kernel: one launch covers steps_per_launch PT steps
-        kern, avg_ms = "k_fused", tm.fused_ms / max(tm.fused_launches, 1)
+    if info.mode in (2, 3):  # one launch covers steps_per_launch PT steps
+        kern = "k_fused" if info.mode == 2 else "k_coop"
+        avg_ms = tm.fused_ms / max(tm.fused_launches, 1)
         bytes_per_launch = info.lds_bytes_per_step * info.steps_per_launch
     else:
         kern, avg_ms = "k_sweep", tm.sweep_ms / max(tm.sweep_launches, 1)
         bytes_per_launch = info.lds_bytes_per_step
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    lds_peak = max(ptx.lds_bandwidth_gbs() for _ in range(3))  # pinned by the microbenchmark
     be, bs, _ = eng.best(0, with_state=False)
     best = int(be.min())
     out = {
@@ -169,13 +172,17 @@ def main():
         "kernel_ms": {"sweep": tm.sweep_ms / max(tm.sweep_launches, 1),
                       "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
                       "icm": tm.icm_ms / max(tm.icm_launches, 1), "fused": tm.fused_ms / max(tm.fused_launches, 1)},
-        "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / LDS_PEAK_GBS, "traffic": None,
-                     "kernel": kern, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+        "roofline": {"bound": "lds", "achieved": achieved, "peak": lds_peak, "unit": "GB/s",
+                     "frac": achieved / lds_peak, "traffic": None, "peak_nominal": LDS_PEAK_GBS,
+                     "peak_source": "k_lds_read microbenchmark (16-byte LDS reads, all CUs)",
+                     "kernel": kern, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+                     "steps_per_launch": info.steps_per_launch},
     }
     del eng
     if args.batched_steps > 0:
         out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
+        out["batched"]["roofline"]["peak"] = lds_peak
+        out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
     print(json.dumps(out))

@@ -211,6 +211,11 @@ ptx_status ptx_selftest_fp64(const double* x, int64_t n, double* exp_out, double
 ptx_status ptx_host_fp64(const double* x, int64_t n, double* exp_out, double* log_out);
 /* Device Philox4x32-10 for counters ctr[n][4] under key[2] -> out[n][4]. */
 ptx_status ptx_selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out);
+/* Microbenchmarks that pin the roofline peaks: a device copy of nbytes
+ * (k_copy16; also calibrates memory-traffic counters) and an LDS read
+ * bandwidth kernel (k_lds_read, 16-byte reads, 256 threads per block). */
+ptx_status ptx_selftest_copy(int64_t nbytes, int32_t reps, double* ms_per_copy);
+ptx_status ptx_selftest_lds_bandwidth(int32_t blocks, int32_t iters, double* gbs);
 
 #ifdef __cplusplus
 }

@@ -17,6 +17,8 @@
 namespace ptx {
 Status selftest_fp64(const double* x, int64_t n, double* eo, double* lo);
 Status selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out);
+Status selftest_copy(int64_t nbytes, int32_t reps, double* ms_per_copy);
+Status selftest_lds(int32_t blocks, int32_t iters, double* gbs);
 }  // namespace ptx
 
 struct ptx_problem {
@@ -353,6 +355,16 @@ ptx_status ptx_host_fp64(const double* x, int64_t n, double* exp_out, double* lo
         log_out[i] = ptx::plog(x[i]);
     }
     return PTX_OK;
+}
+
+ptx_status ptx_selftest_copy(int64_t nbytes, int32_t reps, double* ms_per_copy) {
+    if (!ms_per_copy) return fail(PTX_ERR_INVALID, "null buffer");
+    return guarded([&]() { return done(ptx::selftest_copy(nbytes, reps, ms_per_copy)); });
+}
+
+ptx_status ptx_selftest_lds_bandwidth(int32_t blocks, int32_t iters, double* gbs) {
+    if (!gbs) return fail(PTX_ERR_INVALID, "null buffer");
+    return guarded([&]() { return done(ptx::selftest_lds(blocks, iters, gbs)); });
 }
 
 ptx_status ptx_selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out) {

@@ -703,6 +703,52 @@ Status selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint
     return hip_check(hipMemcpy(out, dout.p, n * 16, hipMemcpyDeviceToHost), "copy");
 }
 
+Status selftest_copy(int64_t nbytes, int32_t reps, double* ms_per_copy) {
+    const int64_t n16 = nbytes / 16;
+    if (n16 <= 0 || reps <= 0) return Status::err(PTX_ERR_INVALID, "copy size and repetitions must be positive");
+    DevBuf<uint32_t> a, b;
+    PTX_TRY(a.alloc((size_t)n16 * 4));
+    PTX_TRY(b.alloc((size_t)n16 * 4));
+    hipEvent_t e0, e1;
+    PTX_TRY(hip_check(hipEventCreate(&e0), "event"));
+    PTX_TRY(hip_check(hipEventCreate(&e1), "event"));
+    const unsigned grid = (unsigned)std::min<int64_t>((n16 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, 0, reinterpret_cast<const uint4*>(a.p),
+                       reinterpret_cast<uint4*>(b.p), n16);
+    PTX_TRY(hip_check(hipEventRecord(e0, 0), "event"));
+    for (int32_t r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, 0, reinterpret_cast<const uint4*>(a.p),
+                           reinterpret_cast<uint4*>(b.p), n16);
+    PTX_TRY(hip_check(hipEventRecord(e1, 0), "event"));
+    PTX_TRY(hip_check(hipEventSynchronize(e1), "event"));
+    float ms = 0;
+    PTX_TRY(hip_check(hipEventElapsedTime(&ms, e0, e1), "event"));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = ms / reps;
+    return Status::ok();
+}
+
+Status selftest_lds(int32_t blocks, int32_t iters, double* gbs) {
+    if (blocks <= 0 || iters <= 0) return Status::err(PTX_ERR_INVALID, "blocks and iterations must be positive");
+    DevBuf<uint32_t> o;
+    PTX_TRY(o.alloc((size_t)blocks));
+    hipEvent_t e0, e1;
+    PTX_TRY(hip_check(hipEventCreate(&e0), "event"));
+    PTX_TRY(hip_check(hipEventCreate(&e1), "event"));
+    hipLaunchKernelGGL(k_lds_read, dim3(blocks), dim3(256), 0, 0, o.p, iters);
+    PTX_TRY(hip_check(hipEventRecord(e0, 0), "event"));
+    hipLaunchKernelGGL(k_lds_read, dim3(blocks), dim3(256), 0, 0, o.p, iters);
+    PTX_TRY(hip_check(hipEventRecord(e1, 0), "event"));
+    PTX_TRY(hip_check(hipEventSynchronize(e1), "event"));
+    float ms = 0;
+    PTX_TRY(hip_check(hipEventElapsedTime(&ms, e0, e1), "event"));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = (double)blocks * 256.0 * 16.0 * iters / (ms * 1e-3) / 1e9;
+    return Status::ok();
+}
+
 template struct DevBuf<uint32_t>;
 template struct DevBuf<uint16_t>;
 template struct DevBuf<unsigned long long>;

@@ -415,6 +415,30 @@ __global__ void k_selftest_philox(const uint32_t* ctr, int64_t n, uint32_t k0, u
     }
 }
 
+// Plain copy of n 16-byte units (calibrates the memory-traffic counters).
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// LDS read bandwidth: every lane reads 16 bytes per iteration from a 32 KiB
+// buffer (consecutive lanes, consecutive 16-byte slots) and folds them into
+// one word so the loads are kept.  Bytes read = blocks * 256 * 16 * iters.
+__global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters) {
+    __shared__ uint4 buf[2048];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 2048; i += 256) buf[i] = make_uint4(i, i + 1, i + 2, i + 3);
+    __syncthreads();
+    uint32_t acc = 0;
+    int idx = tid;
+    for (int it = 0; it < iters; ++it) {
+        const uint4 v = buf[idx];
+        acc ^= v.x + v.y + v.z + v.w;
+        idx = (idx + 256) & 2047;
+    }
+    if (acc == 0x12345678u) out[blockIdx.x] = acc;
+}
+
 // explicit instantiations used by the host launcher
 #define PTX_SPLIT_INST(F)                                          \
     template __global__ void k_init<F, 1>(DevArgs, int);          \

@@ -1,0 +1,44 @@
+"""Microbenchmarks that pin the roofline peaks used by bench.py.
+
+    python pt-icm_amd/microbench.py lds      # LDS read GB/s (16-byte reads)
+    python pt-icm_amd/microbench.py copy     # device copy GB/s (read + write)
+    python pt-icm_amd/microbench.py copy --mib 256 --reps 5   # for counter calibration
+
+The copy kernel (k_copy16) also calibrates the FETCH_SIZE / WRITE_SIZE
+counters: a run of it under rocprofv3 --pmc reads back known byte counts.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_ptx():
+    spec = importlib.util.spec_from_file_location("ptx", os.path.join(HERE, "ptx.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["ptx"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["lds", "copy"])
+    ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--reps", type=int, default=20)
+    a = ap.parse_args()
+    ptx = load_ptx()
+    if a.what == "lds":
+        gbs = max(ptx.lds_bandwidth_gbs() for _ in range(3))
+        print(json.dumps({"lds_read_gbs": gbs}))
+    else:
+        nbytes = a.mib << 20
+        ms = ptx.copy_ms(nbytes, a.reps)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "gbs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}))
+
+
+if __name__ == "__main__":
+    main()

@@ -107,6 +107,8 @@ _SIGS = {
     "ptx_selftest_fp64": (C.c_int32, [_P(C.c_double), C.c_int64, _P(C.c_double), _P(C.c_double)]),
     "ptx_host_fp64": (C.c_int32, [_P(C.c_double), C.c_int64, _P(C.c_double), _P(C.c_double)]),
     "ptx_selftest_philox": (C.c_int32, [_P(C.c_uint32), C.c_int64, _P(C.c_uint32), _P(C.c_uint32)]),
+    "ptx_selftest_copy": (C.c_int32, [C.c_int64, C.c_int32, _P(C.c_double)]),
+    "ptx_selftest_lds_bandwidth": (C.c_int32, [C.c_int32, C.c_int32, _P(C.c_double)]),
 }
 
 
@@ -400,3 +402,17 @@ def device_philox(ctr, key):
     out = np.zeros_like(ctr)
     check(lib().ptx_selftest_philox(_ptr(ctr, C.c_uint32), len(ctr), _ptr(key, C.c_uint32), _ptr(out, C.c_uint32)))
     return out
+
+
+def copy_ms(nbytes, reps=20):
+    """Average milliseconds of the k_copy16 kernel copying nbytes."""
+    out = C.c_double()
+    check(lib().ptx_selftest_copy(int(nbytes), int(reps), C.byref(out)))
+    return out.value
+
+
+def lds_bandwidth_gbs(blocks=4096, iters=4096):
+    """Aggregate LDS read bandwidth (GB/s) of the k_lds_read microbenchmark."""
+    out = C.c_double()
+    check(lib().ptx_selftest_lds_bandwidth(int(blocks), int(iters), C.byref(out)))
+    return out.value
