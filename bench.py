@@ -25,6 +25,10 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 GRAPHS = os.path.join(REPO, "tests", "golden", "graphs")
 LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
+# HBM bytes per 16-step k_fused launch of the batched workload, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE runs of `microbench.py batched`, scaled by
+# the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1).
+BATCHED_HBM_BYTES_PER_LAUNCH = (2 * 58156.7 + 37864.6) * 1024
 
 
 def load_ptx():
@@ -76,9 +80,8 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
     ICM by the slot rule, fused per-instance kernel."""
     sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
     import instances
-    rng = np.random.default_rng(seed)
-    Ls = rng.choice([6, 8, 10, 12], size=n_inst)
-    probs = [ptx.Problem.from_coo(*instances.ea_3d(int(L), seed=1000 + i)) for i, L in enumerate(Ls)]
+    _, coos = instances.ea_batch(n_inst, seed)
+    probs = [ptx.Problem.from_coo(*c) for c in coos]
     lad = ptx.beta_linspace(0.2, 3.0, 32, True)
     eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
     eng.step(16)
@@ -99,13 +102,14 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
                 n_inst * 3600.0 / (dt / steps * 10000), "waves_per_group": info.waves_per_group,
             "roofline": {"bound": "lds", "achieved": bpl / (avg_ms * 1e-3) / 1e9, "peak": LDS_PEAK_GBS,
                          "unit": "GB/s", "frac": bpl / (avg_ms * 1e-3) / 1e9 / LDS_PEAK_GBS,
-                         "kernel": "k_fused", "avg_launch_ms": avg_ms}}
+                         "traffic": BATCHED_HBM_BYTES_PER_LAUNCH if info.steps_per_launch == 16 else None,
+                         "kernel": "k_fused", "avg_launch_ms": avg_ms, "steps_per_launch": info.steps_per_launch}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -424,19 +424,28 @@ __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
 // LDS read bandwidth: every lane reads 16 bytes per iteration from a 32 KiB
 // buffer (consecutive lanes, consecutive 16-byte slots) and folds them into
 // one word so the loads are kept.  Bytes read = blocks * 256 * 16 * iters.
-__global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters) {
+__global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint32_t salt, int stride) {
+    // 16-byte LDS reads at a runtime stride: the compiler cannot fold the
+    // loop, and 4 independent accumulators keep 4 reads in flight per lane.
     __shared__ uint4 buf[2048];
     const int tid = threadIdx.x;
-    for (int i = tid; i < 2048; i += 256) buf[i] = make_uint4(i, i + 1, i + 2, i + 3);
+    for (int i = tid; i < 2048; i += 256) buf[i] = make_uint4(i * salt, i + salt, i ^ salt, i);
     __syncthreads();
-    uint32_t acc = 0;
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     int idx = tid;
-    for (int it = 0; it < iters; ++it) {
-        const uint4 v = buf[idx];
-        acc ^= v.x + v.y + v.z + v.w;
-        idx = (idx + 256) & 2047;
+    for (int it = 0; it < iters; it += 4) {
+        const uint4 v0 = buf[idx];
+        const uint4 v1 = buf[(idx + stride) & 2047];
+        const uint4 v2 = buf[(idx + 2 * stride) & 2047];
+        const uint4 v3 = buf[(idx + 3 * stride) & 2047];
+        a0 += v0.x + v1.y;
+        a1 += v1.x + v2.y;
+        a2 += v2.x + v3.y + v0.z + v2.w;
+        a3 += v3.x + v0.y + v1.z + v3.w + v2.z + v3.z + v0.w + v1.w;
+        idx = (idx + 4 * stride) & 2047;
     }
-    if (acc == 0x12345678u) out[blockIdx.x] = acc;
+    const uint32_t acc = a0 ^ a1 ^ a2 ^ a3;
+    if (acc == salt) out[blockIdx.x] = acc;
 }
 
 // explicit instantiations used by the host launcher

@@ -67,6 +67,13 @@ def erdos_renyi(n, p, seed=0, pm=False):
     return n, e[:, 0], e[:, 1], w
 
 
+def ea_batch(n_inst, seed=0, sizes=(6, 8, 10, 12)):
+    """The batched workload: n_inst 3D EA +-J instances, L drawn uniformly
+    from `sizes`; instance i uses seed 1000 + i.  Returns (Ls, [coo, ...])."""
+    Ls = np.random.default_rng(seed).choice(list(sizes), size=n_inst)
+    return Ls, [ea_3d(int(L), seed=1000 + i) for i, L in enumerate(Ls)]
+
+
 def write_gset(path, n, src, dst, w):
     with open(path, "w") as f:
         f.write(f"{n} {len(src)}\n")

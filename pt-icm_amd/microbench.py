@@ -3,6 +3,7 @@
     python pt-icm_amd/microbench.py lds      # LDS read GB/s (16-byte reads)
     python pt-icm_amd/microbench.py copy     # device copy GB/s (read + write)
     python pt-icm_amd/microbench.py copy --mib 256 --reps 5   # for counter calibration
+    python pt-icm_amd/microbench.py batched --steps 32        # 4096 3D EA instances, fused kernel
 
 The copy kernel (k_copy16) also calibrates the FETCH_SIZE / WRITE_SIZE
 counters: a run of it under rocprofv3 --pmc reads back known byte counts.
@@ -26,12 +27,33 @@ def load_ptx():
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["lds", "copy"])
+    ap.add_argument("what", choices=["lds", "copy", "batched"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--instances", type=int, default=4096)
     a = ap.parse_args()
     ptx = load_ptx()
-    if a.what == "lds":
+    if a.what == "batched":
+        import time
+        sys.path.insert(0, HERE)
+        import instances
+        _, coos = instances.ea_batch(a.instances, 0)
+        probs = [ptx.Problem.from_coo(*c) for c in coos]
+        lad = ptx.beta_linspace(0.2, 3.0, 32, True)
+        eng = ptx.Engine(probs, [lad] * len(probs), seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
+        eng.step(8)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.step(a.steps)
+        eng.sync()
+        dt = time.perf_counter() - t0
+        info = eng.info()
+        print(json.dumps({"instances": len(probs), "steps": a.steps, "ms_per_step": dt * 1e3 / a.steps,
+                          "spin_updates_per_s": info.spin_updates_per_step * a.steps / dt,
+                          "lds_bytes_per_step": info.lds_bytes_per_step,
+                          "steps_per_launch": info.steps_per_launch}))
+    elif a.what == "lds":
         gbs = max(ptx.lds_bandwidth_gbs() for _ in range(3))
         print(json.dumps({"lds_read_gbs": gbs}))
     else:
