@@ -217,6 +217,41 @@ ptx_status ptx_selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* k
 ptx_status ptx_selftest_copy(int64_t nbytes, int32_t reps, double* ms_per_copy);
 ptx_status ptx_selftest_lds_bandwidth(int32_t blocks, int32_t iters, double* gbs);
 
+/* ------------------------------------------------------------------ */
+/* Ladder surrogate: a fully connected network (ReLU hidden layers,   */
+/* softplus outputs) trained and evaluated with BF16 MFMA kernels and */
+/* FP32 accumulation.  No counterpart in the reference: the ladder it */
+/* predicts stands in for the one get_beta (src/cae_estimator.cc:     */
+/* 295-428) computes from a long pilot run, which remains the         */
+/* fallback.  Parameters are flat FP32: per layer W[in][out] (row-    */
+/* major) then b[out].  An MLP owns one HIP stream; not thread-safe.  */
+/* ------------------------------------------------------------------ */
+typedef struct ptx_mlp ptx_mlp;
+/* dims[0..n_dims-1]: layer widths (input first); weights He-uniform from seed. */
+ptx_status ptx_mlp_create(const int32_t* dims, int32_t n_dims, int32_t max_batch, uint64_t seed, ptx_mlp** out);
+void ptx_mlp_destroy(ptx_mlp* m);
+ptx_status ptx_mlp_param_count(const ptx_mlp* m, int64_t* count);
+ptx_status ptx_mlp_set_params(ptx_mlp* m, const float* params, int64_t count);
+ptx_status ptx_mlp_get_params(ptx_mlp* m, float* params, int64_t count);
+/* Gradients of the last ptx_mlp_loss_grad (same layout as the parameters). */
+ptx_status ptx_mlp_get_grads(ptx_mlp* m, float* grads, int64_t count);
+/* y[batch][out] = softplus(network(x[batch][in])). */
+ptx_status ptx_mlp_forward(ptx_mlp* m, const float* x, int32_t batch, float* y);
+/* Loss = (1/batch) sum over outputs of: mode 1 (y - t)^2, mode 2
+ * max(0, t - y)^2, mode 0 nothing; computes all gradients on the device. */
+ptx_status ptx_mlp_loss_grad(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int32_t batch,
+                             double* loss);
+/* One Adam step with the gradients of the last loss_grad. */
+ptx_status ptx_mlp_adam(ptx_mlp* m, float lr, float beta1, float beta2, float eps);
+/* Minibatch training with the data resident on the device: epochs passes of
+ * shuffled batches (a remainder smaller than batch is skipped per epoch);
+ * epoch_loss[epochs] receives the mean batch loss of each epoch (may be NULL). */
+ptx_status ptx_mlp_train(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int64_t n_samples,
+                         int32_t batch, int32_t epochs, float lr, uint64_t seed, double* epoch_loss);
+/* C[M][N] = A[M][K] . B[K][N] with BF16 operands (raw bits) on the MFMA
+ * kernel the network uses; M, N multiples of 32, K of 8. */
+ptx_status ptx_selftest_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, const uint16_t* B, float* C);
+
 #ifdef __cplusplus
 }
 #endif

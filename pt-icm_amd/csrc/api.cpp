@@ -12,6 +12,7 @@
 #include "../../include/ptx.h"
 #include "engine.hpp"
 #include "ladder.hpp"
+#include "mlp.hpp"
 #include "problem.hpp"
 
 namespace ptx {
@@ -26,6 +27,9 @@ struct ptx_problem {
 };
 struct ptx_engine {
     ptx::Engine e;
+};
+struct ptx_mlp {
+    ptx::Mlp m;
 };
 
 namespace {
@@ -370,6 +374,70 @@ ptx_status ptx_selftest_lds_bandwidth(int32_t blocks, int32_t iters, double* gbs
 ptx_status ptx_selftest_philox(const uint32_t* ctr, int64_t n, const uint32_t* key, uint32_t* out) {
     if (!ctr || !key || !out || n < 0) return fail(PTX_ERR_INVALID, "null buffer");
     return guarded([&]() { return done(ptx::selftest_philox(ctr, n, key, out)); });
+}
+
+ptx_status ptx_mlp_create(const int32_t* dims, int32_t n_dims, int32_t max_batch, uint64_t seed, ptx_mlp** out) {
+    if (!dims || !out) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() -> ptx_status {
+        std::unique_ptr<ptx_mlp> h(new ptx_mlp);
+        ptx::Status s = h->m.create(dims, n_dims, max_batch, seed);
+        if (!s) return done(s);
+        *out = h.release();
+        return PTX_OK;
+    });
+}
+
+void ptx_mlp_destroy(ptx_mlp* m) { delete m; }
+
+ptx_status ptx_mlp_param_count(const ptx_mlp* m, int64_t* count) {
+    if (!m || !count) return fail(PTX_ERR_INVALID, "null argument");
+    *count = m->m.nparams;
+    return PTX_OK;
+}
+
+ptx_status ptx_mlp_set_params(ptx_mlp* m, const float* params, int64_t count) {
+    if (!m || !params) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(m->m.set_params(params, count)); });
+}
+
+ptx_status ptx_mlp_get_params(ptx_mlp* m, float* params, int64_t count) {
+    if (!m || !params) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(m->m.get_params(params, count)); });
+}
+
+ptx_status ptx_mlp_get_grads(ptx_mlp* m, float* grads, int64_t count) {
+    if (!m || !grads) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(m->m.get_grads(grads, count)); });
+}
+
+ptx_status ptx_mlp_forward(ptx_mlp* m, const float* x, int32_t batch, float* y) {
+    if (!m || !x || !y) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(m->m.forward(x, batch, y)); });
+}
+
+ptx_status ptx_mlp_loss_grad(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int32_t batch,
+                             double* loss) {
+    if (!m || !x || !target || !mode) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(m->m.loss_grad(x, target, mode, batch, loss)); });
+}
+
+ptx_status ptx_mlp_adam(ptx_mlp* m, float lr, float beta1, float beta2, float eps) {
+    if (!m) return fail(PTX_ERR_INVALID, "null argument");
+    if (!(lr > 0.0f) || !(beta1 >= 0.0f && beta1 < 1.0f) || !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f))
+        return fail(PTX_ERR_INVALID, "Adam hyper-parameters out of range");
+    return guarded([&]() { return done(m->m.adam(lr, beta1, beta2, eps)); });
+}
+
+ptx_status ptx_mlp_train(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int64_t n_samples,
+                         int32_t batch, int32_t epochs, float lr, uint64_t seed, double* epoch_loss) {
+    if (!m || !x || !target || !mode) return fail(PTX_ERR_INVALID, "null argument");
+    if (!(lr > 0.0f)) return fail(PTX_ERR_INVALID, "learning rate must be positive");
+    return guarded([&]() { return done(m->m.train(x, target, mode, n_samples, batch, epochs, lr, seed, epoch_loss)); });
+}
+
+ptx_status ptx_selftest_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, const uint16_t* B, float* C) {
+    if (!A || !B || !C) return fail(PTX_ERR_INVALID, "null buffer");
+    return guarded([&]() { return done(ptx::selftest_gemm_bf16(M, N, K, A, B, C)); });
 }
 
 }  // extern "C"

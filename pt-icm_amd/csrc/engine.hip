@@ -17,11 +17,6 @@ Status hip_check(hipError_t e, const char* what) {
     return Status::err(code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define PTX_TRY(expr)                  \
-    do {                               \
-        ::ptx::Status _s = (expr);     \
-        if (!_s) return _s;            \
-    } while (0)
 
 template <class T>
 Status DevBuf<T>::alloc(size_t count) {
@@ -754,5 +749,7 @@ template struct DevBuf<uint16_t>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<int32_t>;
 template struct DevBuf<double>;
+template struct DevBuf<float>;
+template struct DevBuf<uint8_t>;
 
 }  // namespace ptx

@@ -12,6 +12,12 @@
 #include "problem.hpp"
 #include "ptx_common.h"
 
+#define PTX_TRY(expr)                  \
+    do {                               \
+        ::ptx::Status _s = (expr);     \
+        if (!_s) return _s;            \
+    } while (0)
+
 namespace ptx {
 
 template <class T>
@@ -20,6 +26,10 @@ struct DevBuf {
     size_t n = 0;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {

@@ -109,6 +109,20 @@ _SIGS = {
     "ptx_selftest_philox": (C.c_int32, [_P(C.c_uint32), C.c_int64, _P(C.c_uint32), _P(C.c_uint32)]),
     "ptx_selftest_copy": (C.c_int32, [C.c_int64, C.c_int32, _P(C.c_double)]),
     "ptx_selftest_lds_bandwidth": (C.c_int32, [C.c_int32, C.c_int32, _P(C.c_double)]),
+    "ptx_mlp_create": (C.c_int32, [_P(C.c_int32), C.c_int32, C.c_int32, C.c_uint64, _P(_vp)]),
+    "ptx_mlp_destroy": (None, [_vp]),
+    "ptx_mlp_param_count": (C.c_int32, [_vp, _P(C.c_int64)]),
+    "ptx_mlp_set_params": (C.c_int32, [_vp, _P(C.c_float), C.c_int64]),
+    "ptx_mlp_get_params": (C.c_int32, [_vp, _P(C.c_float), C.c_int64]),
+    "ptx_mlp_get_grads": (C.c_int32, [_vp, _P(C.c_float), C.c_int64]),
+    "ptx_mlp_forward": (C.c_int32, [_vp, _P(C.c_float), C.c_int32, _P(C.c_float)]),
+    "ptx_mlp_loss_grad": (C.c_int32, [_vp, _P(C.c_float), _P(C.c_float), _P(C.c_uint8), C.c_int32,
+                                      _P(C.c_double)]),
+    "ptx_mlp_adam": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "ptx_mlp_train": (C.c_int32, [_vp, _P(C.c_float), _P(C.c_float), _P(C.c_uint8), C.c_int64, C.c_int32,
+                                  C.c_int32, C.c_float, C.c_uint64, _P(C.c_double)]),
+    "ptx_selftest_gemm_bf16": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32, _P(C.c_uint16), _P(C.c_uint16),
+                                           _P(C.c_float)]),
 }
 
 
@@ -416,3 +430,79 @@ def lds_bandwidth_gbs(blocks=4096, iters=4096):
     out = C.c_double()
     check(lib().ptx_selftest_lds_bandwidth(int(blocks), int(iters), C.byref(out)))
     return out.value
+
+
+class Mlp:
+    """The ladder surrogate network on the GPU (BF16 MFMA kernels, FP32
+    accumulate).  Parameters are a flat float32 vector: per layer W[in][out]
+    then b[out]."""
+
+    def __init__(self, dims, max_batch=256, seed=0):
+        self.dims = [int(d) for d in dims]
+        d = np.ascontiguousarray(self.dims, np.int32)
+        h = C.c_void_p()
+        check(lib().ptx_mlp_create(_ptr(d, C.c_int32), len(d), int(max_batch), int(seed), C.byref(h)))
+        self.h = h
+        n = C.c_int64()
+        check(lib().ptx_mlp_param_count(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().ptx_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.nparams, np.float32)
+        check(lib().ptx_mlp_get_params(self.h, _ptr(out, C.c_float), self.nparams))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().ptx_mlp_set_params(self.h, _ptr(p, C.c_float), len(p)))
+
+    def grads(self):
+        out = np.zeros(self.nparams, np.float32)
+        check(lib().ptx_mlp_get_grads(self.h, _ptr(out, C.c_float), self.nparams))
+        return out
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((x.shape[0], self.dims[-1]), np.float32)
+        check(lib().ptx_mlp_forward(self.h, _ptr(x, C.c_float), x.shape[0], _ptr(y, C.c_float)))
+        return y
+
+    def loss_grad(self, x, t, mode):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mode, np.uint8)
+        loss = C.c_double()
+        check(lib().ptx_mlp_loss_grad(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), _ptr(m, C.c_uint8),
+                                      x.shape[0], C.byref(loss)))
+        return loss.value
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(lib().ptx_mlp_adam(self.h, lr, beta1, beta2, eps))
+
+    def train(self, x, t, mode, batch=128, epochs=10, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mode, np.uint8)
+        losses = np.zeros(epochs, np.float64)
+        check(lib().ptx_mlp_train(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), _ptr(m, C.c_uint8), x.shape[0],
+                                  int(batch), int(epochs), float(lr), int(seed), _ptr(losses, C.c_double)))
+        return losses
+
+
+def gemm_bf16(a_bits, b_bits):
+    """C = A . B on the MFMA kernel; operands as raw BF16 bit patterns (uint16)."""
+    a = np.ascontiguousarray(a_bits, np.uint16)
+    b = np.ascontiguousarray(b_bits, np.uint16)
+    M, K = a.shape
+    K2, N = b.shape
+    assert K == K2
+    c = np.zeros((M, N), np.float32)
+    check(lib().ptx_selftest_gemm_bf16(M, N, K, _ptr(a, C.c_uint16), _ptr(b, C.c_uint16), _ptr(c, C.c_float)))
+    return c
