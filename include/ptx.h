@@ -95,6 +95,7 @@ ptx_status ptx_pilot_betas(double T0, double T1, int32_t nt, int32_t inverse, do
 
 #define PTX_LADDER_EXACT 0 /* the reference's O(nbins^2) acceptance sum */
 #define PTX_LADDER_FAST 1  /* O(nbins) prefix-sum form of the same sum */
+#define PTX_LADDER_NEWTON 2 /* FAST, with the DOS equations solved by Newton instead of fixed-point iteration */
 /* get_beta after the pilot, src/cae_estimator.cc:330-428: multihistogram DOS
  * (Ferrenberg-Swendsen in log space) from per-slot energy histograms, then the
  * constant-acceptance ladder search toward rate r.  counts is [nt][nbins] in
@@ -127,6 +128,10 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
 #define PTX_MODE_FUSED 2 /* one workgroup per instance runs whole PT steps */
 #define PTX_MODE_COOP 3  /* one instance: one workgroup per replica, grid barrier per step */
 
+#define PTX_CSR_AUTO 0   /* adjacency in LDS when it fits, else read through L2 */
+#define PTX_CSR_LDS 1
+#define PTX_CSR_GLOBAL 2
+
 typedef struct {
     int32_t chains;          /* 2 for the main run (initialization.cc:166), 1 for a pilot */
     int32_t sweeps;          /* --sweeps */
@@ -138,6 +143,7 @@ typedef struct {
     int32_t steps_per_launch;/* fused mode: PT steps per launch (0 = auto) */
     int32_t use_graph;       /* capture step sequences in a hipGraph */
     int32_t spins_per_lane;  /* cooperative mode: 1 or 4 positions per lane (0 = auto) */
+    int32_t csr_placement;   /* fused mode: PTX_CSR_AUTO, PTX_CSR_LDS or PTX_CSR_GLOBAL */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 
@@ -162,6 +168,7 @@ typedef struct {
     int64_t total_replicas;
     int64_t spin_updates_per_step; /* sum over replicas of n * sweeps */
     int64_t lds_bytes_per_step;    /* algorithmic LDS bytes of the sweep (4 d_i + 6 per update) */
+    int32_t csr_in_lds;            /* fused mode: 1 if the adjacency is staged in LDS */
 } ptx_engine_info;
 ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out);
 

@@ -182,8 +182,9 @@ ptx_status ptx_ladder_from_histograms(const uint32_t* counts, int32_t nt, int64_
         auto H = ptx::rebin(counts, nt, nbins, e_base, estep);
         for (const auto& s : H)
             if (s.h.empty()) return fail(PTX_ERR_INVALID, "a pilot histogram is empty");
-        ptx::Dos D = ptx::solve_dos(H, hist_betas, estep);
-        *nt_out = ptx::ladder_search(D, nt, beta_start, beta_end, step_size, r, mode == PTX_LADDER_FAST, betas_out);
+        ptx::Dos D = mode == PTX_LADDER_NEWTON ? ptx::solve_dos_newton(H, hist_betas, estep)
+                                               : ptx::solve_dos(H, hist_betas, estep);
+        *nt_out = ptx::ladder_search(D, nt, beta_start, beta_end, step_size, r, mode != PTX_LADDER_EXACT, betas_out);
         return PTX_OK;
     });
 }
@@ -197,7 +198,8 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
         auto H = ptx::rebin(counts, nt, nbins, e_base, estep);
         for (const auto& s : H)
             if (s.h.empty()) return fail(PTX_ERR_INVALID, "a pilot histogram is empty");
-        ptx::Dos D = ptx::solve_dos(H, hist_betas, estep);
+        ptx::Dos D = mode == PTX_LADDER_NEWTON ? ptx::solve_dos_newton(H, hist_betas, estep)
+                                               : ptx::solve_dos(H, hist_betas, estep);
         std::vector<double> p1, p2;
         for (int32_t k = 0; k + 1 < n_ladder; ++k) {
             double b1 = ladder[k], b2 = ladder[k + 1];
@@ -211,7 +213,7 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
                 p1[i] = D.logdos[i] - e * b1 - z1;
                 p2[i] = D.logdos[i] - e * b2 - z2;
             }
-            rates_out[k] = ptx::acceptance_rate(p1, p2, b1, b2, estep, D.min_emin, mode == PTX_LADDER_FAST);
+            rates_out[k] = ptx::acceptance_rate(p1, p2, b1, b2, estep, D.min_emin, mode != PTX_LADDER_EXACT);
         }
         return PTX_OK;
     });
@@ -230,6 +232,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->steps_per_launch = 0;
     c->use_graph = 0;
     c->spins_per_lane = 0;
+    c->csr_placement = PTX_CSR_AUTO;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -275,6 +278,7 @@ ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out) {
     out->total_replicas = E.total_rep;
     out->spin_updates_per_step = E.updates_per_step;
     out->lds_bytes_per_step = E.lds_bytes_per_step;
+    out->csr_in_lds = E.mode == PTX_MODE_SPLIT ? 0 : (E.fused_global ? 0 : 1);
     return PTX_OK;
 }
 

@@ -30,7 +30,8 @@ const char* kUsage =
     "      --log                    log data (accepted, no effect, as the reference)\n"
     "      --icm                    isoenergetic cluster moves\n"
     "      --verbose N              verbosity 0..3                          [0]\n"
-    "      --ladder-mode exact|fast acceptance sum of the ladder search    [exact]\n";
+    "      --ladder-mode exact|fast|newton  ladder search: reference sums | O(nbins) sums |\n"
+    "                             O(nbins) sums + Newton multihistogram solve    [exact]\n";
 
 struct Args {
     std::string problem;
@@ -102,7 +103,8 @@ bool parse(int argc, char** argv, Args& a, std::string& err) {
             if (!value(s)) return false;
             if (s == "exact") p.ladder_mode = PTX_LADDER_EXACT;
             else if (s == "fast") p.ladder_mode = PTX_LADDER_FAST;
-            else { err = "--ladder-mode: exact or fast"; return false; }
+            else if (s == "newton") p.ladder_mode = PTX_LADDER_NEWTON;
+            else { err = "--ladder-mode: exact, fast or newton"; return false; }
         } else { err = "Unknown argument: " + k; return false; }
     }
     if (!a.help && a.problem.empty()) { err = "-p: required."; return false; }

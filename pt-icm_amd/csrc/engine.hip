@@ -224,22 +224,45 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode runs a single instance");
     }
     if (mode == PTX_MODE_SPLIT && (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO)) {
-        Wf = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
-        if (Wf != 4 && Wf != 8 && Wf != 16) Wf = Wf < 6 ? 4 : (Wf < 12 ? 8 : 16);
-        int lds_max = 0;
-        for (int32_t i = 0; i < ninst; ++i) {
-            const FusedLayout L =
-                fused_layout(desc[i].n, (int)desc[i].adj_len16, desc[i].nrep, desc[i].nt, desc[i].fmax, Wf);
-            lds_max = std::max(lds_max, L.total);
-        }
+        int Wpref = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
+        if (Wpref != 4 && Wpref != 8 && Wpref != 16) Wpref = Wpref < 6 ? 4 : (Wpref < 12 ? 8 : 16);
+        // LDS bytes of the largest instance for W waves, adjacency in LDS or not
+        auto need = [&](int w, int csr_lds) {
+            int mx = 0;
+            for (int32_t i = 0; i < ninst; ++i)
+                mx = std::max(mx, fused_layout(desc[i].n, (int)desc[i].adj_len16, desc[i].nrep, desc[i].nt,
+                                               desc[i].fmax, w, csr_lds, desc[i].nc > 0 ? 1 : 0)
+                                      .total);
+            return mx;
+        };
         const int lds_limit = 160 * 1024;
-        if (lds_max <= lds_limit) {
-            mode = PTX_MODE_FUSED;
-            fused_lds = lds_max;
-        } else if (cfg.mode == PTX_MODE_FUSED) {
-            return Status::err(PTX_ERR_UNSUPPORTED, "instance does not fit the fused kernel's LDS (" +
-                                                        std::to_string(lds_max) + " bytes); use split mode");
+        const int place = cfg.csr_placement;
+        // an explicit waves_per_group is kept; otherwise fewer waves may make an instance fit
+        std::vector<int> ws;
+        for (int w = Wpref; w >= 4; w /= 2) {
+            ws.push_back(w);
+            if (cfg.waves_per_group > 0) break;
         }
+        bool placed = false;
+        for (int csr_lds = 1; csr_lds >= 0 && !placed; --csr_lds) {
+            if ((csr_lds && place == PTX_CSR_GLOBAL) || (!csr_lds && place == PTX_CSR_LDS)) continue;
+            for (int w : ws) {
+                const int b = need(w, csr_lds);
+                if (b <= lds_limit) {
+                    mode = PTX_MODE_FUSED;
+                    Wf = w;
+                    fused_lds = b;
+                    fused_global = csr_lds ? 0 : 1;
+                    placed = true;
+                    break;
+                }
+            }
+        }
+        if (!placed && cfg.mode == PTX_MODE_FUSED)
+            return Status::err(PTX_ERR_UNSUPPORTED, "instance does not fit the fused kernel's LDS (" +
+                                                        std::to_string(need(ws.back(), place == PTX_CSR_LDS ? 1 : 0)) +
+                                                        " bytes at " + std::to_string(ws.back()) +
+                                                        " waves); use split mode");
         K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 16;
     }
     std::vector<int32_t> fused_order(order.begin(), order.end());
@@ -425,17 +448,31 @@ Status Engine::launch_fused(int nsteps) {
         }
         return Status::ok();
     };
+    if (fused_global) {
+        if (fmt == kFmtUnit) {
+            switch (Wf) {
+                case 4: return go(k_fused<kFmtUnit, 4, 1>);
+                case 8: return go(k_fused<kFmtUnit, 8, 1>);
+                default: return go(k_fused<kFmtUnit, 16, 1>);
+            }
+        }
+        switch (Wf) {
+            case 4: return go(k_fused<kFmtWide, 4, 1>);
+            case 8: return go(k_fused<kFmtWide, 8, 1>);
+            default: return go(k_fused<kFmtWide, 16, 1>);
+        }
+    }
     if (fmt == kFmtUnit) {
         switch (Wf) {
-            case 4: return go(k_fused<kFmtUnit, 4>);
-            case 8: return go(k_fused<kFmtUnit, 8>);
-            default: return go(k_fused<kFmtUnit, 16>);
+            case 4: return go(k_fused<kFmtUnit, 4, 0>);
+            case 8: return go(k_fused<kFmtUnit, 8, 0>);
+            default: return go(k_fused<kFmtUnit, 16, 0>);
         }
     }
     switch (Wf) {
-        case 4: return go(k_fused<kFmtWide, 4>);
-        case 8: return go(k_fused<kFmtWide, 8>);
-        default: return go(k_fused<kFmtWide, 16>);
+        case 4: return go(k_fused<kFmtWide, 4, 0>);
+        case 8: return go(k_fused<kFmtWide, 8, 0>);
+        default: return go(k_fused<kFmtWide, 16, 0>);
     }
 }
 

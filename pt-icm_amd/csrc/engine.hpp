@@ -58,6 +58,7 @@ struct Engine {
     int32_t Wf = 8;              // waves per workgroup (fused mode)
     int32_t K = 16;              // PT steps per fused launch
     int32_t fused_lds = 0;       // bytes of dynamic LDS per fused workgroup
+    int32_t fused_global = 0;    // fused mode reads the adjacency from global memory
     int32_t nrep_max = 0, nw_max = 0, n_max = 0, fmax_max = 1;
     int32_t spin_stride = 0, thr_stride = 1;
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;

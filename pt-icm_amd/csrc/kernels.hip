@@ -205,13 +205,17 @@ struct FusedLayout {
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
-__host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nrep, int nt, int fmax, int W) {
+// csr_lds = 0: the adjacency stays in global memory (read through L2), only
+// the per-replica state is staged.
+// icm = 0: no cluster moves, so the per-wave scratch only holds spin bytes.
+__host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nrep, int nt, int fmax, int W,
+                                                    int csr_lds, int icm) {
     FusedLayout L;
     int o = 0;
     L.info = o;
-    o += align16(4 * n);
+    o += csr_lds ? align16(4 * n) : 16;
     L.adj = o;
-    o += 16 * (adj_len16 > 0 ? adj_len16 : 1);
+    o += csr_lds ? 16 * (adj_len16 > 0 ? adj_len16 : 1) : 16;
     L.thr = o;
     o += align16(4 * nt * (fmax > 0 ? fmax : 1));
     L.beta = o;
@@ -238,7 +242,7 @@ __host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nr
     o += align16(4 * nrep);
     const int nw = (n + 31) / 32;
     const int sweep_bytes = 32 * nw;
-    const int icm_bytes = icm_scratch_bytes(n);
+    const int icm_bytes = icm ? icm_scratch_bytes(n) : 0;
     L.scratch_stride = align16(sweep_bytes > icm_bytes ? sweep_bytes : icm_bytes);
     L.scratch = o;
     o += W * L.scratch_stride;
@@ -246,7 +250,7 @@ __host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nr
     return L;
 }
 
-template <int Fmt, int W>
+template <int Fmt, int W, int G>
 __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -255,7 +259,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     const InstDesc d = a.desc[inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
     const int adj16 = (int)d.adj_len16;
-    const FusedLayout L = fused_layout(n, adj16, nrep, nt, d.fmax, W);
+    const FusedLayout L = fused_layout(n, adj16, nrep, nt, d.fmax, W, G ? 0 : 1, d.nc > 0 ? 1 : 0);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -273,8 +277,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     uint8_t* scratch = smem + L.scratch + wave * L.scratch_stride;
 
     // ---- stage the instance ----
-    for (int p = tid; p < n; p += W * kWave) info[p] = a.rowinfo[d.pos_off + p];
-    {
+    if (!G) {
+        for (int p = tid; p < n; p += W * kWave) info[p] = a.rowinfo[d.pos_off + p];
         const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
         uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
         for (int q = tid; q < adj16; q += W * kWave) dst[q] = src[q];
@@ -290,7 +294,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
         succ[s] = 0;
     }
     __syncthreads();
-    const Csr<Fmt> csr{info, smem + L.adj};
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16}
+                           : Csr<Fmt>{info, smem + L.adj};
     const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
@@ -459,9 +464,12 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint
     template __global__ void k_sweep<F, 4>(DevArgs, uint32_t);    \
     template __global__ void k_sweep<F, 8>(DevArgs, uint32_t);    \
     template __global__ void k_icm<F>(DevArgs, uint32_t, int);    \
-    template __global__ void k_fused<F, 4>(DevArgs, int);         \
-    template __global__ void k_fused<F, 8>(DevArgs, int);         \
-    template __global__ void k_fused<F, 16>(DevArgs, int);
+    template __global__ void k_fused<F, 4, 0>(DevArgs, int);      \
+    template __global__ void k_fused<F, 8, 0>(DevArgs, int);      \
+    template __global__ void k_fused<F, 16, 0>(DevArgs, int);     \
+    template __global__ void k_fused<F, 4, 1>(DevArgs, int);      \
+    template __global__ void k_fused<F, 8, 1>(DevArgs, int);      \
+    template __global__ void k_fused<F, 16, 1>(DevArgs, int);
 PTX_SPLIT_INST(kFmtWide)
 PTX_SPLIT_INST(kFmtUnit)
 

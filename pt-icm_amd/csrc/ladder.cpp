@@ -155,6 +155,156 @@ Dos solve_dos(const std::vector<SlotHist>& hist, const double* beta_h, double es
     return D;
 }
 
+// The same fixed point as solve_dos, found by damped Newton on the convex
+// function whose stationarity conditions are the multihistogram equations:
+//   F(g) = sum_e N(e) log sum_j n e^{g_j - beta_j e} - n sum_j g_j,
+// g_j = -log Z_j (gauge g_0 = 0), N(e) = all counts in the energy bin.
+// Gradient_k = sum_e N(e) p_k(e) - n, Hessian_kl = sum_e N(e)(p_k d_kl - p_k p_l)
+// with p_k(e) the normalised weights e^{g_k - beta_k e} / sum_j.  Starts from
+// thermodynamic integration of the slot means (d log Z / d beta = -<E>).
+Dos solve_dos_newton(const std::vector<SlotHist>& hist, const double* beta_h, double estep) {
+    Dos D;
+    D.estep = estep;
+    const int nh = (int)hist.size();
+    D.max_emax = -DBL_MAX;
+    D.min_emin = DBL_MAX;
+    for (const auto& H : hist) {
+        D.max_emax = std::max(D.max_emax, H.emax);
+        D.min_emin = std::min(D.min_emin, H.emin);
+    }
+    uint64_t n = 0;
+    for (uint32_t c : hist[0].h) n += c;
+    const size_t nbins = (size_t)std::ceil((D.max_emax - D.min_emin) / estep);
+    std::vector<double> N(nbins, 0.0);
+    std::vector<double> mean(nh, 0.0);
+    for (int j = 0; j < nh; ++j) {
+        const SlotHist& H = hist[j];
+        double s = 0, w = 0;
+        for (size_t b = 0; b < H.h.size(); ++b) {
+            if (!H.h[b]) continue;
+            const double e = H.emin + (double)b * estep;
+            s += e * H.h[b];
+            w += H.h[b];
+            const size_t i = bin_of(e, D.min_emin, estep);
+            if (i < nbins) N[i] += H.h[b];
+        }
+        mean[j] = w > 0 ? s / w : 0.0;
+    }
+    std::vector<double> E, Nv;  // occupied bins only
+    for (size_t i = 0; i < nbins; ++i)
+        if (N[i] > 0) {
+            E.push_back(D.min_emin + (double)i * estep);
+            Nv.push_back(N[i]);
+        }
+    const double ln_n = std::log((double)n), dn = (double)n;
+    std::vector<double> g(nh, 0.0);
+    for (int j = 1; j < nh; ++j) g[j] = g[j - 1] + (beta_h[j] - beta_h[j - 1]) * 0.5 * (mean[j] + mean[j - 1]);
+    std::vector<double> t(nh), p(nh), grad(nh), Hm((size_t)nh * nh), step(nh), gt(nh);
+    auto objective = [&](const std::vector<double>& gg) {
+        double F = 0;
+        for (size_t i = 0; i < E.size(); ++i) {
+            double mx = -DBL_MAX;
+            for (int j = 0; j < nh; ++j) mx = std::max(mx, gg[j] - beta_h[j] * E[i]);
+            double s = 0;
+            for (int j = 0; j < nh; ++j) s += std::exp(gg[j] - beta_h[j] * E[i] - mx);
+            F += Nv[i] * (ln_n + mx + std::log(s));
+        }
+        for (int j = 0; j < nh; ++j) F -= dn * gg[j];
+        return F;
+    };
+    double F = objective(g);
+    for (int iter = 0; iter < 200; ++iter) {
+        std::fill(grad.begin(), grad.end(), -dn);
+        std::fill(Hm.begin(), Hm.end(), 0.0);
+        for (size_t i = 0; i < E.size(); ++i) {
+            double mx = -DBL_MAX;
+            for (int j = 0; j < nh; ++j) mx = std::max(mx, t[j] = g[j] - beta_h[j] * E[i]);
+            double s = 0;
+            for (int j = 0; j < nh; ++j) s += (p[j] = std::exp(t[j] - mx));
+            const double inv = 1.0 / s;
+            for (int j = 0; j < nh; ++j) p[j] *= inv;
+            for (int k = 0; k < nh; ++k) {
+                if (p[k] < 1e-300) continue;
+                const double a = Nv[i] * p[k];
+                grad[k] += a;
+                Hm[(size_t)k * nh + k] += a;
+                double* row = &Hm[(size_t)k * nh];
+                for (int l = 0; l < nh; ++l) row[l] -= a * p[l];
+            }
+        }
+        double gmax = 0;
+        for (int k = 1; k < nh; ++k) gmax = std::max(gmax, std::fabs(grad[k]));
+        if (gmax <= 1e-10 * dn) break;
+        // solve H' step = -grad' on slots 1..nh-1 (Cholesky with a tiny ridge)
+        const int m = nh - 1;
+        std::vector<double> A((size_t)m * m), b(m);
+        double tr = 0;
+        for (int k = 0; k < m; ++k) tr += Hm[(size_t)(k + 1) * nh + (k + 1)];
+        const double ridge = 1e-12 * (tr > 0 ? tr / m : 1.0);
+        for (int k = 0; k < m; ++k) {
+            b[k] = -grad[k + 1];
+            for (int l = 0; l < m; ++l) A[(size_t)k * m + l] = Hm[(size_t)(k + 1) * nh + (l + 1)];
+            A[(size_t)k * m + k] += ridge;
+        }
+        bool ok = true;
+        for (int k = 0; k < m && ok; ++k) {
+            for (int l = 0; l < k; ++l) A[(size_t)k * m + k] -= A[(size_t)k * m + l] * A[(size_t)k * m + l];
+            if (!(A[(size_t)k * m + k] > 0)) {
+                ok = false;
+                break;
+            }
+            A[(size_t)k * m + k] = std::sqrt(A[(size_t)k * m + k]);
+            for (int r = k + 1; r < m; ++r) {
+                double v = A[(size_t)r * m + k];
+                for (int l = 0; l < k; ++l) v -= A[(size_t)r * m + l] * A[(size_t)k * m + l];
+                A[(size_t)r * m + k] = v / A[(size_t)k * m + k];
+            }
+        }
+        if (!ok) break;
+        for (int k = 0; k < m; ++k) {  // forward then backward substitution
+            double v = b[k];
+            for (int l = 0; l < k; ++l) v -= A[(size_t)k * m + l] * b[l];
+            b[k] = v / A[(size_t)k * m + k];
+        }
+        for (int k = m - 1; k >= 0; --k) {
+            double v = b[k];
+            for (int l = k + 1; l < m; ++l) v -= A[(size_t)l * m + k] * b[l];
+            b[k] = v / A[(size_t)k * m + k];
+        }
+        step[0] = 0;
+        for (int k = 0; k < m; ++k) step[k + 1] = b[k];
+        double a = 1.0, Fn = F;
+        bool moved = false;
+        for (int ls = 0; ls < 40; ++ls, a *= 0.5) {
+            for (int j = 0; j < nh; ++j) gt[j] = g[j] + a * step[j];
+            Fn = objective(gt);
+            if (Fn <= F) {
+                moved = true;
+                break;
+            }
+        }
+        if (!moved) break;
+        const double dF = F - Fn;
+        g.swap(gt);
+        F = Fn;
+        if (dF <= 1e-15 * std::fabs(F)) break;
+    }
+    // log g(e) and log Z_j as the reference computes them from the fixed point
+    D.logdos.assign(nbins, -DBL_MAX);
+    for (size_t i = 0; i < nbins; ++i) {
+        if (!(N[i] > 0)) continue;
+        const double e = D.min_emin + (double)i * estep;
+        double mx = -DBL_MAX;
+        for (int j = 0; j < nh; ++j) mx = std::max(mx, g[j] - beta_h[j] * e);
+        double s = 0;
+        for (int j = 0; j < nh; ++j) s += std::exp(g[j] - beta_h[j] * e - mx);
+        D.logdos[i] = std::log(N[i]) - (ln_n + mx + std::log(s));
+    }
+    D.logz.resize(nh);
+    for (int j = 0; j < nh; ++j) D.logz[j] = partition_func(D.logdos, beta_h[j], D.min_emin, estep);
+    return D;
+}
+
 double acceptance_rate(const std::vector<double>& p1, const std::vector<double>& p2, double beta_1, double beta_2,
                        double estep, double emin, bool fast) {
     const size_t nb = p1.size();

@@ -34,6 +34,9 @@ struct Dos {
 };
 // get_partition_iterate + calculate_dos (src/cae_estimator.cc:155-253)
 Dos solve_dos(const std::vector<SlotHist>& hist, const double* hist_betas, double estep);
+// The same equations solved by damped Newton (converges in a few iterations
+// where the reference's fixed-point iteration needs thousands).
+Dos solve_dos_newton(const std::vector<SlotHist>& hist, const double* hist_betas, double estep);
 
 double partition_func(const std::vector<double>& logdos, double beta, double emin, double estep);
 // acceptance_rate (src/cae_estimator.cc:262-293); fast = O(nbins) prefix form.

@@ -42,24 +42,23 @@ def erdos_renyi(n, p, seed=0, pm=False):
     """G(n, p) without self-loops; pm: w = +-1, else w = +1 (MaxCut, J = -w)."""
     g = _rng(seed, 4)
     m = g.binomial(n * (n - 1) // 2, p)
-    # sample m distinct unordered pairs
-    chosen = set()
-    out = []
-    while len(out) < m:
-        need = m - len(out)
+    # sample m distinct unordered pairs: batches of candidate pairs, taken in
+    # draw order, skipping self-loops and pairs already chosen
+    chosen = np.zeros(0, np.int64)
+    while len(chosen) < m:
+        need = m - len(chosen)
         u = g.integers(0, n, 2 * need + 16)
         v = g.integers(0, n, 2 * need + 16)
-        for a, b in zip(u.tolist(), v.tolist()):
-            if a == b:
-                continue
-            key = (a, b) if a < b else (b, a)
-            if key in chosen:
-                continue
-            chosen.add(key)
-            out.append(key)
-            if len(out) == m:
-                break
-    e = np.array(sorted(out), dtype=np.int32).reshape(-1, 2)
+        key = np.minimum(u, v).astype(np.int64) * n + np.maximum(u, v)
+        ok = u != v
+        _, first = np.unique(key, return_index=True)
+        is_first = np.zeros(len(key), bool)
+        is_first[first] = True
+        ok &= is_first & ~np.isin(key, chosen)
+        chosen = np.concatenate([chosen, key[ok][:need]])
+    chosen = np.sort(chosen)
+    out = np.stack([chosen // n, chosen % n], 1)
+    e = out.astype(np.int32).reshape(-1, 2)
     if pm:
         w = np.where(g.integers(0, 2, len(e)) == 1, 1.0, -1.0)
     else:

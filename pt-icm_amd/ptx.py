@@ -18,7 +18,8 @@ FLAG_PILOT = 1 << 2
 FLAG_NO_EXCHANGE = 1 << 3
 FLAG_HISTOGRAM = 1 << 5
 MODE_AUTO, MODE_SPLIT, MODE_FUSED, MODE_COOP = 0, 1, 2, 3
-LADDER_EXACT, LADDER_FAST = 0, 1
+LADDER_EXACT, LADDER_FAST, LADDER_NEWTON = 0, 1, 2
+CSR_AUTO, CSR_LDS, CSR_GLOBAL = 0, 1, 2
 
 
 class PtxError(RuntimeError):
@@ -36,13 +37,14 @@ class ProblemInfo(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("chains", C.c_int32), ("sweeps", C.c_int32), ("flags", C.c_uint32), ("nc", C.c_int32),
                 ("hist_start", C.c_int64), ("mode", C.c_int32), ("waves_per_group", C.c_int32),
-                ("steps_per_launch", C.c_int32), ("use_graph", C.c_int32), ("spins_per_lane", C.c_int32)]
+                ("steps_per_launch", C.c_int32), ("use_graph", C.c_int32), ("spins_per_lane", C.c_int32),
+                ("csr_placement", C.c_int32)]
 
 
 class EngineInfo(C.Structure):
     _fields_ = [("n_instances", C.c_int32), ("steps_done", C.c_int64), ("mode", C.c_int32),
                 ("waves_per_group", C.c_int32), ("steps_per_launch", C.c_int32), ("total_replicas", C.c_int64),
-                ("spin_updates_per_step", C.c_int64), ("lds_bytes_per_step", C.c_int64)]
+                ("spin_updates_per_step", C.c_int64), ("lds_bytes_per_step", C.c_int64), ("csr_in_lds", C.c_int32)]
 
 
 class InstanceInfo(C.Structure):
@@ -279,7 +281,7 @@ class Engine:
     """A batch of independent PT-ICM instances on the GPU (one is the plain path)."""
 
     def __init__(self, problems, ladders, seed=0, chains=2, sweeps=1, flags=0, nc=-1, hist_start=0, mode=MODE_AUTO,
-                 waves_per_group=0, steps_per_launch=0, use_graph=0, spins_per_lane=0):
+                 waves_per_group=0, steps_per_launch=0, use_graph=0, spins_per_lane=0, csr_placement=CSR_AUTO):
         if isinstance(problems, Problem):
             problems = [problems]
             ladders = [ladders]
@@ -289,6 +291,7 @@ class Engine:
         cfg.chains, cfg.sweeps, cfg.flags, cfg.nc = chains, sweeps, flags, nc
         cfg.hist_start, cfg.mode, cfg.waves_per_group = hist_start, mode, waves_per_group
         cfg.steps_per_launch, cfg.use_graph, cfg.spins_per_lane = steps_per_launch, use_graph, spins_per_lane
+        cfg.csr_placement = csr_placement
         self.cfg = cfg
         arr = (C.c_void_p * len(self.problems))(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h
                                                   for p in self.problems])

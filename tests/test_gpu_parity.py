@@ -96,7 +96,6 @@ SWEEP_CASES = ["lattice_2d_5.gset", "lattice_2d_20.gset", "pm_torus20", "G022", 
 
 
 MODES = [1, 2, 3]  # split kernels, fused per-instance kernel, cooperative per-replica kernel
-BATCH_MODES = [1, 2]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -216,16 +215,20 @@ def test_pilot_histograms_match_oracle_logs(ptx, oracle, mode):
 
 
 # ---------------------------------------------------------------- batched ---
-@pytest.mark.parametrize("mode", BATCH_MODES)
+@pytest.mark.parametrize("mode,csr", [(1, 0), (2, 1), (2, 2)])
 @pytest.mark.parametrize("with_field", [False, True])
-def test_batched_instances_match_single_oracles(ptx, oracle, mode, with_field):
+def test_batched_instances_match_single_oracles(ptx, oracle, mode, csr, with_field):
+    """csr 1: adjacency staged in LDS; csr 2: the fused kernel reads it through L2."""
     kinds = ["lattice_2d_10.gset", "ea3d_6", "pm_torus20", "lattice_2d_5.gset", "er_small"]
     if with_field:
         kinds.append("field_random")  # forces the 32-bit adjacency format in the fused kernel
     pairs = [make_pair(ptx, oracle, k) for k in kinds]
     nts = [4, 8, 6, 3, 5, 7][: len(kinds)]
     ladders = [oracle.beta_linspace(0.5, 3.0, nt, 0) for nt in nts]
-    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=77, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=2)
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=77, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=2,
+                     csr_placement=csr)
+    if mode == 2:
+        assert eng.info().csr_in_lds == (1 if csr == 1 else 0)
     oes = [oracle.Engine(g, lad, chains=2, seed=77, instance=i, flags=oracle.FLAG_ICM)
            for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
     for s in range(6):

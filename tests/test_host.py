@@ -123,6 +123,59 @@ def test_fast_acceptance_equals_exact(ptx, oracle):
     assert np.all((r_exact > 0) & (r_exact <= 1))
 
 
+def _torus4_dos():
+    """Exact density of states of the 4x4 periodic ferromagnet (w = -1, E = sum w s s)."""
+    L = 4
+    idx = np.arange(16).reshape(L, L)
+    edges = np.concatenate([np.stack([idx.ravel(), np.roll(idx, -1, 1).ravel()], 1),
+                            np.stack([idx.ravel(), np.roll(idx, -1, 0).ravel()], 1)])
+    states = ((np.arange(1 << 16)[:, None] >> np.arange(16)) & 1) * 2 - 1
+    E = -np.sum(states[:, edges[:, 0]] * states[:, edges[:, 1]], axis=1)
+    ev, g = np.unique(E, return_counts=True)
+    return ev.astype(float), g.astype(float)
+
+
+def _exact_rate(ev, g, b1, b2):
+    P1 = g * np.exp(-b1 * ev); P1 /= P1.sum()
+    P2 = g * np.exp(-b2 * ev); P2 /= P2.sum()
+    hi, lo = max(b1, b2), min(b1, b2)
+    Pa, Pb = (P1, P2) if b1 >= b2 else (P2, P1)
+    return float(sum(Pa[i] * Pb[j] * min(1.0, np.exp((hi - lo) * (ev[i] - ev[j])))
+                     for i in range(len(ev)) for j in range(len(ev))))
+
+
+def test_newton_and_iteration_recover_exact_acceptance(ptx):
+    """Histograms proportional to the exact 4x4 DOS times Boltzmann factors:
+    both multihistogram solvers must predict the exact swap rates; the Newton
+    solve (true fixed point) at least as closely as the reference iteration."""
+    ev, g = _torus4_dos()
+    hb = np.array([0.9, 0.7, 0.5, 0.35, 0.2])
+    n = 2_000_000
+    e_base = -32
+    counts = np.zeros((len(hb), 33), np.uint32)
+    for k, b in enumerate(hb):
+        P = g * np.exp(-b * ev); P /= P.sum()
+        counts[k, ((ev - e_base) // 2).astype(int)] = np.rint(P * n).astype(np.uint32)
+    lad = np.array([0.85, 0.6, 0.45, 0.3, 0.22])
+    want = np.array([_exact_rate(ev, g, lad[k], lad[k + 1]) for k in range(len(lad) - 1)])
+    r_newton = ptx.ladder_predict_rates(counts, e_base, hb, lad, mode=ptx.LADDER_NEWTON)
+    r_iter = ptx.ladder_predict_rates(counts, e_base, hb, lad, mode=ptx.LADDER_EXACT)
+    err_n = np.max(np.abs(r_newton - want))
+    err_i = np.max(np.abs(r_iter - want))
+    assert err_n < 1e-5, (r_newton, want)  # exact up to the rounding of the counts
+    assert err_n <= err_i + 1e-4
+
+
+def test_newton_ladder_close_to_reference_iteration(ptx, oracle):
+    g2, hb2, bs2, be2, logs2 = _pilot_logs(oracle, "lattice_2d_10.gset", 8, 3000)
+    c2 = _logs_to_engine_hist(logs2, -200)
+    want = ptx.ladder_from_histograms(c2, -200, hb2, bs2, be2, mode=ptx.LADDER_EXACT)
+    got = ptx.ladder_from_histograms(c2, -200, hb2, bs2, be2, mode=ptx.LADDER_NEWTON)
+    assert abs(len(got) - len(want)) <= 1
+    k = min(len(got), len(want))
+    assert np.allclose(got[:k], want[:k], atol=0.03)
+
+
 # ------------------------------------------------------------- errors -------
 def test_errors_are_status_codes(ptx, tmp_path):
     with pytest.raises(ptx.PtxError) as ei:
