@@ -194,8 +194,9 @@ ptx_status ptx_engine_read_best(ptx_engine* e, int32_t inst, int64_t* best_e, in
                                 uint8_t* best_state);
 /* Histogram counts [nrep_slots][hist_bins], energy = e_base + 2*bin. */
 ptx_status ptx_engine_read_histograms(ptx_engine* e, int32_t inst, uint32_t* counts);
-/* Energies of every instance's replica at slot 0..nrep-1 in one call:
- * out[sum of nrep before inst + slot].  Used by time-to-target loops. */
+/* Lowest best energy of every instance over its replicas, out[n_instances]
+ * (INT64_MAX where nothing is recorded yet), in one call: the poll of
+ * time-to-target loops.  Best energies as in ptx_engine_read_best. */
 ptx_status ptx_engine_read_min_energies(ptx_engine* e, int64_t* out_per_instance);
 
 /* Device time of the kernels enqueued since the last reset, measured with

@@ -24,7 +24,11 @@ Status DevBuf<T>::alloc(size_t count) {
     if (count == 0) count = 1;
     PTX_TRY(hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"));
     n = count;
-    return hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+    // The fill runs on the null stream, which does not order against the
+    // non-blocking streams engines and networks use: wait for it here so a
+    // later copy or kernel on another stream cannot be overwritten by it.
+    PTX_TRY(hip_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset"));
+    return hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
 }
 template <class T>
 Status DevBuf<T>::upload(const std::vector<T>& v, hipStream_t s) {

@@ -203,6 +203,37 @@ class Surrogate:
         return [ladder_from_increments(y, bs, be) for y in self.predict(X)]
 
 
+def choose_ladders(ptx, sur, problems, stats, T0=0.5, T1=3.0, seed=0, short=(500, 1000), validate_steps=2000,
+                   long=(5000, 2000), inverse=False):
+    """Surrogate ladders with the multihistogram preprocess as fallback.
+
+    Every instance gets the network's ladder from its 8-slot pilot; a
+    2-chain validation run of validate_steps measures the adjacent swap
+    rates, and an instance whose rates are not all within TARGET +- TOL
+    falls back to the 64-slot multihistogram ladder (get_beta's).  Returns
+    (ladders, accepted flags, measured surrogate rates, seconds per stage)."""
+    t = {}
+    t0 = time.perf_counter()
+    X, bs, be = short_features(ptx, problems, stats, T0, T1, short[0], short[1], seed, inverse)
+    t["pilot8_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lads = sur.ladders(X, bs, be)
+    t["mlp_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rates = measured_rates(ptx, problems, lads, validate_steps, seed + 1)
+    t["validate_s"] = time.perf_counter() - t0
+    ok = [in_band(r) for r in rates]
+    fb = [i for i, o in enumerate(ok) if not o]
+    t0 = time.perf_counter()
+    if fb:
+        mh, _, _ = multihistogram_ladders(ptx, [problems[i] for i in fb], N_OUT, T0, T1, long[0], long[1], seed + 2,
+                                          inverse)
+        for i, lad in zip(fb, mh):
+            lads[i] = lad
+    t["fallback_s"] = time.perf_counter() - t0
+    return lads, ok, rates, t
+
+
 def chunked(seq, n):
     for i in range(0, len(seq), n):
         yield seq[i:i + n]

@@ -1,0 +1,213 @@
+"""Time to target energy: the batched engine with surrogate ladders against the
+plain single-instance path, on one GPU.
+
+Target energy E* of an instance: the lowest energy that the plain path
+(multihistogram --preprocess ladder, ICM, src/solver.cc:67-127) finds in a
+calibration of several seeds at 4x the measured run's step budget; stored in
+data/ttt_targets.json with the workload that produced it.
+
+Time to target of an instance = wall time from the start of its ladder
+construction until any of its replicas has energy <= E* (the step is read
+from the per-replica best-energy sweep, so polling granularity does not
+bias it).  Both paths are timed the same way:
+
+  plain     per instance, one after another: 64-slot GPU pilot + FP64
+            multihistogram ladder (Newton solve), then the single-instance
+            engine (cooperative kernel), polled every `poll` steps until E*
+            or the step budget;
+  batched   all instances at once: 8-slot pilot + network ladders +
+            validation run + multihistogram fallback, then one batched
+            engine (fused kernel) polled until 99 % of the instances reached
+            E* or the budget.
+
+instances/hour = instances finished (hit or budget spent) per wall hour.
+
+Both paths hand the engine their ladders in ascending beta order.  The
+reference's Nc rule (src/initialization.cc:158-164) counts ICM slots from the
+last slot down while beta >= the characteristic scale, so on the descending
+preprocess ladder it gives all slots or none (here none: the hottest beta is
+below 1); ascending, ICM runs on the cold slots, which is what "ICM on"
+means for this workload.
+
+    python pt-icm_amd/ttt.py calibrate     # writes data/ttt_targets.json (GPU)
+    python pt-icm_amd/ttt.py run           # one JSON line (GPU)
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import surrogate as S  # noqa: E402
+
+TARGETS = os.path.join(HERE, "data", "ttt_targets.json")
+T0, T1 = 0.5, 3.0
+
+
+def ttt_specs(n_inst, seed=0, L=32, n_er=1000, p_er=0.02):
+    """Half 2D +-J tori L x L, half Erdos-Renyi +-1 graphs G(n_er, p_er)."""
+    out = []
+    for i in range(n_inst):
+        if i % 2 == 0:
+            out.append({"family": "torus", "L": L, "pm": True, "seed": int(seed * 100003 + i)})
+        else:
+            out.append({"family": "er", "n": n_er, "p": p_er, "pm": True, "seed": int(seed * 100003 + i)})
+    return out
+
+
+def build_problems(ptx, specs):
+    coos = [S.build(s) for s in specs]
+    return [ptx.Problem.from_coo(*c) for c in coos], [S.problem_stats(c) for c in coos]
+
+
+def first_hits(eng, targets):
+    """Per instance: the first sweep at which a replica reached <= E* (-1: not reached)."""
+    out = np.full(len(targets), -1, np.int64)
+    for i, t in enumerate(targets):
+        be, bs, _ = eng.best(i, with_state=False)
+        hit = be <= t
+        if hit.any():
+            out[i] = int(bs[hit].min())
+    return out
+
+
+def calibrate(ptx, specs, seeds, steps, log=print):
+    probs, _ = build_problems(ptx, specs)
+    t0 = time.perf_counter()
+    lads, _, _ = S.multihistogram_ladders(ptx, probs, S.N_OUT, T0, T1, 5000, 2000, seed=7)
+    lads = [np.sort(l) for l in lads]
+    log(f"multihistogram ladders in {time.perf_counter() - t0:.1f} s")
+    best = np.full(len(probs), np.iinfo(np.int64).max)
+    for sd in seeds:
+        t0 = time.perf_counter()
+        eng = ptx.Engine(probs, lads, seed=sd, flags=ptx.FLAG_ICM)
+        eng.step(steps)
+        eng.sync()
+        m = eng.min_energies()
+        log(f"seed {sd}: {steps} steps in {time.perf_counter() - t0:.1f} s, improved {int(np.sum(m < best))}")
+        best = np.minimum(best, m)
+        del eng
+    return best, [len(l) for l in lads]
+
+
+def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print):
+    probs, stats = build_problems(ptx, specs)
+    targets = np.asarray(targets)
+    t_start = time.perf_counter()
+    sur = S.Surrogate(ptx)
+    lads, ok, _, stages = S.choose_ladders(ptx, sur, probs, stats, T0, T1, seed=seed)
+    lads = [np.sort(l) for l in lads]
+    t_lad = time.perf_counter() - t_start
+    log(f"batched ladders in {t_lad:.2f} s ({sum(ok)} of {len(ok)} surrogate ladders accepted) {stages}")
+    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        k = min(poll, max_steps - steps)
+        eng.step(k)
+        steps += k
+        if np.mean(eng.min_energies() <= targets) >= 0.99:
+            break
+    t_run = time.perf_counter() - t0
+    per_step = t_run / steps
+    hits = first_hits(eng, targets)
+    info = eng.info()
+    ttt = np.where(hits >= 0, t_lad + (hits + 1) * per_step, np.inf)
+    wall = t_lad + t_run
+    return {"instances": len(specs), "hit_rate": float(np.mean(hits >= 0)), "p50_ttt_s": float(np.median(ttt)),
+            "wall_s": wall, "ladder_s": t_lad, "run_s": t_run, "steps": steps, "ms_per_step": per_step * 1e3,
+            "instances_per_hour": len(specs) * 3600.0 / wall, "surrogate_accepted": int(sum(ok)),
+            "ladder_stages_s": {k: round(v, 3) for k, v in stages.items()},
+            "spin_updates_per_s": info.spin_updates_per_step * steps / t_run, "mode": info.mode,
+            "mean_slots": float(np.mean([len(l) for l in lads])),
+            "mean_icm_slots": float(np.mean([eng.instance_info(i).nc for i in range(len(specs))]))}
+
+
+def run_plain(ptx, specs, targets, max_steps, poll, seed, log=print):
+    probs, _ = build_problems(ptx, specs)
+    rows = []
+    for i, (p, t) in enumerate(zip(probs, targets)):
+        t0 = time.perf_counter()
+        lad, _, _ = S.multihistogram_ladders(ptx, [p], S.N_OUT, T0, T1, 5000, 2000, seed=seed + i, threads=1)
+        t_pre = time.perf_counter() - t0
+        eng = ptx.Engine(p, np.sort(lad[0]), seed=seed + 1000 + i, flags=ptx.FLAG_ICM)
+        steps = 0
+        t1 = time.perf_counter()
+        while steps < max_steps:
+            k = min(poll, max_steps - steps)
+            eng.step(k)
+            steps += k
+            if eng.min_energies()[0] <= t:
+                break
+        t_run = time.perf_counter() - t1
+        h = first_hits(eng, [t])[0]
+        rows.append({"pre_s": t_pre, "run_s": t_run, "steps": steps, "hit": bool(h >= 0),
+                     "ttt_s": t_pre + (h + 1) * t_run / steps if h >= 0 else float("inf"),
+                     "mode": eng.info().mode})
+        del eng
+    wall = sum(r["pre_s"] + r["run_s"] for r in rows)
+    return {"instances": len(rows), "hit_rate": float(np.mean([r["hit"] for r in rows])),
+            "p50_ttt_s": float(np.median([r["ttt_s"] for r in rows])), "wall_s": wall,
+            "mean_preprocess_s": float(np.mean([r["pre_s"] for r in rows])),
+            "instances_per_hour": len(rows) * 3600.0 / wall, "mode": rows[0]["mode"]}
+
+
+def load_targets(path=TARGETS):
+    with open(path) as f:
+        return json.load(f)
+
+
+def run(ptx, n_plain=8, seed=2024, log=print, path=TARGETS):
+    cfg = load_targets(path)
+    specs, targets = cfg["specs"], np.array(cfg["targets"], np.int64)
+    b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
+    # an even sample of both families (specs alternate torus, ER)
+    stride = max(2, 2 * (len(specs) // max(n_plain, 1) // 2))
+    idx = sorted(list(range(0, len(specs), stride))[: (n_plain + 1) // 2] +
+                 list(range(1, len(specs), stride))[: n_plain // 2])
+    pl = run_plain(ptx, [specs[i] for i in idx], targets[idx], cfg["max_steps"], cfg["poll"], seed + 7, log)
+    return {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "batched_surrogate": b, "plain": pl,
+            "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"]}
+
+
+def main():
+    ap = argparse.ArgumentParser(description="time to target energy, batched + surrogate vs plain")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("calibrate")
+    c.add_argument("--instances", type=int, default=512)
+    c.add_argument("--L", type=int, default=32)
+    c.add_argument("--n-er", type=int, default=1000)
+    c.add_argument("--p-er", type=float, default=0.02)
+    c.add_argument("--max-steps", type=int, default=10000)
+    c.add_argument("--poll", type=int, default=250)
+    c.add_argument("--seeds", type=int, default=4)
+    c.add_argument("--out", default=TARGETS)
+    r = sub.add_parser("run")
+    r.add_argument("--plain", type=int, default=8)
+    r.add_argument("--targets", default=TARGETS)
+    a = ap.parse_args()
+    ptx = S.load_ptx()
+    if a.cmd == "calibrate":
+        specs = ttt_specs(a.instances, 0, a.L, a.n_er, a.p_er)
+        best, slots = calibrate(ptx, specs, [101 + k for k in range(a.seeds)], 4 * a.max_steps,
+                                log=lambda *s: print(*s, flush=True))
+        out = {"workload": f"{a.instances // 2} 2D +-J tori L={a.L} + {a.instances - a.instances // 2} ER +-1 "
+                           f"G({a.n_er}, {a.p_er}); T {T0}..{T1}; 2 chains; ICM",
+               "specs": specs, "targets": [int(x) for x in best], "calibration_slots": slots,
+               "calibration": f"multihistogram ladders, {a.seeds} seeds x {4 * a.max_steps} steps",
+               "max_steps": a.max_steps, "poll": a.poll}
+        os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+        with open(a.out, "w") as f:
+            json.dump(out, f)
+        print(json.dumps({k: v for k, v in out.items() if k not in ("specs", "targets", "calibration_slots")}))
+    else:
+        print(json.dumps(run(ptx, a.plain, log=lambda *s: print(*s, file=sys.stderr, flush=True),
+                             path=a.targets)))
+
+
+if __name__ == "__main__":
+    main()
