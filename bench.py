@@ -118,6 +118,9 @@ def main():
                     help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
+    ap.add_argument("--ttt-plain", type=int, default=8,
+                    help="instances of the plain path timed for time-to-target (0 = skip the time-to-target part)")
+    ap.add_argument("--ttt-targets", default=os.path.join(REPO, "pt-icm_amd", "data", "ttt_targets.json"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
@@ -187,6 +190,19 @@ def main():
         out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
         out["batched"]["roofline"]["peak"] = lds_peak
         out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
+    if args.ttt_plain > 0 and os.path.exists(args.ttt_targets):
+        sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+        import ttt
+        r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=args.ttt_targets)
+        b, pl = r["batched_surrogate"], r["plain"]
+        out["time_to_target"] = {
+            "workload": r["workload"], "max_steps": r["max_steps"],
+            "batched_surrogate": {k: b[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
+                                                    "surrogate_accepted", "ladder_s", "run_s", "ms_per_step",
+                                                    "spin_updates_per_s", "mean_slots", "mean_icm_slots")},
+            "plain": {k: pl[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
+                                         "mean_preprocess_s")},
+            "instances_per_hour_ratio": r["instances_per_hour_ratio"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
     print(json.dumps(out))
