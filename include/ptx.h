@@ -208,9 +208,11 @@ typedef struct {
 ptx_status ptx_engine_timing_enable(ptx_engine* e, int32_t on);
 ptx_status ptx_engine_timing_get(ptx_engine* e, ptx_engine_timing* out);
 
-/* Fused mode: wall-clock microseconds that workgroup 0 spent in the sweep,
- * exchange and cluster-move phases since profiling was switched on (on >= 0
- * resets and switches; on < 0 only reads).  us may be NULL. */
+/* Fused and cooperative modes: wall-clock microseconds that workgroup 0
+ * spent in the sweep (us[0]), exchange (us[1]) and cluster-move (us[2])
+ * phases, and (cooperative mode) publishing states and waiting at the grid
+ * barrier (us[3]), since profiling was switched on (on >= 0 resets and
+ * switches; on < 0 only reads).  us may be NULL, else holds 4 values. */
 ptx_status ptx_engine_phase_profile(ptx_engine* e, int32_t on, double* us);
 
 /* Device self-test of the FP64 helpers the exchange uses (exp/log portable

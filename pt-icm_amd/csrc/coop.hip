@@ -64,32 +64,8 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
                 }
             }
         } else {
-            for (int g = (cs >> 2) + gl; 4 * g < ce; g += nthr) {
-                const int base = 4 * g;
-                int dE[4];
-                bool any_up = false;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int p = base + k;
-                    dE[k] = 0;
-                    if (p >= cs && p < ce) {
-                        const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
-                        dE[k] = sp[p] ? -2 * f : 2 * f;
-                        any_up |= dE[k] > 0;
-                    }
-                }
-                Philox4 rnd = {{0u, 0u, 0u, 0u}};
-                if (any_up) rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int p = base + k;
-                    if (p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1])) {
-                        sp[p] ^= 1u;
-                        dsum += dE[k];
-                        ++nacc;
-                    }
-                }
-            }
+            for (int g = (cs >> 2) + gl; 4 * g < ce; g += nthr)
+                group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc);
         }
         dsum = wave_sum_i32(dsum);
         nacc = wave_sum_i32(nacc);
@@ -163,7 +139,7 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     L.tpos = take(4 * nrep);
     L.level = take(4 * nrep);
     const int nw = (n + 31) / 32;
-    L.sp = take(32 * nw);
+    L.sp = take(spin_bytes(nw));
     L.icm = take(icm_scratch_bytes(n));
     L.acc = take(4 * 6);
     L.flag = take(4);
@@ -199,10 +175,10 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     int* flag = reinterpret_cast<int*>(smem + L.flag);
     IcmScratch sc;
     sc.S1 = reinterpret_cast<uint32_t*>(smem + L.icm);
-    sc.S2 = sc.S1 + nw;
-    sc.D = sc.S2 + nw;
-    sc.V = sc.D + nw;
-    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw);
+    sc.S2 = sc.S1 + (nw + 1);
+    sc.D = sc.S2 + (nw + 1);
+    sc.V = sc.D + (nw + 1);
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
 
     // ---- stage the instance and this replica ----
@@ -239,6 +215,15 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     st.flips = 0;
     unsigned long long nsw_total = 0;
     uint32_t cc = 0;  // running class counter of the accumulator rotation (acc[] is zero at start)
+    const bool prof = a.phase != nullptr && blockIdx.x == 0 && tid == 0;
+    unsigned long long t_mark = prof ? wall_clock64() : 0ull;
+    auto phase_done = [&](int k) {
+        if (prof) {
+            const unsigned long long t = wall_clock64();
+            a.phase[k] += t - t_mark;
+            t_mark = t;
+        }
+    };
     int it = 0;
     for (; it < nsteps; ++it) {
         const uint32_t step = step0 + (uint32_t)it;
@@ -250,6 +235,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             sweep_replica_wg<Fmt, SPL>(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, sp,
                                        thr_row, best_words, st, tid, nthr, acc, cc);
         }
+        phase_done(0);
         // ---- publish and wait ----
         pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
         if (tid == 0) c.pub_E[par * nrep + r] = st.E;
@@ -258,6 +244,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             break;
         for (int s = tid; s < nrep; s += nthr) E[s] = c.pub_E[par * nrep + s];
         __syncthreads();
+        phase_done(3);
         // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0) ----
         if (wave == 0) {
             if (r == 0) record_histogram(a, d, step, E, s2r, lane);
@@ -269,6 +256,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             }
         }
         __syncthreads();
+        phase_done(1);
         // ---- cluster move of this replica's slot pair (both partners compute it) ----
         if (d.nc > 0) {
             const int t = r2s[r] % nt;
@@ -289,6 +277,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                 __syncthreads();
             }
         }
+        phase_done(2);
     }
     // ---- write back ----
     pack_bits(sp, nw, a.spins + d.spin_off + (int64_t)r * nw, tid, nthr);

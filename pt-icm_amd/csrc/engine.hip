@@ -94,7 +94,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     for (const auto& pp : problems) {
         if (pp->max_deg > kMaxDegree)
             return Status::err(PTX_ERR_UNSUPPORTED, "a vertex has more than 511 neighbours");
-        if (pp->n > 32768) unit_ok = false;
+        if (pp->n > 32736) unit_ok = false;  // the sentinel position 32 * nw needs 15 bits
         for (int32_t wv : pp->w) unit_ok = unit_ok && (wv == 1 || wv == -1);
     }
     fmt = unit_ok ? kFmtUnit : kFmtWide;
@@ -119,9 +119,12 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         }
         d.nc = nc;
         d.inst_id = (uint32_t)i;
-        // adjacency rows by position, each padded to 16 bytes
+        // adjacency rows by position, each padded to 16 bytes (at least one
+        // chunk); padding records name the sentinel position 32 * nw
         d.pos_off = (int64_t)rowinfo.size();
         d.adj_off16 = (int64_t)adjw.size() / 4;
+        // (the wide format masks padding by weight 0, so there any in-range position will do)
+        const uint32_t sentinel = std::min(32u * (uint32_t)d.nw, 65535u);
         int64_t cur16 = 0;
         for (int32_t p = 0; p < P.n; ++p) {
             const int32_t o = P.pos2orig[p];
@@ -129,8 +132,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             if (cur16 >= (1ll << 23)) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
             rowinfo.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
             if (fmt == kFmtUnit) {
-                const int32_t chunks = (dg + 7) / 8;
-                std::vector<uint16_t> rec((size_t)chunks * 8, 0);
+                const int32_t chunks = std::max(1, (dg + 7) / 8);
+                std::vector<uint16_t> rec((size_t)chunks * 8, (uint16_t)sentinel);
                 for (int32_t j = 0; j < dg; ++j) {
                     const int32_t q = P.rowptr[o] + j;
                     rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 0x8000u : 0u));
@@ -138,13 +141,13 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
                 cur16 += chunks;
             } else {
-                const int32_t chunks = (dg + 3) / 4;
+                const int32_t chunks = std::max(1, (dg + 3) / 4);
                 for (int32_t j = 0; j < chunks * 4; ++j) {
                     if (j < dg) {
                         const int32_t q = P.rowptr[o] + j;
                         adjw.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
                     } else {
-                        adjw.push_back(0u);  // weight 0: contributes nothing
+                        adjw.push_back(sentinel);  // weight 0: contributes nothing
                     }
                 }
                 cur16 += chunks;
@@ -188,7 +191,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W <= 0) W = (total_rep < 4 * 256) ? 1 : 4;  // spread few replicas over CUs, group many
     if (W > 8) W = 8;                                 // split kernels take at most 8 waves
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
-    spin_stride = (int32_t)align_up((int64_t)nw_max * 32, 16);
+    spin_stride = (int32_t)align_up((int64_t)spin_bytes(nw_max), 16);
     thr_stride = (int32_t)align_up(fmax_max, 4);
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
     if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
@@ -417,7 +420,7 @@ Status Engine::launch_step(uint32_t s) {
         }));
     }
     if (!pairs.empty()) {
-        const size_t lds = (size_t)nw_max * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
+        const size_t lds = (size_t)(nw_max + 1) * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
         PTX_TRY(timed(2, [&] {
             if (fmt == kFmtUnit)
                 hipLaunchKernelGGL(k_icm<kFmtUnit>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
@@ -601,7 +604,7 @@ Status Engine::phase_profile(int on, double* us) {
         if (d_phase.p) PTX_TRY(d_phase.download(t, 4, 0, stream));
         int rate_khz = 0;
         PTX_TRY(hip_check(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0), "wall clock rate"));
-        for (int k = 0; k < 3; ++k) us[k] = rate_khz > 0 ? (double)t[k] * 1e3 / rate_khz : 0.0;
+        for (int k = 0; k < 4; ++k) us[k] = rate_khz > 0 ? (double)t[k] * 1e3 / rate_khz : 0.0;
     }
     return Status::ok();
 }

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
     uint32_t* words = a.spins + d.spin_off + (int64_t)r * d.nw;
     if (mode == 0) {
         const uint32_t c3 = ctr3(kTagInit | a.tag_base, d.inst_id);
-        for (int p = lane; p < d.nw * 32; p += kWave) {
+        for (int p = lane; p < spin_bytes(d.nw); p += kWave) {  // includes the zero sentinel word
             uint8_t b = 0;
             if (p < d.n) {
                 const uint32_t i = (uint32_t)a.p2o[d.pos_off + p];
@@ -173,10 +173,10 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
     const int t = pr.y;
     IcmScratch sc;
     sc.S1 = reinterpret_cast<uint32_t*>(smem);
-    sc.S2 = sc.S1 + nw_max;
-    sc.D = sc.S2 + nw_max;
-    sc.V = sc.D + nw_max;
-    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw_max);
+    sc.S2 = sc.S1 + (nw_max + 1);
+    sc.D = sc.S2 + (nw_max + 1);
+    sc.V = sc.D + (nw_max + 1);
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw_max + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
     const int64_t ro = d.rep_off;
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
@@ -241,7 +241,7 @@ __host__ __device__ inline FusedLayout fused_layout(int n, int adj_len16, int nr
     L.level = o;
     o += align16(4 * nrep);
     const int nw = (n + 31) / 32;
-    const int sweep_bytes = 32 * nw;
+    const int sweep_bytes = spin_bytes(nw);
     const int icm_bytes = icm ? icm_scratch_bytes(n) : 0;
     L.scratch_stride = align16(sweep_bytes > icm_bytes ? sweep_bytes : icm_bytes);
     L.scratch = o;
@@ -304,10 +304,10 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     unsigned long long nsw_total = 0;
     IcmScratch sc;
     sc.S1 = reinterpret_cast<uint32_t*>(scratch);
-    sc.S2 = sc.S1 + nw;
-    sc.D = sc.S2 + nw;
-    sc.V = sc.D + nw;
-    sc.tail = reinterpret_cast<int32_t*>(sc.V + nw);
+    sc.S2 = sc.S1 + (nw + 1);
+    sc.D = sc.S2 + (nw + 1);
+    sc.V = sc.D + (nw + 1);
+    sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
 
     const bool prof = a.phase != nullptr && blockIdx.x == 0 && tid == 0;

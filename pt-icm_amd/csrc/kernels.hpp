@@ -15,9 +15,11 @@ namespace ptx {
 
 constexpr int kWave = 64;
 
-// Adjacency record formats.  Wide: u32 = position | (int16 weight << 16),
-// padding records have weight 0.  Unit: u16 = position | (weight < 0) << 15,
-// for |w| == 1 and n <= 32768.  Rows start on 16-byte boundaries.
+// Adjacency record formats.  Wide: u32 = position | (int16 weight << 16).
+// Unit: u16 = position | (weight < 0) << 15, for |w| == 1 and n <= 32736.
+// Rows start on 16-byte boundaries and take at least one 16-byte chunk;
+// padding records name the sentinel position 32 * nw (weight 0 / sign 0),
+// whose spin byte is kept 0, so they add nothing to a local field.
 enum : int { kFmtWide = 0, kFmtUnit = 1 };
 constexpr int kMaxDegree = 511;
 
@@ -131,21 +133,40 @@ struct Csr {
         const int d = (int)(ri & (uint32_t)kMaxDegree);
         const uint4* r = reinterpret_cast<const uint4*>(base + (size_t)(ri >> 9) * 16u);
         if (Fmt == kFmtUnit) {
-            // All 8 gathers of a chunk are issued unconditionally (padding
-            // records read position 0) and masked afterwards, so they overlap
-            // instead of each waiting on the previous one.
+            // Padding records read the zero sentinel byte with sign 0, so every
+            // gather of a chunk is issued unconditionally and none is masked.
+            // Rows of more than one chunk load up to four chunks before any
+            // gather, so their latencies overlap.
+            const int nch = (d + 7) >> 3;
             int agree = 0;  // neighbours with s_j * sign(w_pj) = +1
-            for (int c = 0; 8 * c < d; ++c) {
-                const uint4 v = r[c];
+            if (nch <= 1) {
+                const uint4 v = r[0];
                 const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
                 uint32_t b[8];
 #pragma unroll
                 for (int h = 0; h < 8; ++h) b[h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
-                const int live = d - 8 * c;
 #pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    const uint32_t sgn = (wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u;
-                    agree += (h < live) ? (int)(b[h] ^ sgn) : 0;
+                for (int h = 0; h < 8; ++h) agree += (int)(b[h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                return d > 0 ? 2 * agree - d : 0;
+            }
+            for (int c0 = 0; c0 < nch; c0 += 4) {
+                uint4 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = r[min(c0 + q, nch - 1)];
+                uint32_t b[4][8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t wv[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) b[q][h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t wv[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+                    int a = 0;
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) a += (int)(b[q][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                    agree += c0 + q < nch ? a : 0;
                 }
             }
             return 2 * agree - d;
@@ -162,15 +183,86 @@ struct Csr {
         }
         return f;
     }
+    // Local fields of the four consecutive positions p0..p0+3 (positions
+    // outside [cs, ce) get 0).  The row words, chunks and neighbour bytes of
+    // all four are loaded before any is used, so their latencies overlap;
+    // padding records read the zero sentinel byte and need no mask.
+    __device__ __forceinline__ void group_field(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
+        uint32_t ri[4];
+        int d[4], nch[4];
+        int mx = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = p0 + k;
+            const bool in = p >= cs && p < ce;
+            const uint32_t v = info[in ? p : cs];
+            ri[k] = in ? v : (v & ~(uint32_t)kMaxDegree);  // out of the class: degree 0, a valid row
+            d[k] = (int)(ri[k] & (uint32_t)kMaxDegree);
+            nch[k] = (d[k] + kPerChunk - 1) / kPerChunk;
+            mx = max(mx, nch[k]);
+            f[k] = 0;
+        }
+        for (int ch = 0; ch < mx; ++ch) {
+            uint4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int cc = min(ch, max(nch[k] - 1, 0));
+                v[k] = reinterpret_cast<const uint4*>(base + (size_t)(ri[k] >> 9) * 16u)[cc];
+            }
+            if (Fmt == kFmtUnit) {
+                uint32_t b[4][8];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) b[k][h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                    int a = 0;
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) a += (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                    f[k] += ch < nch[k] ? a : 0;
+                }
+            } else {
+                uint32_t b[4][4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) b[k][h] = sp[wv[h] & 0xFFFFu];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                    int a = 0;
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        const int w = (int32_t)wv[h] >> 16;
+                        a += b[k][h] ? w : -w;
+                    }
+                    f[k] += ch < nch[k] ? a : 0;
+                }
+            }
+        }
+        if (Fmt == kFmtUnit) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) f[k] = 2 * f[k] - d[k];
+        }
+    }
 };
 
 // Bytes <-> bits.  A state word covers 32 consecutive positions; the byte
-// buffer has nw*32 entries (padding bytes stay 0).  lane/stride let several
-// waves share the work.
+// buffer has (nw + 1) * 32 entries: the last 32 (the sentinel word the
+// padding records read) and the padding bytes past n stay 0.  lane/stride
+// let several waves share the work.
+__host__ __device__ inline int spin_bytes(int nw) { return 32 * (nw + 1); }
+
 __device__ __forceinline__ void expand_bits(const uint32_t* __restrict__ words, int nw, uint8_t* sp, int lane,
                                             int stride = kWave) {
-    for (int w = lane; w < nw; w += stride) {
-        const uint32_t v = words[w];
+    for (int w = lane; w <= nw; w += stride) {
+        const uint32_t v = w < nw ? words[w] : 0u;
         uint32_t* dst = reinterpret_cast<uint32_t*>(sp + 32 * w);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -207,6 +299,44 @@ struct SweepState {
     unsigned long long flips;
 };
 
+// Metropolis updates of the positions 4g..4g+3 that lie in the class
+// [cs, ce): one Philox block serves the four (word k for position 4g+k);
+// the four spin bytes are read and written as one 32-bit word (no other lane
+// owns any byte of it during the class).
+template <int Fmt>
+__device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs, int ce,
+                                             const int32_t* __restrict__ hf, bool has_field, uint32_t c3, uint32_t k0,
+                                             uint32_t k1, int r, uint32_t gsweep, uint8_t* sp,
+                                             const uint32_t* thr_row, int& dsum, int& nacc) {
+    const int p0 = 4 * g;
+    int f[4];
+    csr.group_field(p0, cs, ce, sp, f);
+    const uint32_t own = *reinterpret_cast<const uint32_t*>(sp + p0);
+    int dE[4];
+    bool any_up = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = p0 + k;
+        const bool in = p >= cs && p < ce;
+        const int fk = f[k] + ((has_field && in) ? hf[p] : 0);
+        dE[k] = in ? (((own >> (8 * k)) & 1u) ? -2 * fk : 2 * fk) : 0;
+        any_up |= dE[k] > 0;
+    }
+    Philox4 rnd = {{0u, 0u, 0u, 0u}};
+    if (any_up) rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
+    uint32_t flip = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = p0 + k;
+        if (p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1])) {
+            flip |= 1u << (8 * k);
+            dsum += dE[k];
+            ++nacc;
+        }
+    }
+    if (flip) *reinterpret_cast<uint32_t*>(sp + p0) = own ^ flip;
+}
+
 template <int Fmt>
 __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
                                               const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
@@ -218,33 +348,7 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
         for (int g = (cs >> 2) + lane; 4 * g < ce; g += kWave) {
-            const int base = 4 * g;
-            int dE[4];
-            bool any_up = false;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int p = base + k;
-                dE[k] = 0;
-                if (p >= cs && p < ce) {
-                    const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
-                    dE[k] = sp[p] ? -2 * f : 2 * f;
-                    any_up |= dE[k] > 0;
-                }
-            }
-            Philox4 rnd = {{0u, 0u, 0u, 0u}};
-            if (any_up) rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int p = base + k;
-                if (p >= cs && p < ce) {
-                    const bool acc = dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1];
-                    if (acc) {
-                        sp[p] ^= 1u;
-                        dsum += dE[k];
-                        ++nacc;
-                    }
-                }
-            }
+            group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc);
         }
         wave_sync();
         dsum = wave_sum_i32(dsum);
@@ -296,12 +400,104 @@ struct ExchScratch {
     int32_t* level;  // [nrep]
 };
 
+// The same exchange with the permutation held in registers, lane k owning
+// slot k (nrep <= 64): sort ranks, dependency levels and the level-by-level
+// decisions move between neighbouring slots with shuffles instead of LDS
+// round trips and wave barriers.  Bit-identical to the LDS form.
+__device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep, bool pilot, uint64_t seed,
+                                                                uint32_t tag_base, uint32_t inst_id, uint32_t step,
+                                                                const double* beta, const int32_t* E, int32_t* s2r,
+                                                                uint32_t* att, uint32_t* succ, int lane) {
+    const int np = nt - 1;
+    const int nhalf = pilot ? 1 : 2;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t c3s = ctr3(kTagShuf | tag_base, inst_id);
+    const uint32_t c3e = ctr3(kTagExch | tag_base, inst_id);
+    const bool valid = lane < nrep;
+    const int kc = valid ? lane % nt : 0;
+    const int my_chain = valid ? lane / nt : 0;
+    const bool is_pair = valid && kc < np;  // slot k has a right neighbour in its chain
+    int rep = valid ? s2r[lane] : 0;
+    int32_t es = valid ? E[rep] : 0;
+    const double bk = valid ? beta[lane] : 0.0;
+    const double bk1 = __shfl_down(bk, 1, kWave);  // beta of slot k + 1 (used only when is_pair)
+    uint32_t natt = 0, nsucc = 0;
+    unsigned long long nsw = 0;
+    for (int h = 0; h < nhalf; ++h) {
+        // rank of pair kc in this half's order: pairs sorted by (key, index)
+        const uint32_t key =
+            is_pair ? pick(philox4x32_10((uint32_t)kc >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)kc) : 0u;
+        int rank = 0;
+        for (int i = 0; i < np; ++i) {
+            const uint32_t ki = __shfl(key, i, kWave);  // lane i < np holds pair i of chain 0
+            rank += (ki < key) || (ki == key && i < kc);
+        }
+        const int a = h * np + rank;
+        const int chain = pilot ? 0 : (a & 1);
+        const int tpos = (is_pair && chain == my_chain) ? a : -1;
+        const double lu =
+            tpos >= 0 ? uniform_log(pick(philox4x32_10((uint32_t)a >> 2, 0u, step, c3e, k0, k1), (uint32_t)a)) : 0.0;
+        // dependency levels: 1 + the highest level of an earlier attempt on k-1 or k+1
+        int level = 0;
+        while (true) {
+            const int tl = __shfl_up(tpos, 1, kWave), ll = __shfl_up(level, 1, kWave);
+            const int tr = __shfl_down(tpos, 1, kWave), lr = __shfl_down(level, 1, kWave);
+            int lv = 0;
+            if (tpos >= 0) {
+                lv = 1;
+                if (kc > 0 && tl >= 0 && tl < tpos) lv = max(lv, ll + 1);
+                if (kc + 1 < np && tr >= 0 && tr < tpos) lv = max(lv, lr + 1);
+            }
+            const bool changed = __ballot(lv != level) != 0ull;
+            level = lv;
+            if (!changed) break;
+        }
+        const int maxlv = wave_max_i32(level);
+        for (int L = 1; L <= maxlv; ++L) {
+            const int rep_r = __shfl_down(rep, 1, kWave);
+            const int32_t es_r = __shfl_down(es, 1, kWave);
+            int sw = 0;
+            if (level == L) {
+                const double lb = (bk - bk1) * (double)(es - es_r);
+                ++natt;
+                if (lu < lb) {
+                    sw = 1;
+                    ++nsucc;
+                    ++nsw;
+                }
+            }
+            const int rep_l = __shfl_up(rep, 1, kWave);
+            const int32_t es_l = __shfl_up(es, 1, kWave);
+            const int swl = __shfl_up(sw, 1, kWave);
+            if (sw) {
+                rep = rep_r;
+                es = es_r;
+            } else if (kc > 0 && swl) {  // the pair on the left swapped slots k-1 and k
+                rep = rep_l;
+                es = es_l;
+            }
+        }
+    }
+    if (valid) {
+        s2r[lane] = rep;
+        att[lane] += natt;
+        succ[lane] += nsucc;
+    }
+    return nsw;
+}
+
 __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bool pilot, uint64_t seed,
                                                             uint32_t tag_base, uint32_t inst_id, uint32_t step,
                                                             const double* beta, const int32_t* E, int32_t* s2r,
                                                             uint32_t* att, uint32_t* succ, ExchScratch x, int lane) {
     const int np = nt - 1;
     if (np <= 0) return 0ull;
+    if (nrep <= kWave) {
+        const unsigned long long r = exchange_wave_reg(nt, nrep, pilot, seed, tag_base, inst_id, step, beta, E, s2r,
+                                                       att, succ, lane);
+        wave_sync();
+        return r;
+    }
     const int nhalf = pilot ? 1 : 2;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const uint32_t c3s = ctr3(kTagShuf | tag_base, inst_id);
@@ -595,9 +791,11 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
 }
 
 // LDS bytes of the ICM scratch for n spins (S1,S2,D,V words, tail, queue of n/2+2).
+// The four word arrays have nw + 1 entries so that padding records (which
+// name position 32 * nw) stay inside them.
 __host__ __device__ inline int icm_scratch_bytes(int n) {
     const int nw = (n + 31) / 32;
-    return 16 * nw + 16 + 2 * (n / 2 + 2);
+    return 16 * (nw + 1) + 16 + 2 * (n / 2 + 2);
 }
 
 }  // namespace ptx

@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--waves", type=int, default=0, help="waves per workgroup (0 = engine default)")
     a = ap.parse_args()
     ptx = load_ptx()
     if a.what == "batched":
@@ -41,7 +42,8 @@ def main():
         _, coos = instances.ea_batch(a.instances, 0)
         probs = [ptx.Problem.from_coo(*c) for c in coos]
         lad = ptx.beta_linspace(0.2, 3.0, 32, True)
-        eng = ptx.Engine(probs, [lad] * len(probs), seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
+        eng = ptx.Engine(probs, [lad] * len(probs), seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED,
+                         waves_per_group=a.waves)
         eng.step(8)
         eng.sync()
         t0 = time.perf_counter()
@@ -49,7 +51,8 @@ def main():
         eng.sync()
         dt = time.perf_counter() - t0
         info = eng.info()
-        print(json.dumps({"instances": len(probs), "steps": a.steps, "ms_per_step": dt * 1e3 / a.steps,
+        print(json.dumps({"instances": len(probs), "steps": a.steps, "waves": info.waves_per_group,
+                          "ms_per_step": dt * 1e3 / a.steps,
                           "spin_updates_per_s": info.spin_updates_per_step * a.steps / dt,
                           "lds_bytes_per_step": info.lds_bytes_per_step,
                           "steps_per_launch": info.steps_per_launch}))

@@ -383,8 +383,9 @@ class Engine:
         return out
 
     def phase_profile(self, on=None):
-        """on=True/False resets and switches; None reads [sweep, exchange, icm] microseconds (workgroup 0)."""
-        us = np.zeros(3, np.float64)
+        """on=True/False resets and switches; None reads [sweep, exchange, icm, barrier] microseconds
+        (workgroup 0)."""
+        us = np.zeros(4, np.float64)
         check(lib().ptx_engine_phase_profile(self.h, -1 if on is None else int(on), _ptr(us, C.c_double)))
         return us
 
