@@ -25,10 +25,13 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 GRAPHS = os.path.join(REPO, "tests", "golden", "graphs")
 LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
-# HBM bytes per 16-step k_fused launch of the batched workload, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE runs of `microbench.py batched`, scaled by
-# the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1).
-BATCHED_HBM_BYTES_PER_LAUNCH = (2 * 58156.7 + 37864.6) * 1024
+# HBM bytes per launch of the batched workload's dominant kernel, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `microbench.py batched`,
+# scaled by the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1):
+# k_sweep of the split kernels (one wave per replica), and k_fused with 16
+# waves and 16 steps per launch.
+BATCHED_HBM_BYTES = {"k_sweep": (2 * 362523.0 + 73585.0) * 1024,
+                     "k_fused": (2 * 69305.5 + 475067.6) * 1024}
 
 
 def load_ptx():
@@ -77,13 +80,17 @@ def cpu_baseline(problem_path, betas, seconds, seed=0):
 def batched_ea(ptx, steps, seed, n_inst=4096):
     """The batched workload: 4096 independent 3D Edwards-Anderson +-J instances,
     L uniform in {6, 8, 10, 12}, 32 inverse temperatures 0.2..3.0 x 2 chains,
-    ICM by the slot rule, fused per-instance kernel."""
+    ICM by the slot rule.  The launch shape comes from the autotuner (timed
+    separately); fused launches are replayed from a hipGraph of 4 launches."""
     sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
     import instances
     _, coos = instances.ea_batch(n_inst, seed)
     probs = [ptx.Problem.from_coo(*c) for c in coos]
     lad = ptx.beta_linspace(0.2, 3.0, 32, True)
-    eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
+    t0 = time.perf_counter()
+    kw, trials = ptx.autotune(probs, [lad] * n_inst, seed=seed, steps=32, flags=ptx.FLAG_ICM)
+    t_tune = time.perf_counter() - t0
+    eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, use_graph=4, **kw)
     eng.step(16)
     eng.sync()
     eng.timing(True)
@@ -94,16 +101,28 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
     dt = time.perf_counter() - t0
     tm = eng.timing()
     info = eng.info()
-    avg_ms = tm.fused_ms / max(tm.fused_launches, 1)
-    bpl = info.lds_bytes_per_step * info.steps_per_launch
+    if info.mode == 2:
+        kern, avg_ms = "k_fused", tm.fused_ms / max(tm.fused_launches, 1)
+        bpl = info.lds_bytes_per_step * info.steps_per_launch
+    else:
+        kern, avg_ms = "k_sweep", tm.sweep_ms / max(tm.sweep_launches, 1)
+        bpl = info.lds_bytes_per_step
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
     return {"workload": "4096 x 3D EA +-J, L in {6,8,10,12}, 32 betas 0.2..3.0, 2 chains, ICM",
             "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
             "ms_per_step": dt * 1e3 / steps, "steps": steps, "instances_per_hour_at_10k_steps":
-                n_inst * 3600.0 / (dt / steps * 10000), "waves_per_group": info.waves_per_group,
-            "roofline": {"bound": "lds", "achieved": bpl / (avg_ms * 1e-3) / 1e9, "peak": LDS_PEAK_GBS,
-                         "unit": "GB/s", "frac": bpl / (avg_ms * 1e-3) / 1e9 / LDS_PEAK_GBS,
-                         "traffic": BATCHED_HBM_BYTES_PER_LAUNCH if info.steps_per_launch == 16 else None,
-                         "kernel": "k_fused", "avg_launch_ms": avg_ms, "steps_per_launch": info.steps_per_launch}}
+                n_inst * 3600.0 / (dt / steps * 10000),
+            "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
+                             "csr_in_lds": info.csr_in_lds, "steps_per_launch": info.steps_per_launch,
+                             "graph_launches": 4 if info.mode == 2 else 0},
+            "autotune_s": t_tune, "autotune_trials": [{k: (round(v, 1) if k == "us_per_step" else v)
+                                                       for k, v in t.items()} for t in trials],
+            "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / LDS_PEAK_GBS,
+                         "traffic": BATCHED_HBM_BYTES["k_sweep"] if info.mode == 1 and info.waves_per_group == 1
+                         else (BATCHED_HBM_BYTES["k_fused"] if info.mode == 2 and info.waves_per_group == 16
+                               and info.steps_per_launch == 16 else None), "kernel": kern, "avg_launch_ms": avg_ms,
+                         "steps_per_launch": info.steps_per_launch}}
 
 
 def main():
@@ -199,7 +218,8 @@ def main():
             "workload": r["workload"], "max_steps": r["max_steps"],
             "batched_surrogate": {k: b[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                                     "surrogate_accepted", "ladder_s", "run_s", "ms_per_step",
-                                                    "spin_updates_per_s", "mean_slots", "mean_icm_slots")},
+                                                    "spin_updates_per_s", "mean_slots", "mean_icm_slots", "mode",
+                                                    "waves_per_group", "autotune_s")},
             "plain": {k: pl[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                          "mean_preprocess_s")},
             "instances_per_hour_ratio": r["instances_per_hour_ratio"]}

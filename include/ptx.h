@@ -141,7 +141,8 @@ typedef struct {
     int32_t mode;            /* PTX_MODE_* */
     int32_t waves_per_group; /* replicas per workgroup (0 = autotuned) */
     int32_t steps_per_launch;/* fused mode: PT steps per launch (0 = auto) */
-    int32_t use_graph;       /* capture step sequences in a hipGraph */
+    int32_t use_graph;       /* fused mode: G > 0 captures G launches of steps_per_launch steps in a
+                                hipGraph and replays it (0 = eager launches) */
     int32_t spins_per_lane;  /* cooperative mode: 1 or 4 positions per lane (0 = auto) */
     int32_t csr_placement;   /* fused mode: PTX_CSR_AUTO, PTX_CSR_LDS or PTX_CSR_GLOBAL */
 } ptx_engine_config;
@@ -155,6 +156,24 @@ ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_insta
                              const int32_t* n_temps, uint64_t seed, const ptx_engine_config* cfg,
                              ptx_engine** out);
 void ptx_engine_destroy(ptx_engine* e);
+
+/* Launch-shape autotuner (no counterpart in the reference, which has one
+ * CPU code path).  Times `steps` PT steps (after a warm-up launch) of every
+ * launch shape that fits the batch -- execution mode, waves (replicas) per
+ * workgroup, positions per lane, adjacency in LDS or read through L2 -- and
+ * writes the fastest into *best (a copy of *base with mode, waves_per_group,
+ * spins_per_lane and csr_placement set).  The random streams are indexed by
+ * position, replica and step, so every shape yields the same trajectories.
+ * trials (may be NULL) receives up to max_trials measured shapes, *n_trials
+ * their count. */
+typedef struct {
+    int32_t mode, waves_per_group, spins_per_lane, csr_placement;
+    double us_per_step;
+} ptx_tune_trial;
+ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
+                               const int32_t* n_temps, uint64_t seed, const ptx_engine_config* base, int32_t steps,
+                               ptx_engine_config* best, ptx_tune_trial* trials, int32_t max_trials,
+                               int32_t* n_trials);
 /* Enqueues n_steps PT steps (asynchronous; errors of the launch are reported). */
 ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps);
 ptx_status ptx_engine_sync(ptx_engine* e);

@@ -3,6 +3,7 @@
 // status code; nothing here exits the process.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -256,6 +257,63 @@ ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_insta
 }
 
 void ptx_engine_destroy(ptx_engine* e) { delete e; }
+
+ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
+                               const int32_t* n_temps, uint64_t seed, const ptx_engine_config* base, int32_t steps,
+                               ptx_engine_config* best, ptx_tune_trial* trials, int32_t max_trials,
+                               int32_t* n_trials) {
+    return guarded([&]() -> ptx_status {
+        if (!problems || !betas || !n_temps || !best || n_instances <= 0 || steps <= 0)
+            return fail(PTX_ERR_INVALID, "bad autotune arguments");
+        ptx_engine_config b0;
+        if (base) b0 = *base;
+        else ptx_engine_config_default(&b0);
+        std::vector<std::shared_ptr<const ptx::Problem>> ps(n_instances);
+        for (int32_t i = 0; i < n_instances; ++i) {
+            if (!problems[i]) return fail(PTX_ERR_INVALID, "null problem");
+            ps[i] = problems[i]->p;
+        }
+        struct Shape {
+            int32_t mode, w, spl, csr;
+        };
+        std::vector<Shape> shapes;
+        if (n_instances == 1)
+            for (int w : {4, 8})
+                for (int spl : {1, 4}) shapes.push_back({PTX_MODE_COOP, w, spl, PTX_CSR_AUTO});
+        for (int w : {4, 8, 16})
+            for (int csr : {PTX_CSR_LDS, PTX_CSR_GLOBAL}) shapes.push_back({PTX_MODE_FUSED, w, 0, csr});
+        for (int w : {1, 4}) shapes.push_back({PTX_MODE_SPLIT, w, 0, PTX_CSR_AUTO});
+        double best_us = 0;
+        int32_t cnt = 0;
+        bool found = false;
+        for (const Shape& sh : shapes) {
+            ptx_engine_config c = b0;
+            c.mode = sh.mode;
+            c.waves_per_group = sh.w;
+            c.spins_per_lane = sh.spl;
+            c.csr_placement = sh.csr;
+            ptx::Engine e;
+            if (!e.create(ps, betas, n_temps, seed, c)) continue;  // shape does not fit this batch
+            if (e.mode != sh.mode) continue;
+            const int64_t warm = e.mode == PTX_MODE_SPLIT ? 2 : e.K;
+            if (!e.step(warm) || !e.sync()) continue;
+            const auto t0 = std::chrono::steady_clock::now();
+            if (!e.step(steps) || !e.sync()) continue;
+            const double us =
+                std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / steps;
+            if (trials && cnt < max_trials) trials[cnt] = ptx_tune_trial{sh.mode, e.mode == PTX_MODE_FUSED ? e.Wf : (e.mode == PTX_MODE_COOP ? e.Wf : e.W), sh.spl, sh.csr, us};
+            ++cnt;
+            if (!found || us < best_us) {
+                found = true;
+                best_us = us;
+                *best = c;
+            }
+        }
+        if (n_trials) *n_trials = cnt;
+        if (!found) return fail(PTX_ERR_UNSUPPORTED, "no launch shape fits this batch");
+        return PTX_OK;
+    });
+}
 
 ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps) {
     if (!e) return fail(PTX_ERR_INVALID, "null engine");

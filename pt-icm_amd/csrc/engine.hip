@@ -49,6 +49,8 @@ Status DevBuf<T>::download(T* v, size_t count, size_t offset, hipStream_t s) con
 }
 
 Engine::~Engine() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (auto& t : pending) {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
@@ -540,9 +542,29 @@ Status Engine::step(int64_t n) {
         return Status::ok();
     }
     if (mode == PTX_MODE_FUSED) {
+        const int64_t G = cfg.use_graph > 0 ? cfg.use_graph : 0;
         while (n > 0) {
+            if (G > 0 && graph_exec == nullptr && steps_done > 0 && n >= G * K) PTX_TRY(build_graph());
+            if (graph_exec != nullptr && n >= G * K) {
+                Timed t;
+                if (timing) {
+                    t.kind = 3;
+                    t.count = (int)G;
+                    PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
+                    PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
+                    PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
+                }
+                PTX_TRY(hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch"));
+                if (timing) {
+                    PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
+                    pending.push_back(t);
+                }
+                steps_done += G * K;
+                n -= G * K;
+                continue;
+            }
             const int k = (int)std::min<int64_t>(n, K);
-            PTX_TRY(launch_fused(k));
+            PTX_TRY(launch_fused(k));  // the first launch also sets the kernel's LDS attribute
             steps_done += k;
             n -= k;
         }
@@ -553,6 +575,26 @@ Status Engine::step(int64_t n) {
         ++steps_done;
     }
     return Status::ok();
+}
+
+// Captures use_graph launches of K steps on the engine stream.  Nothing in a
+// fused launch is a memset or a host copy, so the graph holds kernel nodes only.
+Status Engine::build_graph() {
+    const bool was_timing = timing;
+    timing = false;
+    PTX_TRY(hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"));
+    Status st = Status::ok();
+    for (int32_t g = 0; g < cfg.use_graph && st; ++g) st = launch_fused(K);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &gr);
+    timing = was_timing;
+    if (!st) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return st;
+    }
+    PTX_TRY(hip_check(e, "hipStreamEndCapture"));
+    graph = gr;
+    return hip_check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
 }
 
 Status Engine::sync() {
@@ -571,7 +613,7 @@ Status Engine::collect_timing() {
         float m = 0.f;
         PTX_TRY(hip_check(hipEventElapsedTime(&m, t.a, t.b), "hipEventElapsedTime"));
         ms[t.kind] += m;
-        launches[t.kind] += 1;
+        launches[t.kind] += t.count;
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
     }

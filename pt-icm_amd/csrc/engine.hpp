@@ -68,6 +68,7 @@ struct Engine {
     struct Timed {
         hipEvent_t a, b;
         int kind;
+        int count = 1;  // kernel launches between the two events
     };
     std::vector<Timed> pending;
     double ms[4] = {0, 0, 0, 0};
@@ -93,6 +94,10 @@ struct Engine {
     int32_t spl = 1;             // positions per lane (cooperative mode)
     int32_t coop_lds = 0;
     bool phase_on = false;
+    // fused mode with use_graph = G > 0: G launches of K steps captured once
+    // and replayed (the kernels read their step counter from device memory)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
 
     ~Engine();
     Status create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -116,6 +121,7 @@ struct Engine {
     Status launch_step(uint32_t s);
     Status launch_fused(int nsteps);
     Status launch_coop(int nsteps);
+    Status build_graph();
     Status collect_timing();
 };
 

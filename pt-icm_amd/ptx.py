@@ -52,6 +52,11 @@ class InstanceInfo(C.Structure):
                 ("n_colours", C.c_int32), ("hist_bins", C.c_int32), ("e_base", C.c_int64)]
 
 
+class TuneTrial(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("waves_per_group", C.c_int32), ("spins_per_lane", C.c_int32),
+                ("csr_placement", C.c_int32), ("us_per_step", C.c_double)]
+
+
 class Timing(C.Structure):
     _fields_ = [("sweep_ms", C.c_double), ("exchange_ms", C.c_double), ("icm_ms", C.c_double),
                 ("fused_ms", C.c_double), ("sweep_launches", C.c_int64), ("exchange_launches", C.c_int64),
@@ -90,6 +95,9 @@ _SIGS = {
     "ptx_engine_create": (C.c_int32, [_P(_vp), C.c_int32, _P(C.c_double), _P(C.c_int32), C.c_uint64,
                                       _P(EngineConfig), _P(_vp)]),
     "ptx_engine_destroy": (None, [_vp]),
+    "ptx_engine_autotune": (C.c_int32, [_P(_vp), C.c_int32, _P(C.c_double), _P(C.c_int32), C.c_uint64,
+                                        _P(EngineConfig), C.c_int32, _P(EngineConfig), _P(TuneTrial), C.c_int32,
+                                        _P(C.c_int32)]),
     "ptx_engine_step": (C.c_int32, [_vp, C.c_int64]),
     "ptx_engine_sync": (C.c_int32, [_vp]),
     "ptx_engine_info_get": (C.c_int32, [_vp, _P(EngineInfo)]),
@@ -396,6 +404,31 @@ class Engine:
         t = Timing()
         check(lib().ptx_engine_timing_get(self.h, C.byref(t)))
         return t
+
+
+def autotune(problems, ladders, seed=0, steps=32, **cfg_kw):
+    """Times every launch shape that fits the batch; returns (Engine keyword
+    arguments of the fastest shape, list of trial dicts)."""
+    if isinstance(problems, Problem):
+        problems = [problems]
+        ladders = [ladders]
+    base = default_config()
+    for k, v in cfg_kw.items():
+        setattr(base, k, v)
+    arr = (C.c_void_p * len(problems))(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h for p in problems])
+    betas = np.ascontiguousarray(np.concatenate([np.asarray(b, np.float64) for b in ladders]), np.float64)
+    nts = np.array([len(b) for b in ladders], np.int32)
+    best = EngineConfig()
+    tr = (TuneTrial * 64)()
+    n = C.c_int32()
+    check(lib().ptx_engine_autotune(C.cast(arr, C.POINTER(C.c_void_p)), len(problems), _ptr(betas, C.c_double),
+                                    _ptr(nts, C.c_int32), seed, C.byref(base), int(steps), C.byref(best), tr, 64,
+                                    C.byref(n)))
+    trials = [{"mode": t.mode, "waves_per_group": t.waves_per_group, "spins_per_lane": t.spins_per_lane,
+               "csr_placement": t.csr_placement, "us_per_step": t.us_per_step} for t in tr[: n.value]]
+    kw = {"mode": best.mode, "waves_per_group": best.waves_per_group, "spins_per_lane": best.spins_per_lane,
+          "csr_placement": best.csr_placement}
+    return kw, trials
 
 
 def device_fp64(x):

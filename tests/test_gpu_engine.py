@@ -1,0 +1,81 @@
+"""Engine features that must not change results: hipGraph replay of the
+fused kernel and the launch-shape autotuner (the random streams are indexed
+by position, replica and step, so any launch shape gives the same bits)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+import instances  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _batch(ptx, k=12, seed=0):
+    _, coos = instances.ea_batch(k, seed, sizes=(4, 6))
+    probs = [ptx.Problem.from_coo(*c) for c in coos]
+    lad = ptx.beta_linspace(0.2, 3.0, 12, True)
+    return probs, [lad] * len(probs)
+
+
+def _state(eng, k):
+    return [(eng.spins(i), eng.energies(i), eng.slot2rep(i), eng.flips(i), eng.best(i)[:2]) for i in range(k)]
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        for u, v in zip(x[:4], y[:4]):
+            assert np.array_equal(u, v)
+        assert np.array_equal(x[4][0], y[4][0]) and np.array_equal(x[4][1], y[4][1])
+
+
+def test_graph_replay_matches_eager_launches(ptx):
+    probs, lads = _batch(ptx)
+    eager = ptx.Engine(probs, lads, seed=5, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED, steps_per_launch=2)
+    graph = ptx.Engine(probs, lads, seed=5, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED, steps_per_launch=2,
+                       use_graph=3)
+    graph.timing(True)
+    done = 0
+    for chunk in (1, 13, 6, 7, 12):  # graph replays of 6 steps, eager remainders, host reads between them
+        eager.step(chunk)
+        graph.step(chunk)
+        done += chunk
+        say("steps", done)
+        _same(_state(eager, len(probs)), _state(graph, len(probs)))
+    t = graph.timing()
+    # launches: 1 eager | 2 replays of 3 + 1 eager | 1 replay | 1 replay + 1 eager | 2 replays
+    assert t.fused_launches == 1 + 7 + 3 + 4 + 6
+
+
+def test_autotuned_shape_gives_identical_results(ptx):
+    probs, lads = _batch(ptx, k=8, seed=1)
+    kw, trials = ptx.autotune(probs, lads, seed=3, steps=8, flags=ptx.FLAG_ICM)
+    say("autotune", kw, [(t["mode"], t["waves_per_group"], t["csr_placement"], round(t["us_per_step"], 1))
+                         for t in trials])
+    assert len(trials) >= 3 and kw["mode"] in (1, 2)
+    assert min(t["us_per_step"] for t in trials) > 0
+    ref = ptx.Engine(probs, lads, seed=3, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT, waves_per_group=1)
+    tuned = ptx.Engine(probs, lads, seed=3, flags=ptx.FLAG_ICM, **kw)
+    ref.step(10)
+    tuned.step(10)
+    _same(_state(ref, len(probs)), _state(tuned, len(probs)))
+
+
+def test_autotune_single_instance_includes_cooperative(ptx):
+    p = ptx.Problem.from_coo(*instances.torus_2d(16, seed=2))
+    lad = ptx.beta_linspace(0.5, 3.0, 10, False)
+    kw, trials = ptx.autotune(p, lad, seed=1, steps=16, flags=ptx.FLAG_ICM)
+    assert any(t["mode"] == 3 for t in trials)
+    a = ptx.Engine(p, lad, seed=1, flags=ptx.FLAG_ICM, **kw)
+    b = ptx.Engine(p, lad, seed=1, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT)
+    a.step(9)
+    b.step(9)
+    _same(_state(a, 1), _state(b, 1))
