@@ -264,7 +264,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                 const int r1 = s2r[t], r2 = s2r[t + nt];
                 if (wave == 0) {
                     long long dE1 = 0, dE2 = 0;
-                    icm_wave(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                    icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
+                             d.has_field != 0, n, nw, t, step, a.seed, c3i,
                              c.pub_spins + ((int64_t)par * nrep + r1) * nw,
                              c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, false);
                     if (lane == 0) st.E += (int32_t)(r == r1 ? dE1 : dE2);

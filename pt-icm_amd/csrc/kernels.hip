@@ -184,7 +184,8 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
     const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
     long long dE1 = 0, dE2 = 0;
-    icm_wave(csr, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed,
+    icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, a.hfield + d.pos_off,
+             d.has_field != 0, d.n, d.nw, t, step, a.seed,
              ctr3(kTagIcm | a.tag_base, d.inst_id), g1, g2, sc, dE1, dE2, lane);
     if (lane == 0) {
         a.E[ro + r1] += (int32_t)dE1;
@@ -368,7 +369,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
                 const int t = nt - 1 - j;
                 const int r1 = s2r[t], r2 = s2r[t + nt];
                 long long dE1 = 0, dE2 = 0;
-                icm_wave(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf, d.has_field != 0, n,
+                         nw, t, step, a.seed, c3i,
                          a.spins + d.spin_off + (int64_t)r1 * nw, a.spins + d.spin_off + (int64_t)r2 * nw, sc, dE1,
                          dE2, lane);
                 if (lane == 0) {

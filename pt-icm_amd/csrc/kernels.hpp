@@ -600,7 +600,8 @@ struct IcmScratch {
 };
 
 template <int Fmt>
-__device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf, bool has_field, int n, int nw,
+__device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const int32_t* hf, bool has_field, int n,
+                                         int nw,
                                          int t, uint32_t step, uint64_t seed, uint32_t c3, const uint32_t* g1_in,
                                          const uint32_t* g2_in, IcmScratch sc, long long& dE1_out, long long& dE2_out,
                                          int lane, bool write_back = true) {
@@ -663,16 +664,22 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
                 const int owner = __ffsll((long long)ball) - 1;
                 int sel = -1;
                 if (lane == owner) {
-                    int kk = k - (base + incl - c);
-                    for (int b = 0; b < 32; ++b) {
-                        if ((dw >> b) & 1u) {
-                            if (kk == 0) {
-                                sel = 32 * w + b;
-                                break;
-                            }
-                            --kk;
+                    // the kk-th set bit of dw by halving: 5 popcounts, no loop over bits
+                    int kk = k - (base + incl - c), at = 0;
+                    uint32_t x = dw;
+#pragma unroll
+                    for (int sh = 16; sh >= 1; sh >>= 1) {
+                        const uint32_t lo = x & ((1u << sh) - 1u);
+                        const int cl = __popc(lo);
+                        if (kk >= cl) {
+                            kk -= cl;
+                            x >>= sh;
+                            at += sh;
+                        } else {
+                            x = lo;
                         }
                     }
+                    sel = 32 * w + at;
                 }
                 seed_pos = __shfl(sel, owner, kWave);
             }
@@ -698,40 +705,51 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, const int32_t* hf,
             const int dg = act ? csr.deg(p) : 0;
             const uint8_t* row = csr.row(p);
             const int nch = (dg + K - 1) / K;
-            const int maxch = wave_max_i32(nch);
             for (int ch = 0; ch < maxch; ++ch) {
                 const int live = ch < nch ? min(K, dg - ch * K) : 0;
-                int col[K];
-                uint32_t cand = 0;
-                if (live > 0) {
-                    int wgt[K];
-                    csr.chunk(row, ch, col, wgt);
+                // Every load and claim below is issued unconditionally (the
+                // chunk index is clamped to the row, padding records name the
+                // in-range sentinel word) so that they overlap; liveness only
+                // masks the results.
+                int col[K], wgt[K];
+                csr.chunk(row, min(ch, max(nch - 1, 0)), col, wgt);
+                uint32_t dwd[K], vwd[K];
 #pragma unroll
-                    for (int h = 0; h < K; ++h) {
-                        const int j = col[h];
-                        const uint32_t bit = 1u << (j & 31);
-                        const bool c = h < live && (D[j >> 5] & bit) && !(V[j >> 5] & bit);
-                        cand |= (c ? 1u : 0u) << h;
-                    }
+                for (int h = 0; h < K; ++h) {
+                    dwd[h] = D[col[h] >> 5];
+                    vwd[h] = V[col[h] >> 5];
+                }
+                uint32_t cand = 0;
+#pragma unroll
+                for (int h = 0; h < K; ++h) {
+                    const uint32_t bit = 1u << (col[h] & 31);
+                    const bool c = (h < live) & ((dwd[h] & bit) != 0u) & ((vwd[h] & bit) == 0u);
+                    cand |= (c ? 1u : 0u) << h;
+                }
+                // only candidates claim (padding records share one sentinel word,
+                // and same-address atomics serialise); the claims carry no
+                // dependence on each other, so they still issue back to back
+                uint32_t old[K];
+#pragma unroll
+                for (int h = 0; h < K; ++h) {
+                    old[h] = 0xFFFFFFFFu;
+                    if ((cand >> h) & 1u) old[h] = atomicOr(&V[col[h] >> 5], 1u << (col[h] & 31));
                 }
                 uint32_t claimed = 0;
 #pragma unroll
-                for (int h = 0; h < K; ++h) {
-                    if ((cand >> h) & 1u) {
-                        const int j = col[h];
-                        const uint32_t bit = 1u << (j & 31);
-                        if (!(atomicOr(&V[j >> 5], bit) & bit)) claimed |= 1u << h;
-                    }
-                }
-                const int cnt_new = __popc(claimed);
-                int incl = cnt_new;
+                for (int h = 0; h < K; ++h)
+                    if (((cand >> h) & 1u) && !(old[h] & (1u << (col[h] & 31)))) claimed |= 1u << h;
+                // claims of the lanes below this one (and of the whole wave)
+                // from one ballot per record slot: no cross-lane data moves
+                int excl = 0, total = 0;
 #pragma unroll
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const int y = __shfl_up(incl, o, kWave);
-                    if (lane >= o) incl += y;
+                for (int h = 0; h < K; ++h) {
+                    const unsigned long long b = __ballot((claimed >> h) & 1u);
+                    excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+                    total += __popcll(b);
                 }
-                const int total = __shfl(incl, kWave - 1, kWave);
-                int pos = tail + incl - cnt_new;
+                int pos = tail + excl;
 #pragma unroll
                 for (int h = 0; h < K; ++h)
                     if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
