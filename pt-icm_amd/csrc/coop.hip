@@ -30,6 +30,8 @@ struct CoopArgs {
 // sweep.  Every position draws word p&3 of the Philox block p>>2, so the
 // result is the same for any SPL and workgroup size.  Class sums go through
 // three rotating LDS accumulators (one barrier per class).
+constexpr int kMaxCoopWaves = 8;  // waves per cooperative workgroup (4 or 8)
+
 template <int Fmt, int SPL>
 __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
                                                  const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
@@ -38,14 +40,11 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
                                                  int gl, int nthr, int* acc, uint32_t& cc) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int lane = gl % kWave;
+    const int nwv = nthr / kWave, wv = gl / kWave;
     for (int c = 0; c < ncls; ++c, ++cc) {
-        // cc counts classes across sweeps and steps: class cc sums into
-        // acc[cc % 3] and clears acc[(cc + 1) % 3], which was last read before
-        // the previous class's barrier and is next written after this one's.
-        if (gl == 0) {
-            acc[2 * ((cc + 1) % 3)] = 0;
-            acc[2 * ((cc + 1) % 3) + 1] = 0;
-        }
+        // cc counts classes across sweeps and steps: each wave stores its
+        // (dE, flips) sums in slot wv of buffer cc % 2; a buffer is read right
+        // after its class's barrier and rewritten only after the next one.
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
         if (SPL == 1) {
@@ -69,33 +68,44 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
         }
         dsum = wave_sum_i32(dsum);
         nacc = wave_sum_i32(nacc);
-        int* a = acc + 2 * (cc % 3);
-        if (lane == 0 && (dsum != 0 || nacc != 0)) {
-            atomicAdd(&a[0], dsum);
-            atomicAdd(&a[1], nacc);
+        int* a = acc + 2 * kMaxCoopWaves * (int)(cc & 1u);
+        if (lane == 0) {
+            a[2 * wv] = dsum;
+            a[2 * wv + 1] = nacc;
         }
         __syncthreads();
-        const int ds = a[0], na = a[1];
+        int ds = 0, na = 0;
+        for (int q = 0; q < nwv; ++q) {
+            ds += a[2 * q];
+            na += a[2 * q + 1];
+        }
         st.E += ds;
         st.flips += (unsigned long long)na;
-        if (na > 0 && st.E < st.bestE) {
+        if (na > 0 && st.E < st.bestE) {  // uniform: every thread holds the same sums
             st.bestE = st.E;
             st.best_sweep = (int32_t)gsweep;
             pack_bits(sp, nw, best_words, gl, nthr);
+            __syncthreads();  // the snapshot is complete before the next class flips spins
         }
     }
 }
 
-// Grid barrier over all workgroups of the launch.  Returns false when the
-// wait exceeded its bound (the error flag is then set and every workgroup
-// leaves the step loop).
-__device__ __forceinline__ bool grid_barrier(unsigned long long* bar, uint32_t* err, unsigned long long target,
-                                             int* lds_flag) {
+// Grid barrier over all workgroups of the launch, in two halves so that
+// waves other than the first can do independent work while thread 0 waits.
+// grid_wait returns false when the wait exceeded its bound (the error flag is
+// then set and every workgroup leaves the step loop).
+__device__ __forceinline__ void grid_arrive(unsigned long long* bar) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        int ok = 1;
         __threadfence();
         atomicAdd(bar, 1ull);
+    }
+}
+
+__device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err, unsigned long long target,
+                                          int* lds_flag) {
+    if (threadIdx.x == 0) {
+        int ok = 1;
         unsigned long long spins = 0;
         while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(2);
@@ -113,7 +123,7 @@ __device__ __forceinline__ bool grid_barrier(unsigned long long* bar, uint32_t* 
 }
 
 struct CoopLayout {
-    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, sp, icm, acc, flag, total;
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, acc, flag, total;
 };
 
 __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax) {
@@ -138,10 +148,11 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     L.perm = take(4 * nrep);
     L.tpos = take(4 * nrep);
     L.level = take(4 * nrep);
+    L.pre = take(2 * kWave * (int)sizeof(ExchPre));  // prepared exchange of the next step, two buffers
     const int nw = (n + 31) / 32;
     L.sp = take(spin_bytes(nw));
     L.icm = take(icm_scratch_bytes(n));
-    L.acc = take(4 * 6);
+    L.acc = take(4 * 2 * 2 * kMaxCoopWaves);
     L.flag = take(4);
     L.total = o;
     return L;
@@ -197,8 +208,21 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         att[s] = 0;
         succ[s] = 0;
     }
-    if (tid < 6) acc[tid] = 0;
+    if (tid < 4 * kMaxCoopWaves) acc[tid] = 0;
     expand_bits(a.spins + d.spin_off + (int64_t)r * nw, nw, sp, tid, nthr);
+    // register-resident exchange (up to 64 replicas): the first step's
+    // preparation here, later ones overlapped with the previous exchange
+    // (waves 1 and 2 prepare one half each and store their fields of ExchPre)
+    const bool prepared = nrep <= kWave && !a.no_exchange && nt > 1;
+    ExchPre* pre = reinterpret_cast<ExchPre*>(smem + L.pre);
+    auto prepare_half = [&](uint32_t s, int h) {
+        const ExchPre x = exch_prepare(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, s, lane, h);
+        ExchPre& slot = pre[(s & 1u) * kWave + lane];
+        slot.tpos[h] = x.tpos[h];
+        slot.level[h] = x.level[h];
+        slot.lu[h] = x.lu[h];
+    };
+    if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[c.inst], wave - 1);
     __syncthreads();
     const Csr<Fmt> csr{info, smem + L.adj};
     const int32_t* hf = a.hfield + d.pos_off;
@@ -239,18 +263,27 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         // ---- publish and wait ----
         pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
         if (tid == 0) c.pub_E[par * nrep + r] = st.E;
-        if (!grid_barrier(c.bar, c.err, (c.bar_base + (unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
-                          flag))
+        grid_arrive(c.bar);
+        // waves 1 and 2 prepare the next step's exchange while thread 0 waits
+        if (prepared && (wave == 1 || wave == 2) && it + 1 < nsteps) prepare_half(step + 1u, wave - 1);
+        if (!grid_wait(c.bar, c.err, (c.bar_base + (unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
+                       flag))
             break;
         for (int s = tid; s < nrep; s += nthr) E[s] = c.pub_E[par * nrep + s];
         __syncthreads();
         phase_done(3);
-        // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0) ----
+        // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
+        //      wave 1 meanwhile prepares the next step's exchange ----
         if (wave == 0) {
             if (r == 0) record_histogram(a, d, step, E, s2r, lane);
             if (!a.no_exchange) {
-                nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta, E,
-                                           s2r, att, succ, xs, lane);
+                if (prepared) {
+                    const ExchPre x = pre[(step & 1u) * kWave + lane];
+                    nsw_total += exch_apply(nt, nrep, a.pilot != 0, x, beta, E, s2r, att, succ, lane);
+                } else {
+                    nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta,
+                                               E, s2r, att, succ, xs, lane);
+                }
                 wave_sync();
                 for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
             }

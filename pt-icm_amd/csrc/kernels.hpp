@@ -68,20 +68,33 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Wave-wide sum and maximum with DPP row shifts and row broadcasts (a
+// prefix scan whose last lane holds the total; VALU only, no LDS crossbar).
+// Call with all 64 lanes active.
 __device__ __forceinline__ int wave_sum_i32(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xe, false);  // row_shr:4, banks 1-3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xc, false);  // row_shr:8, banks 2-3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
 }
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
 }
+// (the identity INT_MIN fills lanes without a source)
 __device__ __forceinline__ int wave_max_i32(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-    return v;
+    const int lo = (int)0x80000000;
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x114, 0xf, 0xe, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x118, 0xf, 0xc, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(lo, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -404,10 +417,21 @@ struct ExchScratch {
 // slot k (nrep <= 64): sort ranks, dependency levels and the level-by-level
 // decisions move between neighbouring slots with shuffles instead of LDS
 // round trips and wave barriers.  Bit-identical to the LDS form.
-__device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep, bool pilot, uint64_t seed,
-                                                                uint32_t tag_base, uint32_t inst_id, uint32_t step,
-                                                                const double* beta, const int32_t* E, int32_t* s2r,
-                                                                uint32_t* att, uint32_t* succ, int lane) {
+//
+// Everything but the decisions depends only on (seed, step): exch_prepare
+// computes, per half and lane, the attempt index (tpos), its dependency level
+// and its log-uniform; exch_apply makes the decisions from them and the
+// energies.  A kernel can prepare step s+1 on one wave while another applies
+// step s.
+struct ExchPre {
+    int32_t tpos[2], level[2];
+    double lu[2];
+};
+
+// only_half >= 0 prepares that half alone (the other keeps tpos -1): two
+// waves can prepare the two halves at once.
+__device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, uint64_t seed, uint32_t tag_base,
+                                                uint32_t inst_id, uint32_t step, int lane, int only_half = -1) {
     const int np = nt - 1;
     const int nhalf = pilot ? 1 : 2;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -417,13 +441,14 @@ __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep
     const int kc = valid ? lane % nt : 0;
     const int my_chain = valid ? lane / nt : 0;
     const bool is_pair = valid && kc < np;  // slot k has a right neighbour in its chain
-    int rep = valid ? s2r[lane] : 0;
-    int32_t es = valid ? E[rep] : 0;
-    const double bk = valid ? beta[lane] : 0.0;
-    const double bk1 = __shfl_down(bk, 1, kWave);  // beta of slot k + 1 (used only when is_pair)
-    uint32_t natt = 0, nsucc = 0;
-    unsigned long long nsw = 0;
+    ExchPre x;
+    for (int h = 0; h < 2; ++h) {
+        x.tpos[h] = -1;
+        x.level[h] = 0;
+        x.lu[h] = 0.0;
+    }
     for (int h = 0; h < nhalf; ++h) {
+        if (only_half >= 0 && h != only_half) continue;
         // rank of pair kc in this half's order: pairs sorted by (key, index)
         const uint32_t key =
             is_pair ? pick(philox4x32_10((uint32_t)kc >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)kc) : 0u;
@@ -435,7 +460,8 @@ __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep
         const int a = h * np + rank;
         const int chain = pilot ? 0 : (a & 1);
         const int tpos = (is_pair && chain == my_chain) ? a : -1;
-        const double lu =
+        x.tpos[h] = tpos;
+        x.lu[h] =
             tpos >= 0 ? uniform_log(pick(philox4x32_10((uint32_t)a >> 2, 0u, step, c3e, k0, k1), (uint32_t)a)) : 0.0;
         // dependency levels: 1 + the highest level of an earlier attempt on k-1 or k+1
         int level = 0;
@@ -452,6 +478,25 @@ __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep
             level = lv;
             if (!changed) break;
         }
+        x.level[h] = level;
+    }
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool pilot, const ExchPre& x,
+                                                         const double* beta, const int32_t* E, int32_t* s2r,
+                                                         uint32_t* att, uint32_t* succ, int lane) {
+    const int nhalf = pilot ? 1 : 2;
+    const bool valid = lane < nrep;
+    const int kc = valid ? lane % nt : 0;
+    int rep = valid ? s2r[lane] : 0;
+    int32_t es = valid ? E[rep] : 0;
+    const double bk = valid ? beta[lane] : 0.0;
+    const double bk1 = __shfl_down(bk, 1, kWave);  // beta of slot k + 1 (used only at a decision)
+    uint32_t natt = 0, nsucc = 0;
+    unsigned long long nsw = 0;
+    for (int h = 0; h < nhalf; ++h) {
+        const int level = x.level[h];
         const int maxlv = wave_max_i32(level);
         for (int L = 1; L <= maxlv; ++L) {
             const int rep_r = __shfl_down(rep, 1, kWave);
@@ -460,7 +505,7 @@ __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep
             if (level == L) {
                 const double lb = (bk - bk1) * (double)(es - es_r);
                 ++natt;
-                if (lu < lb) {
+                if (x.lu[h] < lb) {
                     sw = 1;
                     ++nsucc;
                     ++nsw;
@@ -484,6 +529,14 @@ __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep
         succ[lane] += nsucc;
     }
     return nsw;
+}
+
+__device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep, bool pilot, uint64_t seed,
+                                                                uint32_t tag_base, uint32_t inst_id, uint32_t step,
+                                                                const double* beta, const int32_t* E, int32_t* s2r,
+                                                                uint32_t* att, uint32_t* succ, int lane) {
+    const ExchPre x = exch_prepare(nt, nrep, pilot, seed, tag_base, inst_id, step, lane);
+    return exch_apply(nt, nrep, pilot, x, beta, E, s2r, att, succ, lane);
 }
 
 __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bool pilot, uint64_t seed,
