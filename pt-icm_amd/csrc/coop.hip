@@ -28,8 +28,8 @@ struct CoopArgs {
 // nthr takes SPL positions per step of the class loop: SPL = 1 -> position
 // cs + gl + k*nthr; SPL = 4 -> the aligned groups of 4 used by the wave-level
 // sweep.  Every position draws word p&3 of the Philox block p>>2, so the
-// result is the same for any SPL and workgroup size.  Class sums go through
-// three rotating LDS accumulators (one barrier per class).
+// result is the same for any SPL and workgroup size.  Each wave's class sums
+// go to its slot in one of two alternating LDS buffers (one barrier per class).
 constexpr int kMaxCoopWaves = 8;  // waves per cooperative workgroup (4 or 8)
 
 template <int Fmt, int SPL>
