@@ -203,12 +203,37 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return (int64_t)desc[x].n * desc[x].max_deg > (int64_t)desc[y].n * desc[y].max_deg;
     });
+    // MI355X hands consecutive workgroups of a launch to its 8 XCDs in turn,
+    // each with its own L2.  With many instances, instance k of the size order
+    // goes to XCD k % 8 (round robin keeps the XCDs' work even) and all of its
+    // sweep and cluster-move items run there, so its adjacency is cached in
+    // one L2 instead of eight: launch slot b holds item b / 8 of XCD b % 8's
+    // list, short lists padded with idle items (instance -1).
     items.clear();
-    for (int32_t i : order)
-        for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
     pairs.clear();
-    for (int32_t i = 0; i < ninst; ++i)
-        for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
+    if (ninst >= 8) {
+        std::vector<std::vector<int2>> ix(8), px(8);
+        for (size_t k = 0; k < order.size(); ++k) {
+            const int32_t i = order[k];
+            const int x = (int)(k % 8);
+            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) ix[x].push_back(make_int2(i, r0));
+            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) px[x].push_back(make_int2(i, t));
+        }
+        auto arrange = [](const std::vector<std::vector<int2>>& per, std::vector<int2>& out) {
+            size_t len = 0;
+            for (const auto& v : per) len = std::max(len, v.size());
+            out.assign(len * 8, make_int2(-1, 0));
+            for (int x = 0; x < 8; ++x)
+                for (size_t j = 0; j < per[x].size(); ++j) out[j * 8 + x] = per[x][j];
+        };
+        arrange(ix, items);
+        arrange(px, pairs);
+    } else {
+        for (int32_t i : order)
+            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
+        for (int32_t i = 0; i < ninst; ++i)
+            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
+    }
 
     // ---- execution mode ----
     mode = PTX_MODE_SPLIT;

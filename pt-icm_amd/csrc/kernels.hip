@@ -23,6 +23,7 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int2 it = a.items[blockIdx.x];
+    if (it.x < 0) return;  // idle item padding an XCD's list
     const InstDesc d = a.desc[it.x];
     const int r = it.y + wave;
     if (r >= d.nrep) return;
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int2 it = a.items[blockIdx.x];
+    if (it.x < 0) return;  // idle item padding an XCD's list
     const InstDesc d = a.desc[it.x];
     const int r = it.y + wave;
     if (r >= d.nrep) return;
@@ -169,6 +171,7 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int2 pr = a.pairs[blockIdx.x];
+    if (pr.x < 0) return;  // idle item padding an XCD's list
     const InstDesc d = a.desc[pr.x];
     const int t = pr.y;
     IcmScratch sc;
