@@ -176,6 +176,14 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
                                int32_t* n_trials);
 /* Enqueues n_steps PT steps (asynchronous; errors of the launch are reported). */
 ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps);
+/* Retires (active[i] == 0) or reactivates instances: later steps run only the
+ * active ones, with their work lists rebuilt, and retired instances keep
+ * their state.  A batch whose instances finish at different times (for
+ * example on reaching a target energy) stops paying for the finished ones.
+ * No counterpart in the reference (one instance per run).  In split mode the
+ * step index of the random streams is the engine's, so an instance
+ * reactivated after a pause draws from the steps it rejoins at. */
+ptx_status ptx_engine_set_active(ptx_engine* e, const uint8_t* active);
 ptx_status ptx_engine_sync(ptx_engine* e);
 
 typedef struct {

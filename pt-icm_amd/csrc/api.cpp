@@ -320,6 +320,11 @@ ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps) {
     return guarded([&]() { return done(e->e.step(n_steps)); });
 }
 
+ptx_status ptx_engine_set_active(ptx_engine* e, const uint8_t* active) {
+    if (!e || !active) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(e->e.set_active(active)); });
+}
+
 ptx_status ptx_engine_sync(ptx_engine* e) {
     if (!e) return fail(PTX_ERR_INVALID, "null engine");
     return done(e->e.sync());

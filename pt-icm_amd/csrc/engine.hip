@@ -203,37 +203,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return (int64_t)desc[x].n * desc[x].max_deg > (int64_t)desc[y].n * desc[y].max_deg;
     });
-    // MI355X hands consecutive workgroups of a launch to its 8 XCDs in turn,
-    // each with its own L2.  With many instances, instance k of the size order
-    // goes to XCD k % 8 (round robin keeps the XCDs' work even) and all of its
-    // sweep and cluster-move items run there, so its adjacency is cached in
-    // one L2 instead of eight: launch slot b holds item b / 8 of XCD b % 8's
-    // list, short lists padded with idle items (instance -1).
-    items.clear();
-    pairs.clear();
-    if (ninst >= 8) {
-        std::vector<std::vector<int2>> ix(8), px(8);
-        for (size_t k = 0; k < order.size(); ++k) {
-            const int32_t i = order[k];
-            const int x = (int)(k % 8);
-            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) ix[x].push_back(make_int2(i, r0));
-            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) px[x].push_back(make_int2(i, t));
-        }
-        auto arrange = [](const std::vector<std::vector<int2>>& per, std::vector<int2>& out) {
-            size_t len = 0;
-            for (const auto& v : per) len = std::max(len, v.size());
-            out.assign(len * 8, make_int2(-1, 0));
-            for (int x = 0; x < 8; ++x)
-                for (size_t j = 0; j < per[x].size(); ++j) out[j * 8 + x] = per[x][j];
-        };
-        arrange(ix, items);
-        arrange(px, pairs);
-    } else {
-        for (int32_t i : order)
-            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
-        for (int32_t i = 0; i < ninst; ++i)
-            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
-    }
+    size_order = order;
+    active.assign((size_t)ninst, 1);
 
     // ---- execution mode ----
     mode = PTX_MODE_SPLIT;
@@ -257,7 +228,10 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     } else if (cfg.mode == PTX_MODE_COOP) {
         return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode runs a single instance");
     }
-    if (mode == PTX_MODE_SPLIT && (cfg.mode == PTX_MODE_FUSED || cfg.mode == PTX_MODE_AUTO)) {
+    // The fused kernel runs when asked for (or chosen by the autotuner): on
+    // the batches measured here the split kernels, which spread every
+    // instance's replicas and cluster moves over the whole GPU, are faster.
+    if (mode == PTX_MODE_SPLIT && cfg.mode == PTX_MODE_FUSED) {
         int Wpref = cfg.waves_per_group > 0 ? cfg.waves_per_group : (nrep_max >= 32 ? 16 : 8);
         if (Wpref != 4 && Wpref != 8 && Wpref != 16) Wpref = Wpref < 6 ? 4 : (Wpref < 12 ? 8 : 16);
         // LDS bytes of the largest instance for W waves, adjacency in LDS or not
@@ -299,12 +273,10 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                                                         " waves); use split mode");
         K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 16;
     }
-    std::vector<int32_t> fused_order(order.begin(), order.end());
-
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
-    if (mode == PTX_MODE_FUSED) PTX_TRY(d_order.upload(fused_order, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
+    PTX_TRY(d_step.alloc(1));
     if (mode == PTX_MODE_COOP) {
         PTX_TRY(d_pub_spins.alloc((size_t)2 * desc[0].nrep * desc[0].nw));
         PTX_TRY(d_pub_E.alloc((size_t)2 * desc[0].nrep));
@@ -318,8 +290,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_p2o.upload(p2o, stream));
     PTX_TRY(d_thr.upload(thr, stream));
     PTX_TRY(d_beta.upload(beta, stream));
-    PTX_TRY(d_items.upload(items, stream));
-    PTX_TRY(d_pairs.upload(pairs, stream));
+    PTX_TRY(build_lists());
     PTX_TRY(d_spins.alloc((size_t)spin_off));
     PTX_TRY(d_best_spins.alloc((size_t)spin_off));
     PTX_TRY(d_E.alloc((size_t)total_rep));
@@ -365,6 +336,7 @@ static DevArgs build_args(Engine& e) {
     a.order = e.d_order.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.inst_step = e.d_inst_step.p;
+    a.split_step = e.d_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
     a.sweeps = e.cfg.sweeps;
@@ -400,7 +372,7 @@ Status Engine::launch_init(int mode) {
     return hip_check(hipGetLastError(), "k_init launch");
 }
 
-Status Engine::launch_step(uint32_t s) {
+Status Engine::launch_step() {
     DevArgs a = build_args(*this);
     auto timed = [&](int kind, auto&& fn) -> Status {
         if (!timing) {
@@ -422,7 +394,7 @@ Status Engine::launch_step(uint32_t s) {
         const size_t lds = (size_t)W * (spin_stride + 4 * (size_t)thr_stride);
         const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
         PTX_TRY(timed(0, [&] {
-            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, s); };
+            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a); };
             if (fmt == kFmtUnit) {
                 switch (W) {
                     case 1: go(k_sweep<kFmtUnit, 1>); break;
@@ -443,24 +415,25 @@ Status Engine::launch_step(uint32_t s) {
     {
         const size_t lds = (size_t)nrep_max * 48 + 64;
         PTX_TRY(timed(1, [&] {
-            hipLaunchKernelGGL(k_exchange, dim3((unsigned)ninst), dim3(kWave), lds, stream, a, s, nrep_max);
+            hipLaunchKernelGGL(k_exchange, dim3((unsigned)inst_list.size()), dim3(kWave), lds, stream, a, nrep_max);
         }));
     }
     if (!pairs.empty()) {
         const size_t lds = (size_t)(nw_max + 1) * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
         PTX_TRY(timed(2, [&] {
             if (fmt == kFmtUnit)
-                hipLaunchKernelGGL(k_icm<kFmtUnit>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
+                hipLaunchKernelGGL(k_icm<kFmtUnit>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, nw_max);
             else
-                hipLaunchKernelGGL(k_icm<kFmtWide>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, s, nw_max);
+                hipLaunchKernelGGL(k_icm<kFmtWide>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, nw_max);
         }));
     }
-    return Status::ok();
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream, d_step.p);
+    return hip_check(hipGetLastError(), "k_step_advance");
 }
 
 Status Engine::launch_fused(int nsteps) {
     DevArgs a = build_args(*this);
-    const dim3 grid((unsigned)ninst), block((unsigned)(Wf * kWave));
+    const dim3 grid((unsigned)inst_list.size()), block((unsigned)(Wf * kWave));
     const size_t lds = (size_t)fused_lds;
     auto go = [&](auto kern) -> Status {
         if (lds > 64 * 1024)
@@ -556,6 +529,7 @@ Status Engine::launch_coop(int nsteps) {
 
 Status Engine::step(int64_t n) {
     if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
+    if (inst_list.empty()) return Status::ok();  // every instance retired
     if (steps_done + n > 0xFFFFFFFFll) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^32 PT steps");
     if (mode == PTX_MODE_COOP) {
         while (n > 0) {
@@ -595,21 +569,109 @@ Status Engine::step(int64_t n) {
         }
         return Status::ok();
     }
-    for (int64_t i = 0; i < n; ++i) {
-        PTX_TRY(launch_step((uint32_t)steps_done));
+    // split mode: with use_graph = G (and timing off, which needs per-kernel
+    // events) G steps of launches are captured once and replayed
+    const int64_t G = (cfg.use_graph > 0 && !timing) ? cfg.use_graph : 0;
+    while (n > 0) {
+        if (G > 0 && n >= G) {
+            if (graph_exec == nullptr) PTX_TRY(build_graph());
+            PTX_TRY(hip_check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch"));
+            steps_done += G;
+            n -= G;
+            continue;
+        }
+        PTX_TRY(launch_step());
         ++steps_done;
+        --n;
     }
     return Status::ok();
 }
 
-// Captures use_graph launches of K steps on the engine stream.  Nothing in a
-// fused launch is a memset or a host copy, so the graph holds kernel nodes only.
+// Uploads a work list, growing its buffer only when it is too small (lists
+// shrink as instances retire).
+template <class T>
+static Status put_list(DevBuf<T>& buf, const std::vector<T>& v, hipStream_t s) {
+    const size_t need = std::max<size_t>(v.size(), 1);
+    if (buf.p == nullptr || buf.n < need) PTX_TRY(buf.alloc(need));
+    if (v.empty()) return Status::ok();
+    return buf.upload(v.data(), v.size(), 0, s);
+}
+
+// Work lists of the active instances.  MI355X hands consecutive workgroups
+// of a launch to its 8 XCDs in turn, each with its own L2.  With many
+// instances, instance k of the size order goes to XCD k % 8 (round robin
+// keeps the XCDs' work even) and all of its sweep and cluster-move items run
+// there, so its adjacency is cached in one L2 instead of eight: launch slot b
+// holds item b / 8 of XCD b % 8's list, short lists padded with idle items
+// (instance -1).  inst_list (one workgroup per instance: exchange, fused
+// kernel) lists the active instances in size order.
+Status Engine::build_lists() {
+    items.clear();
+    pairs.clear();
+    inst_list.clear();
+    std::vector<int32_t> act;
+    for (int32_t i : size_order)
+        if (active[(size_t)i]) act.push_back(i);
+    inst_list = act;
+    updates_per_step = 0;
+    lds_bytes_per_step = 0;
+    for (int32_t i : act) {
+        const Problem& P = *probs[(size_t)i];
+        updates_per_step += (int64_t)desc[i].nrep * P.n * cfg.sweeps;
+        lds_bytes_per_step += (int64_t)desc[i].nrep * cfg.sweeps * (4 * P.nnz() + 6ll * P.n);
+    }
+    if (act.size() >= 8) {
+        std::vector<std::vector<int2>> ix(8), px(8);
+        for (size_t k = 0; k < act.size(); ++k) {
+            const int32_t i = act[k];
+            const int x = (int)(k % 8);
+            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) ix[x].push_back(make_int2(i, r0));
+            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) px[x].push_back(make_int2(i, t));
+        }
+        auto arrange = [](const std::vector<std::vector<int2>>& per, std::vector<int2>& out) {
+            size_t len = 0;
+            for (const auto& v : per) len = std::max(len, v.size());
+            out.assign(len * 8, make_int2(-1, 0));
+            for (int x = 0; x < 8; ++x)
+                for (size_t j = 0; j < per[x].size(); ++j) out[j * 8 + x] = per[x][j];
+        };
+        arrange(ix, items);
+        arrange(px, pairs);
+    } else {
+        for (int32_t i : act)
+            for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
+        for (int32_t i : act)
+            for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
+    }
+    PTX_TRY(put_list(d_items, items, stream));
+    PTX_TRY(put_list(d_pairs, pairs, stream));
+    PTX_TRY(put_list(d_order, inst_list, stream));
+    return hip_check(hipStreamSynchronize(stream), "work lists");
+}
+
+Status Engine::set_active(const uint8_t* flags) {
+    PTX_TRY(sync());
+    if (graph_exec) {  // the graph holds the old grid sizes
+        (void)hipGraphExecDestroy(graph_exec);
+        (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+    for (int32_t i = 0; i < ninst; ++i) active[(size_t)i] = flags[i] ? 1 : 0;
+    return build_lists();
+}
+
+// Captures use_graph fused launches of K steps, or use_graph split-mode PT
+// steps (sweep, exchange, cluster moves, step advance), on the engine
+// stream.  No launch here is a memset or a host copy, so the graph holds
+// kernel nodes only; every kernel reads its step from device memory, so a
+// replay continues where the last one ended.
 Status Engine::build_graph() {
     const bool was_timing = timing;
     timing = false;
     PTX_TRY(hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"));
     Status st = Status::ok();
-    for (int32_t g = 0; g < cfg.use_graph && st; ++g) st = launch_fused(K);
+    for (int32_t g = 0; g < cfg.use_graph && st; ++g) st = mode == PTX_MODE_FUSED ? launch_fused(K) : launch_step();
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &gr);
     timing = was_timing;
@@ -695,6 +757,8 @@ Status Engine::read_spins(int32_t inst, uint8_t* out) {
 
 Status Engine::write_spins(int32_t inst, const uint8_t* in) {
     PTX_TRY(check_inst(*this, inst));
+    if (!active[(size_t)inst])  // its energies are recomputed by the active work list
+        return Status::err(PTX_ERR_INVALID, "instance is retired; reactivate it before writing its states");
     const InstDesc& d = desc[inst];
     const Problem& P = *probs[inst];
     std::vector<uint32_t> w((size_t)d.nrep * d.nw, 0u);

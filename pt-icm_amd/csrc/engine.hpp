@@ -63,6 +63,9 @@ struct Engine {
     int32_t spin_stride = 0, thr_stride = 1;
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;
     std::vector<int2> items, pairs;
+    std::vector<int32_t> inst_list;  // active instances in size order (one workgroup each)
+    std::vector<int32_t> size_order; // all instances, largest first
+    std::vector<uint8_t> active;
     hipStream_t stream = nullptr;
     bool timing = false;
     struct Timed {
@@ -85,6 +88,7 @@ struct Engine {
     DevBuf<int2> d_items, d_pairs;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
+    DevBuf<uint32_t> d_step;     // split mode: the PT step the next launches run
     DevBuf<unsigned long long> d_phase;
     // cooperative mode
     DevBuf<uint32_t> d_pub_spins, d_err;
@@ -118,10 +122,14 @@ struct Engine {
 
   private:
     Status launch_init(int mode);
-    Status launch_step(uint32_t s);
+    Status launch_step();
     Status launch_fused(int nsteps);
     Status launch_coop(int nsteps);
     Status build_graph();
+    Status build_lists();
+
+  public:
+    Status set_active(const uint8_t* flags);
     Status collect_timing();
 };
 

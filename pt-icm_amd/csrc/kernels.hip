@@ -70,7 +70,8 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
 // workgroup's instance.  LDS per wave: spin bytes + the replica's threshold row.
 // ---------------------------------------------------------------------------
 template <int Fmt, int W>
-__global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a, uint32_t step) {
+__global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
+    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int2 it = a.items[blockIdx.x];
@@ -123,10 +124,12 @@ __device__ __forceinline__ void record_histogram(const DevArgs& a, const InstDes
 // ---------------------------------------------------------------------------
 // k_exchange: one wavefront per instance (LDS: 48 bytes per slot).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, int nrep_max) {
+__global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, int nrep_max) {
+    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
-    const InstDesc d = a.desc[blockIdx.x];
+    const int inst = a.order[blockIdx.x];  // the active instances
+    const InstDesc d = a.desc[inst];
     const int nrep = d.nrep;
     double* beta = reinterpret_cast<double*>(smem);
     ExchScratch x;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, in
     const unsigned long long nsw = exchange_wave(d.nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta,
                                                  E, s2r, att, succ, x, lane);
     const unsigned long long total = (unsigned long long)wave_sum_i64((long long)nsw);
-    if (lane == 0) a.swaps[blockIdx.x] += total;
+    if (lane == 0) a.swaps[inst] += total;
     wave_sync();
     for (int s = lane; s < nrep; s += kWave) {
         a.slot2rep[ro + s] = s2r[s];
@@ -167,7 +170,8 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, uint32_t step, in
 // k_icm: one wavefront per (instance, slot t) pair, t in [nt-nc, nt).
 // ---------------------------------------------------------------------------
 template <int Fmt>
-__global__ __launch_bounds__(kWave) void k_icm(DevArgs a, uint32_t step, int nw_max) {
+__global__ __launch_bounds__(kWave) void k_icm(DevArgs a, int nw_max) {
+    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int2 pr = a.pairs[blockIdx.x];
@@ -403,6 +407,10 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     }
 }
 
+// Ends a split-mode PT step: the next launches read the next step index.
+// (A kernel rather than a host copy, so that graph replays advance it too.)
+__global__ void k_step_advance(uint32_t* step) { *step += 1u; }
+
 // ---------------------------------------------------------------------------
 // Self-tests.
 // ---------------------------------------------------------------------------
@@ -464,11 +472,11 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint
     template __global__ void k_init<F, 2>(DevArgs, int);          \
     template __global__ void k_init<F, 4>(DevArgs, int);          \
     template __global__ void k_init<F, 8>(DevArgs, int);          \
-    template __global__ void k_sweep<F, 1>(DevArgs, uint32_t);    \
-    template __global__ void k_sweep<F, 2>(DevArgs, uint32_t);    \
-    template __global__ void k_sweep<F, 4>(DevArgs, uint32_t);    \
-    template __global__ void k_sweep<F, 8>(DevArgs, uint32_t);    \
-    template __global__ void k_icm<F>(DevArgs, uint32_t, int);    \
+    template __global__ void k_sweep<F, 1>(DevArgs);              \
+    template __global__ void k_sweep<F, 2>(DevArgs);              \
+    template __global__ void k_sweep<F, 4>(DevArgs);              \
+    template __global__ void k_sweep<F, 8>(DevArgs);              \
+    template __global__ void k_icm<F>(DevArgs, int);              \
     template __global__ void k_fused<F, 4, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 8, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 16, 0>(DevArgs, int);     \

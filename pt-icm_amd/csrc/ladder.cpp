@@ -216,20 +216,25 @@ Dos solve_dos_newton(const std::vector<SlotHist>& hist, const double* beta_h, do
     for (int iter = 0; iter < 200; ++iter) {
         std::fill(grad.begin(), grad.end(), -dn);
         std::fill(Hm.begin(), Hm.end(), 0.0);
+        std::vector<int> live;
+        live.reserve(nh);
         for (size_t i = 0; i < E.size(); ++i) {
             double mx = -DBL_MAX;
             for (int j = 0; j < nh; ++j) mx = std::max(mx, t[j] = g[j] - beta_h[j] * E[i]);
             double s = 0;
             for (int j = 0; j < nh; ++j) s += (p[j] = std::exp(t[j] - mx));
             const double inv = 1.0 / s;
-            for (int j = 0; j < nh; ++j) p[j] *= inv;
-            for (int k = 0; k < nh; ++k) {
-                if (p[k] < 1e-300) continue;
+            live.clear();
+            for (int j = 0; j < nh; ++j) {
+                p[j] *= inv;
+                if (p[j] > 1e-17) live.push_back(j);  // a bin weighs on a few temperatures only
+            }
+            for (int k = 0; k < nh; ++k) grad[k] += Nv[i] * p[k];
+            for (int k : live) {
                 const double a = Nv[i] * p[k];
-                grad[k] += a;
                 Hm[(size_t)k * nh + k] += a;
                 double* row = &Hm[(size_t)k * nh];
-                for (int l = 0; l < nh; ++l) row[l] -= a * p[l];
+                for (int l : live) row[l] -= a * p[l];
             }
         }
         double gmax = 0;

@@ -100,6 +100,7 @@ _SIGS = {
                                         _P(C.c_int32)]),
     "ptx_engine_step": (C.c_int32, [_vp, C.c_int64]),
     "ptx_engine_sync": (C.c_int32, [_vp]),
+    "ptx_engine_set_active": (C.c_int32, [_vp, _P(C.c_uint8)]),
     "ptx_engine_info_get": (C.c_int32, [_vp, _P(EngineInfo)]),
     "ptx_engine_instance_info": (C.c_int32, [_vp, C.c_int32, _P(InstanceInfo)]),
     "ptx_engine_read_spins": (C.c_int32, [_vp, C.c_int32, _P(C.c_uint8)]),
@@ -323,6 +324,12 @@ class Engine:
 
     def sync(self):
         check(lib().ptx_engine_sync(self.h))
+
+    def set_active(self, active):
+        """Later steps run only the instances with a nonzero flag."""
+        a = np.ascontiguousarray(np.asarray(active, bool).astype(np.uint8))
+        assert len(a) == self.ninst
+        check(lib().ptx_engine_set_active(self.h, _ptr(a, C.c_uint8)))
 
     def info(self):
         i = EngineInfo()

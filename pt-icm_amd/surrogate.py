@@ -142,19 +142,27 @@ def ladder_from_increments(y, bs, be, max_slots=N_OUT):
     return np.array(lad)
 
 
-def multihistogram_ladders(ptx, problems, nt, T0, T1, burn_in, samples, seed, inverse=False, threads=16):
+def multihistogram_ladders(ptx, problems, nt, T0, T1, burn_in, samples, seed, inverse=False, threads=16,
+                           timing=None):
     """The reference's --preprocess ladder for every problem (GPU pilot,
-    FP64 DOS and ladder search on host threads)."""
+    FP64 DOS and ladder search on host threads).  timing (a dict) receives
+    the seconds of the GPU pilot and of the host solve."""
+    t0 = time.perf_counter()
     eng, hb, bs, be = pilot(ptx, problems, nt, T0, T1, burn_in, samples, seed, inverse)
     hists = [eng.histograms(i) for i in range(len(problems))]
     del eng
+    t1 = time.perf_counter()
 
     def one(h):
         counts, e_base = h
         return ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_NEWTON)
 
     with ThreadPoolExecutor(threads) as ex:
-        return list(ex.map(one, hists)), hists, (hb, bs, be)
+        lads = list(ex.map(one, hists))
+    if timing is not None:
+        timing["pilot_s"] = t1 - t0
+        timing["host_s"] = time.perf_counter() - t1
+    return lads, hists, (hb, bs, be)
 
 
 def short_features(ptx, problems, stats, T0, T1, burn_in, samples, seed, inverse=False):
@@ -226,8 +234,10 @@ def choose_ladders(ptx, sur, problems, stats, T0=0.5, T1=3.0, seed=0, short=(500
     fb = [i for i, o in enumerate(ok) if not o]
     t0 = time.perf_counter()
     if fb:
+        ft = {}
         mh, _, _ = multihistogram_ladders(ptx, [problems[i] for i in fb], N_OUT, T0, T1, long[0], long[1], seed + 2,
-                                          inverse)
+                                          inverse, timing=ft)
+        t["fallback_pilot_s"], t["fallback_host_s"] = ft["pilot_s"], ft["host_s"]
         for i, lad in zip(fb, mh):
             lads[i] = lad
     t["fallback_s"] = time.perf_counter() - t0

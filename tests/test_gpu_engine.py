@@ -55,6 +55,27 @@ def test_graph_replay_matches_eager_launches(ptx):
     assert t.fused_launches == 1 + 7 + 3 + 4 + 6
 
 
+def test_graph_replay_of_split_steps_matches_eager(ptx):
+    probs, lads = _batch(ptx, k=10, seed=7)
+    eager = ptx.Engine(probs, lads, seed=8, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT)
+    graph = ptx.Engine(probs, lads, seed=8, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT, use_graph=4)
+    assert eager.info().mode == 1 and graph.info().mode == 1
+    for chunk in (3, 9, 4, 1, 8):  # replays of 4 steps and eager remainders, reads in between
+        eager.step(chunk)
+        graph.step(chunk)
+        _same(_state(eager, len(probs)), _state(graph, len(probs)))
+    graph.set_active(np.arange(len(probs)) % 2 == 0)  # new work lists: the graph is rebuilt
+    eager.set_active(np.arange(len(probs)) % 2 == 0)
+    eager.step(8)
+    graph.step(8)
+    _same(_state(eager, len(probs)), _state(graph, len(probs)))
+
+
+def test_auto_mode_runs_batches_on_the_split_kernels(ptx):
+    probs, lads = _batch(ptx, k=6, seed=9)
+    assert ptx.Engine(probs, lads, seed=1, flags=ptx.FLAG_ICM).info().mode == 1
+
+
 def test_autotuned_shape_gives_identical_results(ptx):
     probs, lads = _batch(ptx, k=8, seed=1)
     kw, trials = ptx.autotune(probs, lads, seed=3, steps=8, flags=ptx.FLAG_ICM)
@@ -79,3 +100,31 @@ def test_autotune_single_instance_includes_cooperative(ptx):
     a.step(9)
     b.step(9)
     _same(_state(a, 1), _state(b, 1))
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_retired_instances_freeze_and_the_rest_match(ptx, mode):
+    probs, lads = _batch(ptx, k=12, seed=2)
+    ref = ptx.Engine(probs, lads, seed=4, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=1)
+    eng = ptx.Engine(probs, lads, seed=4, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=1)
+    ref.step(5)
+    eng.step(5)
+    frozen = _state(eng, len(probs))
+    mask = np.array([i % 3 != 1 for i in range(len(probs))])
+    eng.set_active(mask)
+    ref.step(7)
+    eng.step(7)
+    now, want = _state(eng, len(probs)), _state(ref, len(probs))
+    for i in range(len(probs)):
+        if mask[i]:
+            _same([want[i]], [now[i]])   # the active ones run exactly as in the full batch
+        else:
+            _same([frozen[i]], [now[i]])  # the retired ones did not move
+    eng.set_active(np.ones(len(probs), bool))
+    eng.step(3)
+    for i, p in enumerate(probs):  # reactivated instances keep consistent energies
+        sp = eng.spins(i)
+        assert np.array_equal(eng.energies(i), [p.energy(x) for x in sp])
+    with pytest.raises(ptx.PtxError):
+        eng.set_active(np.zeros(len(probs), bool))
+        eng.set_spins(eng.spins(0), 0)

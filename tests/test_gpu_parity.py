@@ -251,8 +251,8 @@ def test_batched_3d_ea_invariants(ptx, oracle):
         probs.append(ptx.Problem.from_coo(n, u, v, w))
         graphs.append(oracle.Graph.from_coo(n, u, v, w))
     lad = oracle.beta_linspace(0.2, 3.0, 32, 1)
-    eng = ptx.Engine(probs, [lad] * len(probs), seed=3, flags=ptx.FLAG_ICM)
-    assert eng.info().mode == 2  # batches run the fused kernel
+    eng = ptx.Engine(probs, [lad] * len(probs), seed=3, flags=ptx.FLAG_ICM, mode=ptx.MODE_FUSED)
+    assert eng.info().mode == 2
     eng.step(40)
     for i in range(0, len(probs), 7):
         sp = eng.spins(i)
