@@ -81,7 +81,9 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
     """The batched workload: 4096 independent 3D Edwards-Anderson +-J instances,
     L uniform in {6, 8, 10, 12}, 32 inverse temperatures 0.2..3.0 x 2 chains,
     ICM by the slot rule.  The launch shape comes from the autotuner (timed
-    separately); fused launches are replayed from a hipGraph of 4 launches."""
+    separately).  The timed run launches eagerly (per-kernel HIP events for the
+    roofline); the same steps are then replayed from hipGraphs and timed by wall
+    clock."""
     sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
     import instances
     _, coos = instances.ea_batch(n_inst, seed)
@@ -100,6 +102,12 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
     eng.sync()
     dt = time.perf_counter() - t0
     tm = eng.timing()
+    # the same number of steps replayed from hipGraphs (per-kernel events off)
+    eng.timing(False)
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    dt_graph = time.perf_counter() - t0
     info = eng.info()
     if info.mode == 2:
         kern, avg_ms = "k_fused", tm.fused_ms / max(tm.fused_launches, 1)
@@ -113,8 +121,9 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
             "ms_per_step": dt * 1e3 / steps, "steps": steps, "instances_per_hour_at_10k_steps":
                 n_inst * 3600.0 / (dt / steps * 10000),
             "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
-                             "csr_in_lds": info.csr_in_lds, "steps_per_launch": info.steps_per_launch,
-                             "graph_launches": 4 if info.mode == 2 else 0},
+                             "csr_in_lds": info.csr_in_lds, "steps_per_launch": info.steps_per_launch},
+            "ms_per_step_graph_replay": dt_graph * 1e3 / steps,
+            "graph": "4 launches (fused) or 4 PT steps (split) per hipGraph replay",
             "autotune_s": t_tune, "autotune_trials": [{k: (round(v, 1) if k == "us_per_step" else v)
                                                        for k, v in t.items()} for t in trials],
             "roofline": {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_GBS, "unit": "GB/s",
@@ -220,6 +229,9 @@ def main():
                                                     "surrogate_accepted", "ladder_s", "run_s", "ms_per_step",
                                                     "spin_updates_per_s", "mean_slots", "mean_icm_slots", "mode",
                                                     "waves_per_group", "autotune_s")},
+            "batched_multihistogram": {k: r["batched_multihistogram"][k]
+                                       for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
+                                                 "ladder_s", "run_s")},
             "plain": {k: pl[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                          "mean_preprocess_s")},
             "instances_per_hour_ratio": r["instances_per_hour_ratio"]}

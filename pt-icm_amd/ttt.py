@@ -97,12 +97,18 @@ def calibrate(ptx, specs, seeds, steps, log=print):
     return best, [len(l) for l in lads]
 
 
-def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print):
+def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate=True):
+    """surrogate=False: every ladder from the batched multihistogram preprocess."""
     probs, stats = build_problems(ptx, specs)
     targets = np.asarray(targets)
     t_start = time.perf_counter()
-    sur = S.Surrogate(ptx)
-    lads, ok, _, stages = S.choose_ladders(ptx, sur, probs, stats, T0, T1, seed=seed)
+    if surrogate:
+        sur = S.Surrogate(ptx)
+        lads, ok, _, stages = S.choose_ladders(ptx, sur, probs, stats, T0, T1, seed=seed)
+    else:
+        ft = {}
+        lads, _, _ = S.multihistogram_ladders(ptx, probs, S.N_OUT, T0, T1, 5000, 2000, seed=seed + 2, timing=ft)
+        ok, stages = [False] * len(probs), {"pilot_s": ft["pilot_s"], "host_s": ft["host_s"]}
     lads = [np.sort(l) for l in lads]
     t_lad = time.perf_counter() - t_start
     log(f"batched ladders in {t_lad:.2f} s ({sum(ok)} of {len(ok)} surrogate ladders accepted) {stages}")
@@ -110,7 +116,7 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print):
     t_tune = time.perf_counter()
     kw, _ = ptx.autotune(probs, lads, seed=seed + 10, steps=32, flags=ptx.FLAG_ICM)
     t_tune = time.perf_counter() - t_tune
-    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM, use_graph=4, **kw)
+    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM, use_graph=(50 if kw["mode"] == 1 else 4), **kw)
     steps = 0
     t0 = time.perf_counter()
     # instances that reached E* are retired from the batch at every poll; the
@@ -185,12 +191,14 @@ def run(ptx, n_plain=8, seed=2024, log=print, path=TARGETS):
     cfg = load_targets(path)
     specs, targets = cfg["specs"], np.array(cfg["targets"], np.int64)
     b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
+    bm = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log, surrogate=False)
     # an even sample of both families (specs alternate torus, ER)
     stride = max(2, 2 * (len(specs) // max(n_plain, 1) // 2))
     idx = sorted(list(range(0, len(specs), stride))[: (n_plain + 1) // 2] +
                  list(range(1, len(specs), stride))[: n_plain // 2])
     pl = run_plain(ptx, [specs[i] for i in idx], targets[idx], cfg["max_steps"], cfg["poll"], seed + 7, log)
-    return {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "batched_surrogate": b, "plain": pl,
+    return {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "batched_surrogate": b,
+            "batched_multihistogram": bm, "plain": pl,
             "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"]}
 
 
