@@ -145,6 +145,7 @@ typedef struct {
                                 hipGraph and replays it (0 = eager launches) */
     int32_t spins_per_lane;  /* cooperative mode: 1 or 4 positions per lane (0 = auto) */
     int32_t csr_placement;   /* fused mode: PTX_CSR_AUTO, PTX_CSR_LDS or PTX_CSR_GLOBAL */
+    int32_t icm_waves;       /* split mode: waves per ICM slot pair, 1, 2 or 4 (0 = 1) */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 
@@ -160,14 +161,15 @@ void ptx_engine_destroy(ptx_engine* e);
 /* Launch-shape autotuner (no counterpart in the reference, which has one
  * CPU code path).  Times `steps` PT steps (after a warm-up launch) of every
  * launch shape that fits the batch -- execution mode, waves (replicas) per
- * workgroup, positions per lane, adjacency in LDS or read through L2 -- and
- * writes the fastest into *best (a copy of *base with mode, waves_per_group,
- * spins_per_lane and csr_placement set).  The random streams are indexed by
- * position, replica and step, so every shape yields the same trajectories.
+ * workgroup, positions per lane, adjacency in LDS or read through L2, waves
+ * per cluster move -- and writes the fastest into *best (a copy of *base
+ * with mode, waves_per_group, spins_per_lane, csr_placement and icm_waves
+ * set).  The random streams are indexed by position, replica and step, so
+ * every shape yields the same trajectories.
  * trials (may be NULL) receives up to max_trials measured shapes, *n_trials
  * their count. */
 typedef struct {
-    int32_t mode, waves_per_group, spins_per_lane, csr_placement;
+    int32_t mode, waves_per_group, spins_per_lane, csr_placement, icm_waves;
     double us_per_step;
 } ptx_tune_trial;
 ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -196,6 +198,8 @@ typedef struct {
     int64_t spin_updates_per_step; /* sum over replicas of n * sweeps */
     int64_t lds_bytes_per_step;    /* algorithmic LDS bytes of the sweep (4 d_i + 6 per update) */
     int32_t csr_in_lds;            /* fused mode: 1 if the adjacency is staged in LDS */
+    int32_t icm_waves;             /* waves that grow one cluster (split: per pair; fused and
+                                      cooperative: 1) */
 } ptx_engine_info;
 ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out);
 

@@ -234,6 +234,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->use_graph = 0;
     c->spins_per_lane = 0;
     c->csr_placement = PTX_CSR_AUTO;
+    c->icm_waves = 0;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -274,15 +275,21 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
             ps[i] = problems[i]->p;
         }
         struct Shape {
-            int32_t mode, w, spl, csr;
+            int32_t mode, w, spl, csr, wi;
         };
         std::vector<Shape> shapes;
         if (n_instances == 1)
             for (int w : {4, 8})
-                for (int spl : {1, 4}) shapes.push_back({PTX_MODE_COOP, w, spl, PTX_CSR_AUTO});
+                for (int spl : {1, 4}) shapes.push_back({PTX_MODE_COOP, w, spl, PTX_CSR_AUTO, 0});
         for (int w : {4, 8, 16})
-            for (int csr : {PTX_CSR_LDS, PTX_CSR_GLOBAL}) shapes.push_back({PTX_MODE_FUSED, w, 0, csr});
-        for (int w : {1, 4}) shapes.push_back({PTX_MODE_SPLIT, w, 0, PTX_CSR_AUTO});
+            for (int csr : {PTX_CSR_LDS, PTX_CSR_GLOBAL}) shapes.push_back({PTX_MODE_FUSED, w, 0, csr, 0});
+        // split: one wave per cluster (the default, 0) or four
+        const bool icm = (b0.flags & PTX_FLAG_ICM) != 0;
+        for (int w : {1, 4})
+            for (int wi : {0, 4}) {
+                if (!icm && wi != 0) continue;  // no cluster moves: one split shape per sweep width
+                shapes.push_back({PTX_MODE_SPLIT, w, 0, PTX_CSR_AUTO, wi});
+            }
         double best_us = 0;
         int32_t cnt = 0;
         bool found = false;
@@ -292,6 +299,7 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
             c.waves_per_group = sh.w;
             c.spins_per_lane = sh.spl;
             c.csr_placement = sh.csr;
+            c.icm_waves = sh.wi;
             ptx::Engine e;
             if (!e.create(ps, betas, n_temps, seed, c)) continue;  // shape does not fit this batch
             if (e.mode != sh.mode) continue;
@@ -301,7 +309,9 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
             if (!e.step(steps) || !e.sync()) continue;
             const double us =
                 std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / steps;
-            if (trials && cnt < max_trials) trials[cnt] = ptx_tune_trial{sh.mode, e.mode == PTX_MODE_FUSED ? e.Wf : (e.mode == PTX_MODE_COOP ? e.Wf : e.W), sh.spl, sh.csr, us};
+            if (trials && cnt < max_trials)
+                trials[cnt] = ptx_tune_trial{sh.mode, e.mode == PTX_MODE_SPLIT ? e.W : e.Wf, sh.spl, sh.csr,
+                                             e.mode == PTX_MODE_SPLIT ? e.icm_waves_now() : 0, us};
             ++cnt;
             if (!found || us < best_us) {
                 found = true;
@@ -342,6 +352,7 @@ ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out) {
     out->spin_updates_per_step = E.updates_per_step;
     out->lds_bytes_per_step = E.lds_bytes_per_step;
     out->csr_in_lds = E.mode == PTX_MODE_SPLIT ? 0 : (E.fused_global ? 0 : 1);
+    out->icm_waves = E.mode == PTX_MODE_SPLIT ? E.icm_waves_now() : 1;
     return PTX_OK;
 }
 

@@ -16,6 +16,7 @@ namespace ptx {
 
 struct CoopArgs {
     uint32_t* pub_spins;        // [2][nrep][nw]
+    int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
     int32_t* pub_E;             // [2][nrep]
     unsigned long long* bar;    // grid barrier counter (monotonic)
     uint32_t* err;              // set to 1 when a barrier wait times out
@@ -32,12 +33,23 @@ struct CoopArgs {
 // go to its slot in one of two alternating LDS buffers (one barrier per class).
 constexpr int kMaxCoopWaves = 8;  // waves per cooperative workgroup (4 or 8)
 
+// The random words of a whole sweep of replica r: pre[p] = word p&3 of the
+// Philox block (p>>2, r, gsweep) -- the words the sweep would draw for p.
+// Threads t0, t0 + stride, ... each take whole blocks.
+__device__ __forceinline__ void fill_sweep_randoms(uint32_t* pre, int n, uint32_t c3, uint64_t seed, int r,
+                                                   uint32_t gsweep, int t0, int stride) {
+    for (int g = t0; 4 * g < n; g += stride) {
+        const Philox4 x = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+        *reinterpret_cast<uint4*>(pre + 4 * g) = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+    }
+}
+
 template <int Fmt, int SPL>
 __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
                                                  const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
                                                  uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
                                                  const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
-                                                 int gl, int nthr, int* acc, uint32_t& cc) {
+                                                 int gl, int nthr, int* acc, uint32_t& cc, const uint32_t* pre) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int lane = gl % kWave;
     const int nwv = nthr / kWave, wv = gl / kWave;
@@ -53,8 +65,10 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
                 const int dE = sp[p] ? -2 * f : 2 * f;
                 bool accept = dE <= 0;
                 if (!accept) {
-                    const Philox4 rnd = philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1);
-                    accept = pick(rnd, (uint32_t)p) < thr_row[(dE >> 1) - 1];
+                    const uint32_t u = pre ? pre[p]
+                                           : pick(philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1),
+                                                  (uint32_t)p);
+                    accept = u < thr_row[(dE >> 1) - 1];
                 }
                 if (accept) {
                     sp[p] ^= 1u;
@@ -64,7 +78,7 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
             }
         } else {
             for (int g = (cs >> 2) + gl; 4 * g < ce; g += nthr)
-                group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc);
+                group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc, pre);
         }
         dsum = wave_sum_i32(dsum);
         nacc = wave_sum_i32(nacc);
@@ -123,10 +137,12 @@ __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err
 }
 
 struct CoopLayout {
-    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, acc, flag, total;
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, acc, flag, rnd,
+        total;
 };
 
-__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax) {
+// pre_rnd = 1 adds the random words of one sweep (4 bytes per position).
+__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax, int pre_rnd) {
     CoopLayout L;
     int o = 0;
     auto take = [&](int bytes) {
@@ -154,6 +170,7 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     L.icm = take(icm_scratch_bytes(n));
     L.acc = take(4 * 2 * 2 * kMaxCoopWaves);
     L.flag = take(4);
+    L.rnd = take(pre_rnd ? 16 * ((n + 3) / 4) : 0);
     L.total = o;
     return L;
 }
@@ -166,7 +183,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const InstDesc d = a.desc[c.inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
     const int r = blockIdx.x;  // this workgroup's replica
-    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax);
+    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -191,6 +208,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     sc.V = sc.D + (nw + 1);
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
+    sc.red = nullptr;  // one wave grows the cluster: no workgroup sums
 
     // ---- stage the instance and this replica ----
     for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
@@ -218,11 +236,23 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     auto prepare_half = [&](uint32_t s, int h) {
         const ExchPre x = exch_prepare(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, s, lane, h);
         ExchPre& slot = pre[(s & 1u) * kWave + lane];
-        slot.tpos[h] = x.tpos[h];
-        slot.level[h] = x.level[h];
-        slot.lu[h] = x.lu[h];
+        // constant indices keep x in registers
+        if (h == 0) {
+            slot.tpos[0] = x.tpos[0];
+            slot.level[0] = x.level[0];
+            slot.lu[0] = x.lu[0];
+        } else {
+            slot.tpos[1] = x.tpos[1];
+            slot.level[1] = x.level[1];
+            slot.lu[1] = x.lu[1];
+        }
     };
     if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[c.inst], wave - 1);
+    // random words of the first sweep of this launch (later ones are drawn
+    // while the grid barrier is pending)
+    uint32_t* pre_rnd = c.pre_rnd ? reinterpret_cast<uint32_t*>(smem + L.rnd) : nullptr;
+    const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
+    if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[c.inst] * (uint32_t)a.sweeps, tid, nthr);
     __syncthreads();
     const Csr<Fmt> csr{info, smem + L.adj};
     const int32_t* hf = a.hfield + d.pos_off;
@@ -257,15 +287,20 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         for (int sw = 0; sw < a.sweeps; ++sw) {
             const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
             sweep_replica_wg<Fmt, SPL>(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, sp,
-                                       thr_row, best_words, st, tid, nthr, acc, cc);
+                                       thr_row, best_words, st, tid, nthr, acc, cc, sw == 0 ? pre_rnd : nullptr);
         }
         phase_done(0);
         // ---- publish and wait ----
         pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
         if (tid == 0) c.pub_E[par * nrep + r] = st.E;
         grid_arrive(c.bar);
-        // waves 1 and 2 prepare the next step's exchange while thread 0 waits
+        // while thread 0 waits: waves 1 and 2 prepare the next step's
+        // exchange, and waves 1.. draw the next step's first-sweep random
+        // words (this step's sweep has finished reading them)
         if (prepared && (wave == 1 || wave == 2) && it + 1 < nsteps) prepare_half(step + 1u, wave - 1);
+        if (pre_rnd && wave > 0 && it + 1 < nsteps)
+            fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
+                               nthr - kWave);
         if (!grid_wait(c.bar, c.err, (c.bar_base + (unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
                        flag))
             break;
@@ -295,19 +330,20 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             const int t = r2s[r] % nt;
             if (t >= nt - d.nc) {
                 const int r1 = s2r[t], r2 = s2r[t + nt];
+                // one wave grows the cluster (its frontiers are mostly narrower
+                // than a wave, so more waves would only add barriers)
                 if (wave == 0) {
                     long long dE1 = 0, dE2 = 0;
-                    icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
-                             d.has_field != 0, n, nw, t, step, a.seed, c3i,
-                             c.pub_spins + ((int64_t)par * nrep + r1) * nw,
-                             c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, false);
-                    if (lane == 0) st.E += (int32_t)(r == r1 ? dE1 : dE2);
+                    icm_team<Fmt, false>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
+                                         d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                                         c.pub_spins + ((int64_t)par * nrep + r1) * nw,
+                                         c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, kWave,
+                                         false);
+                    if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
                 }
                 __syncthreads();
+                st.E = flag[0];  // the new energy, shared with every wave
                 expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
-                if (tid == 0) flag[0] = st.E;  // share the new energy with every wave
-                __syncthreads();
-                st.E = flag[0];
                 __syncthreads();
             }
         }

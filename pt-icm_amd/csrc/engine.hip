@@ -193,6 +193,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W <= 0) W = (total_rep < 4 * 256) ? 1 : 4;  // spread few replicas over CUs, group many
     if (W > 8) W = 8;                                 // split kernels take at most 8 waves
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
+    Wi = cfg.icm_waves;
+    if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
     spin_stride = (int32_t)align_up((int64_t)spin_bytes(nw_max), 16);
     thr_stride = (int32_t)align_up(fmax_max, 4);
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
@@ -213,7 +215,11 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
         PTX_TRY(hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
         PTX_TRY(hip_check(hipDeviceGetAttribute(&coopok, hipDeviceAttributeCooperativeLaunch, dev), "coop attr"));
-        const CoopLayout L = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax);
+        // with the next sweep's random words in LDS when they fit, else without
+        const CoopLayout L1 = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax, 1);
+        const CoopLayout L0 = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax, 0);
+        coop_pre_rnd = L1.total <= 160 * 1024 ? 1 : 0;
+        const CoopLayout& L = coop_pre_rnd ? L1 : L0;
         const bool fits = coopok && L.total <= 160 * 1024 && desc[0].nrep <= ncu;
         if (fits) {
             mode = PTX_MODE_COOP;
@@ -419,12 +425,24 @@ Status Engine::launch_step() {
         }));
     }
     if (!pairs.empty()) {
-        const size_t lds = (size_t)(nw_max + 1) * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
+        const int wi = icm_waves_now();
+        const size_t lds = 8 * kIcmRedSlots + (size_t)(nw_max + 1) * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
+        const dim3 grid((unsigned)pairs.size()), block((unsigned)(wi * kWave));
         PTX_TRY(timed(2, [&] {
-            if (fmt == kFmtUnit)
-                hipLaunchKernelGGL(k_icm<kFmtUnit>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, nw_max);
-            else
-                hipLaunchKernelGGL(k_icm<kFmtWide>, dim3((unsigned)pairs.size()), dim3(kWave), lds, stream, a, nw_max);
+            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, nw_max); };
+            if (fmt == kFmtUnit) {
+                switch (wi) {
+                    case 1: go(k_icm<kFmtUnit, 1>); break;
+                    case 2: go(k_icm<kFmtUnit, 2>); break;
+                    default: go(k_icm<kFmtUnit, 4>); break;
+                }
+            } else {
+                switch (wi) {
+                    case 1: go(k_icm<kFmtWide, 1>); break;
+                    case 2: go(k_icm<kFmtWide, 2>); break;
+                    default: go(k_icm<kFmtWide, 4>); break;
+                }
+            }
         }));
     }
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream, d_step.p);
@@ -493,6 +511,7 @@ Status Engine::launch_coop(int nsteps) {
     c.bar_base = barriers_done;
     c.nblocks = desc[0].nrep;
     c.inst = 0;
+    c.pre_rnd = coop_pre_rnd;
     const dim3 grid((unsigned)desc[0].nrep), block((unsigned)(Wf * kWave));
     const unsigned lds = (unsigned)coop_lds;
     auto go = [&](auto kern) -> Status {

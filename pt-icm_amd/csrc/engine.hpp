@@ -53,6 +53,7 @@ struct Engine {
     int64_t total_rep = 0;
     int64_t steps_done = 0;
     int32_t W = 1;               // waves per sweep workgroup (split mode)
+    int32_t Wi = 0;              // waves per ICM slot pair (split mode; 0 = by the number of pairs)
     int32_t mode = PTX_MODE_SPLIT;
     int32_t fmt = 0;             // adjacency record format of the fused kernel
     int32_t Wf = 8;              // waves per workgroup (fused mode)
@@ -96,6 +97,7 @@ struct Engine {
     DevBuf<unsigned long long> d_bar;
     unsigned long long barriers_done = 0;
     int32_t spl = 1;             // positions per lane (cooperative mode)
+    int32_t coop_pre_rnd = 0;    // cooperative mode: next sweep's random words drawn during the grid barrier
     int32_t coop_lds = 0;
     bool phase_on = false;
     // fused mode with use_graph = G > 0: G launches of K steps captured once
@@ -130,6 +132,12 @@ struct Engine {
 
   public:
     Status set_active(const uint8_t* flags);
+    // waves per ICM slot pair: the configured number, else 1.  Clusters grow
+    // level by level and their frontiers are mostly narrower than one wave
+    // (2D lattices: ~10 sites per level over ~30 levels), so more waves
+    // only add barriers; the autotuner still tries 4.
+    int icm_waves_now() const { return Wi > 0 ? Wi : 1; }
+
     Status collect_timing();
 };
 

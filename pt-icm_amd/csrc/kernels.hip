@@ -6,7 +6,7 @@
 //   k_exchange  slot-energy histogram + (NN)_a exchange, one wavefront per
 //               instance (replica_exchange.cc:5-100; cae_estimator.cc:122-139
 //               in pilot mode)
-//   k_icm       Houdayer cluster move, one wavefront per slot pair (icm.cc:8-215)
+//   k_icm       Houdayer cluster move, 1, 2 or 4 wavefronts per slot pair (icm.cc:8-215)
 // Fused form (batched instances): k_fused runs whole PT steps -- sweeps of
 // every replica, the exchange and the cluster moves -- for one instance per
 // workgroup with the adjacency staged in LDS, several steps per launch.
@@ -167,19 +167,21 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, int nrep_max) {
 }
 
 // ---------------------------------------------------------------------------
-// k_icm: one wavefront per (instance, slot t) pair, t in [nt-nc, nt).
+// k_icm: one workgroup of W waves per (instance, slot t) pair, t in [nt-nc, nt).
+// LDS: W reduction slots, then the ICM scratch of the largest instance.
 // ---------------------------------------------------------------------------
-template <int Fmt>
-__global__ __launch_bounds__(kWave) void k_icm(DevArgs a, int nw_max) {
+template <int Fmt, int W>
+__global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max) {
     const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     const int2 pr = a.pairs[blockIdx.x];
     if (pr.x < 0) return;  // idle item padding an XCD's list
     const InstDesc d = a.desc[pr.x];
     const int t = pr.y;
     IcmScratch sc;
-    sc.S1 = reinterpret_cast<uint32_t*>(smem);
+    sc.red = reinterpret_cast<long long*>(smem);
+    sc.S1 = reinterpret_cast<uint32_t*>(smem + 8 * kIcmRedSlots);
     sc.S2 = sc.S1 + (nw_max + 1);
     sc.D = sc.S2 + (nw_max + 1);
     sc.V = sc.D + (nw_max + 1);
@@ -191,10 +193,10 @@ __global__ __launch_bounds__(kWave) void k_icm(DevArgs a, int nw_max) {
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
     const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
     long long dE1 = 0, dE2 = 0;
-    icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, a.hfield + d.pos_off,
-             d.has_field != 0, d.n, d.nw, t, step, a.seed,
-             ctr3(kTagIcm | a.tag_base, d.inst_id), g1, g2, sc, dE1, dE2, lane);
-    if (lane == 0) {
+    icm_team<Fmt, (W > 1)>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, a.hfield + d.pos_off,
+                           d.has_field != 0, d.n, d.nw, t, step, a.seed, ctr3(kTagIcm | a.tag_base, d.inst_id), g1,
+                           g2, sc, dE1, dE2, tid, W * kWave);
+    if (tid == 0) {
         a.E[ro + r1] += (int32_t)dE1;
         a.E[ro + r2] += (int32_t)dE2;
     }
@@ -317,6 +319,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     sc.V = sc.D + (nw + 1);
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
+    sc.red = nullptr;  // one wave per pair: no workgroup sums
 
     const bool prof = a.phase != nullptr && blockIdx.x == 0 && tid == 0;
     unsigned long long t_mark = prof ? wall_clock64() : 0ull;
@@ -376,10 +379,10 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
                 const int t = nt - 1 - j;
                 const int r1 = s2r[t], r2 = s2r[t + nt];
                 long long dE1 = 0, dE2 = 0;
-                icm_wave(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf, d.has_field != 0, n,
-                         nw, t, step, a.seed, c3i,
-                         a.spins + d.spin_off + (int64_t)r1 * nw, a.spins + d.spin_off + (int64_t)r2 * nw, sc, dE1,
-                         dE2, lane);
+                icm_team<Fmt, false>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
+                                     d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                                     a.spins + d.spin_off + (int64_t)r1 * nw, a.spins + d.spin_off + (int64_t)r2 * nw,
+                                     sc, dE1, dE2, lane, kWave);
                 if (lane == 0) {
                     E[r1] += (int32_t)dE1;
                     E[r2] += (int32_t)dE2;
@@ -476,7 +479,9 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint
     template __global__ void k_sweep<F, 2>(DevArgs);              \
     template __global__ void k_sweep<F, 4>(DevArgs);              \
     template __global__ void k_sweep<F, 8>(DevArgs);              \
-    template __global__ void k_icm<F>(DevArgs, int);              \
+    template __global__ void k_icm<F, 1>(DevArgs, int);           \
+    template __global__ void k_icm<F, 2>(DevArgs, int);           \
+    template __global__ void k_icm<F, 4>(DevArgs, int);           \
     template __global__ void k_fused<F, 4, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 8, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 16, 0>(DevArgs, int);     \

@@ -321,7 +321,9 @@ template <int Fmt>
 __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs, int ce,
                                              const int32_t* __restrict__ hf, bool has_field, uint32_t c3, uint32_t k0,
                                              uint32_t k1, int r, uint32_t gsweep, uint8_t* sp,
-                                             const uint32_t* thr_row, int& dsum, int& nacc) {
+                                             const uint32_t* thr_row, int& dsum, int& nacc,
+                                             const uint32_t* pre = nullptr) {
+    // pre (optional): this sweep's random words, pre[p] = word p&3 of block p>>2
     const int p0 = 4 * g;
     int f[4];
     csr.group_field(p0, cs, ce, sp, f);
@@ -337,7 +339,17 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
         any_up |= dE[k] > 0;
     }
     Philox4 rnd = {{0u, 0u, 0u, 0u}};
-    if (any_up) rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
+    if (any_up) {
+        if (pre) {
+            const uint4 q = *reinterpret_cast<const uint4*>(pre + p0);
+            rnd.v[0] = q.x;
+            rnd.v[1] = q.y;
+            rnd.v[2] = q.z;
+            rnd.v[3] = q.w;
+        } else {
+            rnd = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
+        }
+    }
     uint32_t flip = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -448,8 +460,9 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
         x.level[h] = 0;
         x.lu[h] = 0.0;
     }
-    for (int h = 0; h < nhalf; ++h) {
-        if (only_half >= 0 && h != only_half) continue;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // unrolled: x is indexed by constants and stays in registers
+        if (h >= nhalf || (only_half >= 0 && h != only_half)) continue;
         // rank of pair kc in this half's order: pairs sorted by (key, index)
         const uint32_t key =
             is_pair ? pick(philox4x32_10((uint32_t)kc >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)kc) : 0u;
@@ -496,7 +509,9 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
     const double bk1 = __shfl_down(bk, 1, kWave);  // beta of slot k + 1 (used only at a decision)
     uint32_t natt = 0, nsucc = 0;
     unsigned long long nsw = 0;
-    for (int h = 0; h < nhalf; ++h) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // unrolled: x is indexed by constants and stays in registers
+        if (h >= nhalf) break;
         const int level = x.level[h];
         const int maxlv = wave_max_i32(level);
         for (int L = 1; L <= maxlv; ++L) {
@@ -636,11 +651,14 @@ __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bo
 }
 
 // ---------------------------------------------------------------------------
-// Houdayer cluster move on one slot pair by one wavefront (icm.cc:8-215).
-// S1/S2 hold both states (bit words, copied in from global), D the
-// disagreement set, V the cluster; Q is a queue of up to n/2 positions.  The
-// cluster is grown breadth-first from a uniform seed in D; the reference grows
-// it depth-first (icm.cc:37-72), which gives the same component.  Returns the
+// Houdayer cluster move on one slot pair (icm.cc:8-215), by one wavefront
+// (Block = false) or by a whole workgroup (Block = true; tid/nthr span the
+// workgroup).  S1/S2 hold both states (bit words, copied in from global), D
+// the disagreement set, V the cluster; Q is a queue of up to n/2 positions.
+// The cluster is grown breadth-first from a uniform seed in D; the reference
+// grows it depth-first (icm.cc:37-72), which gives the same component.  With
+// several waves the queue order depends on timing, but the component, the
+// flipped states and the (integer) energy changes do not.  Returns the
 // energy changes of both replicas; the new states stay in S1/S2 and are
 // written to g1/g2 unless write_back is false.
 // ---------------------------------------------------------------------------
@@ -651,14 +669,50 @@ struct IcmScratch {
     uint32_t* V;
     int32_t* tail;
     uint16_t* Q;
+    long long* red;  // Block: one slot per wave for workgroup sums
 };
 
-template <int Fmt>
-__device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const int32_t* hf, bool has_field, int n,
-                                         int nw,
-                                         int t, uint32_t step, uint64_t seed, uint32_t c3, const uint32_t* g1_in,
-                                         const uint32_t* g2_in, IcmScratch sc, long long& dE1_out, long long& dE2_out,
-                                         int lane, bool write_back = true) {
+// Per-wave slots of the workgroup reductions in the ICM team code.
+constexpr int kIcmRedSlots = 16;
+
+template <bool Block>
+__device__ __forceinline__ void team_sync() {
+    if (Block)
+        __syncthreads();
+    else
+        wave_sync();
+}
+
+template <bool Block>
+__device__ __forceinline__ long long team_sum_i64(long long v, long long* red, int tid, int nthr) {
+    v = wave_sum_i64(v);
+    if (!Block || nthr <= kWave) return v;
+    __syncthreads();  // the previous sum's reads of red are done
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
+    __syncthreads();
+    long long s = 0;
+    for (int q = 0; q < nthr / kWave; ++q) s += red[q];
+    return s;
+}
+
+template <bool Block>
+__device__ __forceinline__ int team_sum_i32(int v, long long* red, int tid, int nthr) {
+    v = wave_sum_i32(v);
+    if (!Block || nthr <= kWave) return v;
+    __syncthreads();
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
+    __syncthreads();
+    long long s = 0;
+    for (int q = 0; q < nthr / kWave; ++q) s += red[q];
+    return (int)s;
+}
+
+template <int Fmt, bool Block>
+__device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const int32_t* hf, bool has_field, int n,
+                                         int nw, int t, uint32_t step, uint64_t seed, uint32_t c3,
+                                         const uint32_t* g1_in, const uint32_t* g2_in, IcmScratch sc,
+                                         long long& dE1_out, long long& dE2_out, int tid, int nthr,
+                                         bool write_back = true) {
     // the new states are left in sc.S1/sc.S2 and, with write_back, stored to g1/g2
     uint32_t* g1 = const_cast<uint32_t*>(g1_in);
     uint32_t* g2 = const_cast<uint32_t*>(g2_in);
@@ -666,9 +720,10 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const i
     uint32_t* S2 = sc.S2;
     uint32_t* D = sc.D;
     uint32_t* V = sc.V;
+    const int lane = tid & (kWave - 1);
     const uint32_t lastmask = (n & 31) ? ((1u << (n & 31)) - 1u) : 0xFFFFFFFFu;
     int cnt = 0;
-    for (int w = lane; w < nw; w += kWave) {
+    for (int w = tid; w < nw; w += nthr) {
         const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
         const uint32_t x1 = g1[w] & m, x2 = g2[w] & m;
         S1[w] = x1;
@@ -677,83 +732,88 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const i
         V[w] = 0u;
         cnt += __popc(x1 ^ x2);
     }
-    cnt = wave_sum_i32(cnt);
+    cnt = team_sum_i32<Block>(cnt, sc.red, tid, nthr);
     long long dE1 = 0, dE2 = 0;
-    wave_sync();
+    team_sync<Block>();
     if (cnt > n / 2) {  // icm.cc:20-28: flip replica 1 globally
         if (has_field) {
-            for (int p = lane; p < n; p += kWave) {
+            for (int p = tid; p < n; p += nthr) {
                 const int s1 = (S1[p >> 5] >> (p & 31)) & 1u;
                 dE1 += -2ll * (s1 ? 1 : -1) * hf[p];
             }
         }
-        wave_sync();
-        for (int w = lane; w < nw; w += kWave) {
+        team_sync<Block>();
+        for (int w = tid; w < nw; w += nthr) {
             const uint32_t m = (w == nw - 1) ? lastmask : 0xFFFFFFFFu;
             S1[w] = ~S1[w] & m;
             D[w] = ~D[w] & m;
         }
         cnt = n - cnt;
-        wave_sync();
+        team_sync<Block>();
     }
     if (cnt > 0) {
-        const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
-        const int k = (int)(((unsigned long long)pick(x, (uint32_t)t) * (unsigned long long)cnt) >> 32);
-        // k-th element of D in position order: per-64-word chunks, inclusive scan
-        int seed_pos = -1, base = 0;
-        for (int w0 = 0; w0 < nw && seed_pos < 0; w0 += kWave) {
-            const int w = w0 + lane;
-            const uint32_t dw = w < nw ? D[w] : 0u;
-            const int c = __popc(dw);
-            int incl = c;
+        if (tid < kWave) {  // the first wave picks the seed
+            const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+            const int k = (int)(((unsigned long long)pick(x, (uint32_t)t) * (unsigned long long)cnt) >> 32);
+            // k-th element of D in position order: per-64-word chunks, inclusive scan
+            int seed_pos = -1, base = 0;
+            for (int w0 = 0; w0 < nw && seed_pos < 0; w0 += kWave) {
+                const int w = w0 + lane;
+                const uint32_t dw = w < nw ? D[w] : 0u;
+                const int c = __popc(dw);
+                int incl = c;
 #pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int y = __shfl_up(incl, o, kWave);
-                if (lane >= o) incl += y;
-            }
-            const int total = __shfl(incl, kWave - 1, kWave);
-            if (k < base + total) {
-                const bool mine = (k < base + incl) && (k >= base + incl - c);
-                const unsigned long long ball = __ballot(mine);
-                const int owner = __ffsll((long long)ball) - 1;
-                int sel = -1;
-                if (lane == owner) {
-                    // the kk-th set bit of dw by halving: 5 popcounts, no loop over bits
-                    int kk = k - (base + incl - c), at = 0;
-                    uint32_t x = dw;
-#pragma unroll
-                    for (int sh = 16; sh >= 1; sh >>= 1) {
-                        const uint32_t lo = x & ((1u << sh) - 1u);
-                        const int cl = __popc(lo);
-                        if (kk >= cl) {
-                            kk -= cl;
-                            x >>= sh;
-                            at += sh;
-                        } else {
-                            x = lo;
-                        }
-                    }
-                    sel = 32 * w + at;
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const int y = __shfl_up(incl, o, kWave);
+                    if (lane >= o) incl += y;
                 }
-                seed_pos = __shfl(sel, owner, kWave);
+                const int total = __shfl(incl, kWave - 1, kWave);
+                if (k < base + total) {
+                    const bool mine = (k < base + incl) && (k >= base + incl - c);
+                    const unsigned long long ball = __ballot(mine);
+                    const int owner = __ffsll((long long)ball) - 1;
+                    int sel = -1;
+                    if (lane == owner) {
+                        // the kk-th set bit of dw by halving: 5 popcounts, no loop over bits
+                        int kk = k - (base + incl - c), at = 0;
+                        uint32_t x = dw;
+#pragma unroll
+                        for (int sh = 16; sh >= 1; sh >>= 1) {
+                            const uint32_t lo = x & ((1u << sh) - 1u);
+                            const int cl = __popc(lo);
+                            if (kk >= cl) {
+                                kk -= cl;
+                                x >>= sh;
+                                at += sh;
+                            } else {
+                                x = lo;
+                            }
+                        }
+                        sel = 32 * w + at;
+                    }
+                    seed_pos = __shfl(sel, owner, kWave);
+                }
+                base += total;
             }
-            base += total;
+            if (lane == 0) {
+                V[seed_pos >> 5] |= 1u << (seed_pos & 31);
+                sc.Q[0] = (uint16_t)seed_pos;
+                if (Block) sc.tail[0] = 1;
+            }
         }
-        if (lane == 0) {
-            V[seed_pos >> 5] |= 1u << (seed_pos & 31);
-            sc.Q[0] = (uint16_t)seed_pos;
-        }
-        wave_sync();
+        team_sync<Block>();
         // Breadth-first growth.  Each lane takes one queued site and walks its
         // row a 16-byte chunk at a time: the D and V words of the chunk's
         // neighbours are read together, candidates are claimed with atomicOr
         // on V, and the newly claimed sites are appended at wave-scanned
-        // offsets (the queue tail is a register: only this wave appends).
+        // offsets.  One wave keeps the queue tail in a register; a workgroup
+        // keeps it in LDS (one atomic add per wave and chunk) and reads it
+        // back between two barriers at the end of each round.
         constexpr int K = Csr<Fmt>::kPerChunk;
         int head = 0, tail = 1;
         while (head < tail) {
             const int tl = tail;
-            const int idx = head + lane;
+            const int idx = head + tid;
             const bool act = idx < tl;
             const int p = act ? (int)sc.Q[idx] : 0;
             const int dg = act ? csr.deg(p) : 0;
@@ -803,19 +863,30 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const i
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
                     total += __popcll(b);
                 }
-                int pos = tail + excl;
+                int pos;
+                if (Block) {
+                    int wbase = 0;
+                    if (lane == 0 && total > 0) wbase = atomicAdd(sc.tail, total);
+                    pos = __shfl(wbase, 0, kWave) + excl;
+                } else {
+                    pos = tail + excl;
+                    tail += total;
+                }
 #pragma unroll
                 for (int h = 0; h < K; ++h)
                     if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
-                tail += total;
             }
-            head = (tl - head > kWave) ? head + kWave : tl;
-            wave_sync();
+            head = (tl - head > nthr) ? head + nthr : tl;
+            team_sync<Block>();
+            if (Block) {
+                tail = sc.tail[0];
+                __syncthreads();  // every wave has the tail before the next round appends
+            }
         }
         const int csize = tail;
         // energy change of flipping the cluster C in each replica: boundary
         // edges (p in C, j not in C) and the field terms of C
-        for (int idx = lane; idx < csize; idx += kWave) {
+        for (int idx = tid; idx < csize; idx += nthr) {
             const int p = sc.Q[idx];
             const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
             const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
@@ -845,17 +916,17 @@ __device__ __forceinline__ void icm_wave(const Csr<Fmt>& csr, int maxch, const i
                 dE2 += -2ll * s2 * hf[p];
             }
         }
-        wave_sync();
-        for (int w = lane; w < nw; w += kWave) {
+        team_sync<Block>();
+        for (int w = tid; w < nw; w += nthr) {
             S1[w] ^= V[w];
             S2[w] ^= V[w];
         }
-        wave_sync();
+        team_sync<Block>();
     }
-    dE1_out = wave_sum_i64(dE1);
-    dE2_out = wave_sum_i64(dE2);
+    dE1_out = team_sum_i64<Block>(dE1, sc.red, tid, nthr);
+    dE2_out = team_sum_i64<Block>(dE2, sc.red, tid, nthr);
     if (write_back) {
-        for (int w = lane; w < nw; w += kWave) {
+        for (int w = tid; w < nw; w += nthr) {
             g1[w] = S1[w];
             g2[w] = S2[w];
         }

@@ -169,6 +169,22 @@ def test_icm_all_slots_parity(ptx, oracle, mode):
         compare(eng, oe, what=f"step {s}")
 
 
+@pytest.mark.parametrize("kind,nc", [("G039", 8), ("pm_torus20", -1), ("field_random", 4)])
+@pytest.mark.parametrize("icm_waves", [1, 2, 4])
+def test_icm_waves_independence(ptx, oracle, kind, nc, icm_waves):
+    """Split mode: a cluster grown by 1, 2 or 4 waves (queue order differs,
+    the component does not) gives the oracle's states and energies."""
+    p, g = make_pair(ptx, oracle, kind)
+    betas = oracle.beta_linspace(0.5, 2.0, 8, 0)
+    eng = ptx.Engine(p, betas, seed=41, flags=ptx.FLAG_ICM, nc=nc, mode=ptx.MODE_SPLIT, icm_waves=icm_waves)
+    assert eng.info().icm_waves == icm_waves
+    oe = oracle.Engine(g, betas, chains=2, seed=41, flags=oracle.FLAG_ICM, nc=nc)
+    for s in range(10):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"{kind} icm_waves={icm_waves} step {s}")
+
+
 @pytest.mark.parametrize("mode,wpg,spl,lanes", [(1, 1, 0, 0), (1, 2, 0, 0), (1, 4, 0, 0), (1, 8, 0, 0), (2, 4, 1, 0),
                                                 (2, 8, 7, 0), (2, 16, 16, 0), (3, 4, 5, 1), (3, 8, 16, 1),
                                                 (3, 4, 1, 4), (3, 8, 4, 4)])
