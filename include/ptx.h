@@ -66,6 +66,9 @@ typedef struct {
     int64_t e_min_bound;  /* -(sum |w| over edges + sum |h|) */
     int64_t e_max_bound;
     int32_t has_field;    /* any h_i != 0 */
+    int32_t lattice_dim;  /* 2 or 3 for a periodic L^d lattice in index order (rows exactly the
+                             wrapped axis neighbours): cluster moves grow bit-set clusters; else 0 */
+    int32_t lattice_L;
 } ptx_problem_info;
 ptx_status ptx_problem_info_get(const ptx_problem* p, ptx_problem_info* out);
 

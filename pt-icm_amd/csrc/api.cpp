@@ -116,6 +116,8 @@ ptx_status ptx_problem_info_get(const ptx_problem* p, ptx_problem_info* out) {
     out->e_min_bound = -P.abs_sum;
     out->e_max_bound = P.abs_sum;
     out->has_field = P.has_field ? 1 : 0;
+    out->lattice_dim = P.lat_dim;
+    out->lattice_L = P.lat_L;
     return PTX_OK;
 }
 

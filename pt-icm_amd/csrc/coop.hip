@@ -17,6 +17,7 @@ namespace ptx {
 struct CoopArgs {
     uint32_t* pub_spins;        // [2][nrep][nw]
     int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
+    int32_t lat_stage;          // 1: the lattice tables are copied into LDS
     int32_t* pub_E;             // [2][nrep]
     unsigned long long* bar;    // grid barrier counter (monotonic)
     uint32_t* err;              // set to 1 when a barrier wait times out
@@ -137,12 +138,15 @@ __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err
 }
 
 struct CoopLayout {
-    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, acc, flag, rnd,
-        total;
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, lat_tab, acc,
+        flag, rnd, total;
 };
 
-// pre_rnd = 1 adds the random words of one sweep (4 bytes per position).
-__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax, int pre_rnd) {
+// pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
+// lat_words > 0 two lattice-order bit sets for cluster growth, and with
+// lat_stage = 1 also a copy of the lattice tables (wpos, obit).
+__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax, int pre_rnd,
+                                                  int lat_words, int lat_stage) {
     CoopLayout L;
     int o = 0;
     auto take = [&](int bytes) {
@@ -168,6 +172,8 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     const int nw = (n + 31) / 32;
     L.sp = take(spin_bytes(nw));
     L.icm = take(icm_scratch_bytes(n));
+    L.lat = take(8 * lat_words);
+    L.lat_tab = take(lat_stage && lat_words > 0 ? 4 * (16 * lat_words + n) : 0);
     L.acc = take(4 * 2 * 2 * kMaxCoopWaves);
     L.flag = take(4);
     L.rnd = take(pre_rnd ? 16 * ((n + 3) / 4) : 0);
@@ -183,7 +189,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const InstDesc d = a.desc[c.inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
     const int r = blockIdx.x;  // this workgroup's replica
-    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd);
+    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -209,6 +215,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
     sc.red = nullptr;  // one wave grows the cluster: no workgroup sums
+    sc.Do = reinterpret_cast<uint32_t*>(smem + L.lat);
+    sc.Vo = sc.Do + d.lat_words;
 
     // ---- stage the instance and this replica ----
     for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
@@ -218,6 +226,14 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         for (int q = tid; q < (int)d.adj_len16; q += nthr) dst[q] = src[q];
     }
     for (int m = tid; m < nt * d.fmax; m += nthr) thr[m] = a.thr[d.thr_off + m];
+    LatView lv = lat_view(a, d);
+    if (c.lat_stage && d.lat_dim > 0) {  // the tables of this lattice shape, read by every cluster move
+        uint32_t* tab = reinterpret_cast<uint32_t*>(smem + L.lat_tab);
+        const uint32_t* src = a.lat + d.lat_off;
+        for (int m = tid; m < 16 * d.lat_words + n; m += nthr) tab[m] = src[m];
+        lv.wpos = reinterpret_cast<const uint16_t*>(tab);
+        lv.obit = tab + 16 * d.lat_words;
+    }
     const int64_t ro = d.rep_off;
     for (int s = tid; s < nrep; s += nthr) {
         beta[s] = a.beta[d.beta_off + s];
@@ -338,7 +354,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                                          d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                          c.pub_spins + ((int64_t)par * nrep + r1) * nw,
                                          c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, kWave,
-                                         false);
+                                         false, lv);
                     if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
                 }
                 __syncthreads();

@@ -88,6 +88,13 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     std::vector<uint32_t> rowinfo, thr;
     std::vector<uint32_t> adjw;  // padded rows as 32-bit words (both formats)
     std::vector<double> beta;
+    std::vector<uint32_t> lat;
+    struct LatShape {
+        int32_t dim, L;
+        const Problem* prob;
+        int64_t off;
+    };
+    std::vector<LatShape> lat_shapes;
     int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0;
     const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
     int64_t t_off = 0;
@@ -175,6 +182,41 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         rep_off += d.nrep;
         d.spin_off = spin_off;
         spin_off += (int64_t)d.nrep * d.nw;
+        // lattice tables, shared by every instance of one lattice shape
+        // (the colouring, and so the position order, depends on the shape only)
+        d.lat_dim = d.lat_L = d.lat_R = d.lat_words = 0;
+        d.lat_off = 0;
+        if (nc > 0 && P.lat_dim > 0) {
+            const int32_t L = P.lat_L, R = (L + 31) / 32;
+            int32_t rows = 1;
+            for (int32_t k = 1; k < P.lat_dim; ++k) rows *= L;
+            int64_t off = -1;
+            for (const auto& t : lat_shapes)
+                if (t.dim == P.lat_dim && t.L == L && t.prob->pos2orig == P.pos2orig) off = t.off;
+            if (off < 0) {
+                off = (int64_t)lat.size();
+                const int32_t words = R * rows;
+                std::vector<uint16_t> wpos((size_t)words * 32, 0);
+                for (int32_t w = 0; w < words; ++w) {
+                    const int32_t row = w / R, k = w % R;
+                    for (int32_t b = 0; b < 32 && 32 * k + b < L; ++b)
+                        wpos[(size_t)w * 32 + b] = (uint16_t)P.orig2pos[row * L + 32 * k + b];
+                }
+                for (size_t q = 0; q < wpos.size(); q += 2) lat.push_back((uint32_t)wpos[q] | ((uint32_t)wpos[q + 1] << 16));
+                for (int32_t p = 0; p < P.n; ++p) {
+                    const int32_t o = P.pos2orig[p], x = o % L, row = o / L;
+                    lat.push_back((uint32_t)((row * R + x / 32) << 5) | (uint32_t)(x % 32));
+                }
+                while (lat.size() % 4) lat.push_back(0u);  // 16-byte aligned tables
+                lat_shapes.push_back({P.lat_dim, L, problems[i].get(), off});
+            }
+            d.lat_dim = P.lat_dim;
+            d.lat_L = L;
+            d.lat_R = R;
+            d.lat_words = R * rows;
+            d.lat_off = off;
+            lat_words_max = std::max(lat_words_max, d.lat_words);
+        }
         d.e_base = (int32_t)(-P.abs_sum);
         d.hist_bins = hist ? (int32_t)(P.abs_sum + 1) : 0;
         d.hist_off = hist_off;
@@ -216,10 +258,17 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         PTX_TRY(hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
         PTX_TRY(hip_check(hipDeviceGetAttribute(&coopok, hipDeviceAttributeCooperativeLaunch, dev), "coop attr"));
         // with the next sweep's random words in LDS when they fit, else without
-        const CoopLayout L1 = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax, 1);
-        const CoopLayout L0 = coop_layout(desc[0].n, (int)desc[0].adj_len16, desc[0].nrep, desc[0].nt, desc[0].fmax, 0);
-        coop_pre_rnd = L1.total <= 160 * 1024 ? 1 : 0;
-        const CoopLayout& L = coop_pre_rnd ? L1 : L0;
+        // with the lattice tables and the next sweep's random words in LDS
+        // when they fit, else without (tables first to go)
+        const InstDesc& d0 = desc[0];
+        auto lay = [&](int pre, int stage) {
+            return coop_layout(d0.n, (int)d0.adj_len16, d0.nrep, d0.nt, d0.fmax, pre, d0.lat_words, stage);
+        };
+        coop_pre_rnd = 1;
+        coop_lat_stage = 1;
+        if (lay(1, 1).total > 160 * 1024) coop_lat_stage = 0;
+        if (lay(1, 0).total > 160 * 1024) coop_pre_rnd = 0;
+        const CoopLayout L = lay(coop_pre_rnd, coop_lat_stage);
         const bool fits = coopok && L.total <= 160 * 1024 && desc[0].nrep <= ncu;
         if (fits) {
             mode = PTX_MODE_COOP;
@@ -294,6 +343,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_cls.upload(cls, stream));
     PTX_TRY(d_h.upload(hv, stream));
     PTX_TRY(d_p2o.upload(p2o, stream));
+    if (lat.empty()) lat.assign(4, 0u);
+    PTX_TRY(d_lat.upload(lat, stream));
     PTX_TRY(d_thr.upload(thr, stream));
     PTX_TRY(d_beta.upload(beta, stream));
     PTX_TRY(build_lists());
@@ -340,6 +391,7 @@ static DevArgs build_args(Engine& e) {
     a.items = e.d_items.p;
     a.pairs = e.d_pairs.p;
     a.order = e.d_order.p;
+    a.lat = e.d_lat.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.inst_step = e.d_inst_step.p;
     a.split_step = e.d_step.p;
@@ -426,10 +478,11 @@ Status Engine::launch_step() {
     }
     if (!pairs.empty()) {
         const int wi = icm_waves_now();
-        const size_t lds = 8 * kIcmRedSlots + (size_t)(nw_max + 1) * 16 + 16 + ((size_t)n_max / 2 + 2) * 2 + 16;
+        const size_t lds = 8 * kIcmRedSlots + (size_t)(nw_max + 1) * 16 + 16 + (32 * (size_t)nw_max + 16) +
+                           8 * (size_t)lat_words_max;
         const dim3 grid((unsigned)pairs.size()), block((unsigned)(wi * kWave));
         PTX_TRY(timed(2, [&] {
-            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, nw_max); };
+            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, nw_max, lat_words_max); };
             if (fmt == kFmtUnit) {
                 switch (wi) {
                     case 1: go(k_icm<kFmtUnit, 1>); break;
@@ -512,6 +565,7 @@ Status Engine::launch_coop(int nsteps) {
     c.nblocks = desc[0].nrep;
     c.inst = 0;
     c.pre_rnd = coop_pre_rnd;
+    c.lat_stage = coop_lat_stage;
     const dim3 grid((unsigned)desc[0].nrep), block((unsigned)(Wf * kWave));
     const unsigned lds = (unsigned)coop_lds;
     auto go = [&](auto kern) -> Status {

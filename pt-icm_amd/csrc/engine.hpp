@@ -61,6 +61,7 @@ struct Engine {
     int32_t fused_lds = 0;       // bytes of dynamic LDS per fused workgroup
     int32_t fused_global = 0;    // fused mode reads the adjacency from global memory
     int32_t nrep_max = 0, nw_max = 0, n_max = 0, fmax_max = 1;
+    int32_t lat_words_max = 0;   // largest lattice-order bit set (words)
     int32_t spin_stride = 0, thr_stride = 1;
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;
     std::vector<int2> items, pairs;
@@ -80,6 +81,7 @@ struct Engine {
 
     DevBuf<InstDesc> d_desc;
     DevBuf<int32_t> d_cls, d_h, d_p2o;
+    DevBuf<uint32_t> d_lat;      // lattice tables (wpos, obit) per lattice shape
     DevBuf<uint32_t> d_rowinfo, d_adj, d_thr;
     DevBuf<double> d_beta;
     DevBuf<uint32_t> d_spins, d_best_spins;
@@ -98,6 +100,7 @@ struct Engine {
     unsigned long long barriers_done = 0;
     int32_t spl = 1;             // positions per lane (cooperative mode)
     int32_t coop_pre_rnd = 0;    // cooperative mode: next sweep's random words drawn during the grid barrier
+    int32_t coop_lat_stage = 0;  // cooperative mode: lattice tables copied into LDS
     int32_t coop_lds = 0;
     bool phase_on = false;
     // fused mode with use_graph = G > 0: G launches of K steps captured once

@@ -168,10 +168,12 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, int nrep_max) {
 
 // ---------------------------------------------------------------------------
 // k_icm: one workgroup of W waves per (instance, slot t) pair, t in [nt-nc, nt).
-// LDS: W reduction slots, then the ICM scratch of the largest instance.
+// LDS: the reduction slots, the ICM scratch of the largest instance (S1, S2,
+// D, V, tail, a queue of 32 * nw_max + 16 bytes) and two lattice-order bit
+// sets.  One wave grows lattice clusters as bit sets.
 // ---------------------------------------------------------------------------
 template <int Fmt, int W>
-__global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max) {
+__global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat_words_max) {
     const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x;
@@ -187,6 +189,8 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max) {
     sc.V = sc.D + (nw_max + 1);
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw_max + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
+    sc.Do = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(sc.Q) + 32 * nw_max + 16);
+    sc.Vo = sc.Do + lat_words_max;
     const int64_t ro = d.rep_off;
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
     uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * d.nw;
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max) {
     long long dE1 = 0, dE2 = 0;
     icm_team<Fmt, (W > 1)>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, a.hfield + d.pos_off,
                            d.has_field != 0, d.n, d.nw, t, step, a.seed, ctr3(kTagIcm | a.tag_base, d.inst_id), g1,
-                           g2, sc, dE1, dE2, tid, W * kWave);
+                           g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{});
     if (tid == 0) {
         a.E[ro + r1] += (int32_t)dE1;
         a.E[ro + r2] += (int32_t)dE2;
@@ -320,6 +324,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
     sc.red = nullptr;  // one wave per pair: no workgroup sums
+    sc.Do = sc.Vo = nullptr;  // queue growth only
 
     const bool prof = a.phase != nullptr && blockIdx.x == 0 && tid == 0;
     unsigned long long t_mark = prof ? wall_clock64() : 0ull;
@@ -479,9 +484,9 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint
     template __global__ void k_sweep<F, 2>(DevArgs);              \
     template __global__ void k_sweep<F, 4>(DevArgs);              \
     template __global__ void k_sweep<F, 8>(DevArgs);              \
-    template __global__ void k_icm<F, 1>(DevArgs, int);           \
-    template __global__ void k_icm<F, 2>(DevArgs, int);           \
-    template __global__ void k_icm<F, 4>(DevArgs, int);           \
+    template __global__ void k_icm<F, 1>(DevArgs, int, int);      \
+    template __global__ void k_icm<F, 2>(DevArgs, int, int);      \
+    template __global__ void k_icm<F, 4>(DevArgs, int, int);      \
     template __global__ void k_fused<F, 4, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 8, 0>(DevArgs, int);      \
     template __global__ void k_fused<F, 16, 0>(DevArgs, int);     \

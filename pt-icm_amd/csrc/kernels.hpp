@@ -50,6 +50,7 @@ struct DevArgs {
     const int2* items;       // (instance, first replica) per sweep workgroup
     const int2* pairs;       // (instance, slot t) per ICM wavefront
     const int32_t* order;    // fused mode: instance per workgroup
+    const uint32_t* lat;     // lattice tables (LatView), one per lattice shape
     int32_t fmt;             // adjacency record format
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
     uint64_t seed;
@@ -670,7 +671,228 @@ struct IcmScratch {
     int32_t* tail;
     uint16_t* Q;
     long long* red;  // Block: one slot per wave for workgroup sums
+    uint32_t* Do;    // lattice growth: D and the cluster in lattice order (LatView::words each)
+    uint32_t* Vo;
 };
+
+// A periodic L^dim lattice (dim 2 or 3) whose sites i = x + L*y [+ L*L*z]
+// are numbered like its rows: bit sets in lattice order keep each row of L
+// sites in R = ceil(L/32) words, and a cluster grows by whole words --
+// shifts along x, word moves along y and z.  wpos[32*w + b] is the position
+// of bit b of word w (padding bits name position 0 and are masked), obit[p]
+// the word << 5 | bit of position p.  dim = 0: no lattice (queue growth).
+struct LatView {
+    const uint16_t* wpos;
+    const uint32_t* obit;
+    int dim, L, R, words;
+};
+
+// lattice words per lane kept in registers during cluster growth
+constexpr int kLatSlots = 4;
+
+__device__ __forceinline__ LatView lat_view(const DevArgs& a, const InstDesc& d) {
+    if (d.lat_dim == 0) return LatView{nullptr, nullptr, 0, 0, 0, 0};
+    const uint32_t* t = a.lat + d.lat_off;  // wpos (16-bit) then obit
+    return LatView{reinterpret_cast<const uint16_t*>(t), t + 16 * (int64_t)d.lat_words, d.lat_dim, d.lat_L, d.lat_R,
+                   d.lat_words};
+}
+
+// Grows the cluster of seed_pos (a position in D) as a bit set in lattice
+// order until no word changes, then writes it to V (position order) and its
+// positions to Q; returns the cluster size.  One wavefront.  Updates are
+// in place and only ever add sites reached from the seed through D, so the
+// order of the word updates does not change the fixed point: the connected
+// component of the seed in D.
+__device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, int seed_pos, IcmScratch sc,
+                                            int lane) {
+    const int L = lv.L, R = lv.R, NW = lv.words;
+    const int lb = L - 32 * (R - 1);  // sites in a row's last word (1..32)
+    const uint32_t lastmask = lb == 32 ? 0xFFFFFFFFu : ((1u << lb) - 1u);
+    uint32_t* Do = sc.Do;
+    uint32_t* Vo = sc.Vo;
+    const uint32_t* D = sc.D;
+    // D in lattice order: the 32 positions of each word are read as four
+    // 16-byte loads, then their D bits
+    for (int w = lane; w < NW; w += kWave) {
+        const uint4* src = reinterpret_cast<const uint4*>(lv.wpos + 32 * w);
+        uint4 c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) c[q] = src[q];
+        uint32_t word = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t pv[4] = {c[q].x, c[q].y, c[q].z, c[q].w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const uint32_t p0 = pv[h] & 0xFFFFu, p1 = pv[h] >> 16;
+                word |= ((D[p0 >> 5] >> (p0 & 31)) & 1u) << (8 * q + 2 * h);
+                word |= ((D[p1 >> 5] >> (p1 & 31)) & 1u) << (8 * q + 2 * h + 1);
+            }
+        }
+        if (w % R == R - 1) word &= lastmask;
+        Do[w] = word;
+        Vo[w] = 0u;
+    }
+    wave_sync();
+    if (lane == 0) {
+        const uint32_t ob = lv.obit[seed_pos];
+        Vo[ob >> 5] |= 1u << (ob & 31);
+    }
+    wave_sync();
+    const int rows = NW / R;
+    // neighbour rows of row r along y and z (wrapped within the lattice)
+    auto rows_of = [&](int row, int& yu, int& yd, int& zu, int& zd) {
+        const int y = row % L, zb = row - y;
+        yu = zb + (y + 1 == L ? 0 : y + 1);
+        yd = zb + (y == 0 ? L - 1 : y - 1);
+        zu = zd = row;
+        if (lv.dim == 3) {
+            const int z = row / L;
+            zu = y + L * (z + 1 == L ? 0 : z + 1);
+            zd = y + L * (z == 0 ? L - 1 : z - 1);
+        }
+    };
+    // x closure of a one-word row: bits reached along x through d (wrapped)
+    auto close_x = [&](uint32_t nv, uint32_t d) {
+        while (true) {
+            const uint32_t t = nv | (((nv << 1) | (nv >> (L - 1)) | (nv >> 1) | (nv << (L - 1))) & d);
+            if (t == nv) return nv;
+            nv = t;
+        }
+    };
+    if (R == 1 && rows <= kWave) {
+        // one row per lane, all in registers: y and z neighbours by lane shuffles
+        const bool own = lane < rows;
+        const uint32_t d = own ? Do[lane] : 0u;
+        uint32_t v = own ? Vo[lane] : 0u;
+        int yu = lane, yd = lane, zu = lane, zd = lane;
+        if (own) rows_of(lane, yu, yd, zu, zd);
+        while (true) {
+            uint32_t ny = __shfl(v, yu, kWave) | __shfl(v, yd, kWave);
+            if (lv.dim == 3) ny |= __shfl(v, zu, kWave) | __shfl(v, zd, kWave);
+            const uint32_t nv = close_x(v | (ny & d), d);
+            const bool grew = nv != v;
+            v = nv;
+            if (__ballot(grew) == 0ull) break;
+        }
+        if (own) Vo[lane] = v;
+        wave_sync();
+    } else if (NW <= kLatSlots * kWave) {
+        // up to kLatSlots words per lane kept in registers; each round reads every
+        // neighbour word first, then writes the grown words
+        const int J = (NW + kWave - 1) / kWave;
+        uint32_t dreg[kLatSlots], vreg[kLatSlots];
+#pragma unroll
+        for (int j = 0; j < kLatSlots; ++j) {
+            const int w = lane + kWave * j;
+            const bool own = j < J && w < NW;
+            dreg[j] = own ? Do[w] : 0u;
+            vreg[j] = own ? Vo[w] : 0u;
+        }
+        while (true) {
+            uint32_t nb[kLatSlots];
+#pragma unroll
+            for (int j = 0; j < kLatSlots; ++j) {
+                nb[j] = 0u;
+                const int w = lane + kWave * j;
+                if (j >= J || w >= NW || vreg[j] == dreg[j]) continue;
+                const int row = w / R, k = w - row * R;
+                int yu, yd, zu, zd;
+                rows_of(row, yu, yd, zu, zd);
+                uint32_t n = Vo[yu * R + k] | Vo[yd * R + k];
+                if (lv.dim == 3) n |= Vo[zu * R + k] | Vo[zd * R + k];
+                const uint32_t v = vreg[j];
+                if (R == 1) {
+                    n |= ((v << 1) | (v >> (L - 1)) | (v >> 1) | (v << (L - 1))) & lastmask;
+                } else {
+                    const int rw = row * R;
+                    const uint32_t cin = k > 0 ? (Vo[w - 1] >> 31) : ((Vo[rw + R - 1] >> (lb - 1)) & 1u);
+                    const uint32_t cout = k < R - 1 ? ((Vo[w + 1] & 1u) << 31) : ((Vo[rw] & 1u) << (lb - 1));
+                    uint32_t nx = (v << 1) | cin | (v >> 1) | cout;
+                    if (k == R - 1) nx &= lastmask;
+                    n |= nx;
+                }
+                nb[j] = n;
+            }
+            wave_sync();  // every read of this round before any write
+            bool grew = false;
+#pragma unroll
+            for (int j = 0; j < kLatSlots; ++j) {
+                const int w = lane + kWave * j;
+                if (j >= J || w >= NW) continue;
+                uint32_t nv = vreg[j] | (nb[j] & dreg[j]);
+                if (R == 1) nv = close_x(nv, dreg[j]);
+                if (nv != vreg[j]) {
+                    vreg[j] = nv;
+                    Vo[w] = nv;
+                    grew = true;
+                }
+            }
+            wave_sync();
+            if (__ballot(grew) == 0ull) break;
+        }
+    } else {
+        // larger lattices: words updated in place (monotone, any order)
+        while (true) {
+            bool grew = false;
+            for (int w = lane; w < NW; w += kWave) {
+                const uint32_t d = Do[w];
+                const uint32_t v = Vo[w];
+                if (v == d) continue;  // every site of D in this word is in already
+                const int row = w / R, k = w - row * R;
+                uint32_t nx;
+                if (R == 1) {
+                    nx = ((v << 1) | (v >> (L - 1)) | (v >> 1) | (v << (L - 1))) & lastmask;
+                } else {
+                    const int rw = row * R;
+                    const uint32_t cin = k > 0 ? (Vo[w - 1] >> 31) : ((Vo[rw + R - 1] >> (lb - 1)) & 1u);
+                    const uint32_t cout = k < R - 1 ? ((Vo[w + 1] & 1u) << 31) : ((Vo[rw] & 1u) << (lb - 1));
+                    nx = (v << 1) | cin | (v >> 1) | cout;
+                    if (k == R - 1) nx &= lastmask;
+                }
+                int yu, yd, zu, zd;
+                rows_of(row, yu, yd, zu, zd);
+                uint32_t ny = Vo[yu * R + k] | Vo[yd * R + k];
+                if (lv.dim == 3) ny |= Vo[zu * R + k] | Vo[zd * R + k];
+                uint32_t nv = v | ((nx | ny) & d);
+                if (R == 1) nv = close_x(nv, d);
+                if (nv != v) {
+                    Vo[w] = nv;
+                    grew = true;
+                }
+            }
+            wave_sync();
+            if (__ballot(grew) == 0ull) break;
+        }
+    }
+    // back to position order: V, and the cluster's positions in Q (the
+    // table reads of four 64-position blocks are issued together)
+    int csize = 0;
+    for (int p00 = 0; p00 < 32 * nw; p00 += 4 * kWave) {
+        uint32_t ob[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p00 + u * kWave + lane;
+            ob[u] = p < n ? lv.obit[p] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p0 = p00 + u * kWave;
+            if (p0 >= 32 * nw) break;
+            const int p = p0 + lane;
+            const bool in = p < n && ((Vo[ob[u] >> 5] >> (ob[u] & 31)) & 1u) != 0u;
+            const unsigned long long b = __ballot(in);
+            const int below =
+                (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+            if (in) sc.Q[csize + below] = (uint16_t)p;
+            if (lane == 0) sc.V[p0 >> 5] = (uint32_t)b;
+            if (lane == 32 && (p0 >> 5) + 1 < nw) sc.V[(p0 >> 5) + 1] = (uint32_t)(b >> 32);
+            csize += __popcll(b);
+        }
+    }
+    wave_sync();
+    return csize;
+}
 
 // Per-wave slots of the workgroup reductions in the ICM team code.
 constexpr int kIcmRedSlots = 16;
@@ -712,7 +934,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                                          int nw, int t, uint32_t step, uint64_t seed, uint32_t c3,
                                          const uint32_t* g1_in, const uint32_t* g2_in, IcmScratch sc,
                                          long long& dE1_out, long long& dE2_out, int tid, int nthr,
-                                         bool write_back = true) {
+                                         bool write_back = true, LatView lv = LatView{nullptr, nullptr, 0, 0, 0, 0}) {
     // the new states are left in sc.S1/sc.S2 and, with write_back, stored to g1/g2
     uint32_t* g1 = const_cast<uint32_t*>(g1_in);
     uint32_t* g2 = const_cast<uint32_t*>(g2_in);
@@ -752,11 +974,12 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
         team_sync<Block>();
     }
     if (cnt > 0) {
+        int seed_pos = -1;
         if (tid < kWave) {  // the first wave picks the seed
             const Philox4 x = philox4x32_10((uint32_t)t >> 2, 0u, step, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
             const int k = (int)(((unsigned long long)pick(x, (uint32_t)t) * (unsigned long long)cnt) >> 32);
             // k-th element of D in position order: per-64-word chunks, inclusive scan
-            int seed_pos = -1, base = 0;
+            int base = 0;
             for (int w0 = 0; w0 < nw && seed_pos < 0; w0 += kWave) {
                 const int w = w0 + lane;
                 const uint32_t dw = w < nw ? D[w] : 0u;
@@ -795,95 +1018,100 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                 }
                 base += total;
             }
-            if (lane == 0) {
+            if (lane == 0 && (Block || lv.dim == 0)) {
                 V[seed_pos >> 5] |= 1u << (seed_pos & 31);
                 sc.Q[0] = (uint16_t)seed_pos;
                 if (Block) sc.tail[0] = 1;
             }
         }
         team_sync<Block>();
-        // Breadth-first growth.  Each lane takes one queued site and walks its
-        // row a 16-byte chunk at a time: the D and V words of the chunk's
-        // neighbours are read together, candidates are claimed with atomicOr
-        // on V, and the newly claimed sites are appended at wave-scanned
-        // offsets.  One wave keeps the queue tail in a register; a workgroup
-        // keeps it in LDS (one atomic add per wave and chunk) and reads it
-        // back between two barriers at the end of each round.
         constexpr int K = Csr<Fmt>::kPerChunk;
-        int head = 0, tail = 1;
-        while (head < tail) {
-            const int tl = tail;
-            const int idx = head + tid;
-            const bool act = idx < tl;
-            const int p = act ? (int)sc.Q[idx] : 0;
-            const int dg = act ? csr.deg(p) : 0;
-            const uint8_t* row = csr.row(p);
-            const int nch = (dg + K - 1) / K;
-            for (int ch = 0; ch < maxch; ++ch) {
-                const int live = ch < nch ? min(K, dg - ch * K) : 0;
-                // Every load and claim below is issued unconditionally (the
-                // chunk index is clamped to the row, padding records name the
-                // in-range sentinel word) so that they overlap; liveness only
-                // masks the results.
-                int col[K], wgt[K];
-                csr.chunk(row, min(ch, max(nch - 1, 0)), col, wgt);
-                uint32_t dwd[K], vwd[K];
-#pragma unroll
-                for (int h = 0; h < K; ++h) {
-                    dwd[h] = D[col[h] >> 5];
-                    vwd[h] = V[col[h] >> 5];
+        int csize;
+        if (!Block && lv.dim > 0) {
+            csize = lattice_grow(lv, n, nw, seed_pos, sc, lane);
+        } else {
+            // Breadth-first growth.  Each lane takes one queued site and walks its
+            // row a 16-byte chunk at a time: the D and V words of the chunk's
+            // neighbours are read together, candidates are claimed with atomicOr
+            // on V, and the newly claimed sites are appended at wave-scanned
+            // offsets.  One wave keeps the queue tail in a register; a workgroup
+            // keeps it in LDS (one atomic add per wave and chunk) and reads it
+            // back between two barriers at the end of each round.
+            int head = 0, tail = 1;
+            while (head < tail) {
+                const int tl = tail;
+                const int idx = head + tid;
+                const bool act = idx < tl;
+                const int p = act ? (int)sc.Q[idx] : 0;
+                const int dg = act ? csr.deg(p) : 0;
+                const uint8_t* row = csr.row(p);
+                const int nch = (dg + K - 1) / K;
+                for (int ch = 0; ch < maxch; ++ch) {
+                    const int live = ch < nch ? min(K, dg - ch * K) : 0;
+                    // Every load and claim below is issued unconditionally (the
+                    // chunk index is clamped to the row, padding records name the
+                    // in-range sentinel word) so that they overlap; liveness only
+                    // masks the results.
+                    int col[K], wgt[K];
+                    csr.chunk(row, min(ch, max(nch - 1, 0)), col, wgt);
+                    uint32_t dwd[K], vwd[K];
+    #pragma unroll
+                    for (int h = 0; h < K; ++h) {
+                        dwd[h] = D[col[h] >> 5];
+                        vwd[h] = V[col[h] >> 5];
+                    }
+                    uint32_t cand = 0;
+    #pragma unroll
+                    for (int h = 0; h < K; ++h) {
+                        const uint32_t bit = 1u << (col[h] & 31);
+                        const bool c = (h < live) & ((dwd[h] & bit) != 0u) & ((vwd[h] & bit) == 0u);
+                        cand |= (c ? 1u : 0u) << h;
+                    }
+                    // only candidates claim (padding records share one sentinel word,
+                    // and same-address atomics serialise); the claims carry no
+                    // dependence on each other, so they still issue back to back
+                    uint32_t old[K];
+    #pragma unroll
+                    for (int h = 0; h < K; ++h) {
+                        old[h] = 0xFFFFFFFFu;
+                        if ((cand >> h) & 1u) old[h] = atomicOr(&V[col[h] >> 5], 1u << (col[h] & 31));
+                    }
+                    uint32_t claimed = 0;
+    #pragma unroll
+                    for (int h = 0; h < K; ++h)
+                        if (((cand >> h) & 1u) && !(old[h] & (1u << (col[h] & 31)))) claimed |= 1u << h;
+                    // claims of the lanes below this one (and of the whole wave)
+                    // from one ballot per record slot: no cross-lane data moves
+                    int excl = 0, total = 0;
+    #pragma unroll
+                    for (int h = 0; h < K; ++h) {
+                        const unsigned long long b = __ballot((claimed >> h) & 1u);
+                        excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
+                        total += __popcll(b);
+                    }
+                    int pos;
+                    if (Block) {
+                        int wbase = 0;
+                        if (lane == 0 && total > 0) wbase = atomicAdd(sc.tail, total);
+                        pos = __shfl(wbase, 0, kWave) + excl;
+                    } else {
+                        pos = tail + excl;
+                        tail += total;
+                    }
+    #pragma unroll
+                    for (int h = 0; h < K; ++h)
+                        if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
                 }
-                uint32_t cand = 0;
-#pragma unroll
-                for (int h = 0; h < K; ++h) {
-                    const uint32_t bit = 1u << (col[h] & 31);
-                    const bool c = (h < live) & ((dwd[h] & bit) != 0u) & ((vwd[h] & bit) == 0u);
-                    cand |= (c ? 1u : 0u) << h;
-                }
-                // only candidates claim (padding records share one sentinel word,
-                // and same-address atomics serialise); the claims carry no
-                // dependence on each other, so they still issue back to back
-                uint32_t old[K];
-#pragma unroll
-                for (int h = 0; h < K; ++h) {
-                    old[h] = 0xFFFFFFFFu;
-                    if ((cand >> h) & 1u) old[h] = atomicOr(&V[col[h] >> 5], 1u << (col[h] & 31));
-                }
-                uint32_t claimed = 0;
-#pragma unroll
-                for (int h = 0; h < K; ++h)
-                    if (((cand >> h) & 1u) && !(old[h] & (1u << (col[h] & 31)))) claimed |= 1u << h;
-                // claims of the lanes below this one (and of the whole wave)
-                // from one ballot per record slot: no cross-lane data moves
-                int excl = 0, total = 0;
-#pragma unroll
-                for (int h = 0; h < K; ++h) {
-                    const unsigned long long b = __ballot((claimed >> h) & 1u);
-                    excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-                    total += __popcll(b);
-                }
-                int pos;
+                head = (tl - head > nthr) ? head + nthr : tl;
+                team_sync<Block>();
                 if (Block) {
-                    int wbase = 0;
-                    if (lane == 0 && total > 0) wbase = atomicAdd(sc.tail, total);
-                    pos = __shfl(wbase, 0, kWave) + excl;
-                } else {
-                    pos = tail + excl;
-                    tail += total;
+                    tail = sc.tail[0];
+                    __syncthreads();  // every wave has the tail before the next round appends
                 }
-#pragma unroll
-                for (int h = 0; h < K; ++h)
-                    if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
             }
-            head = (tl - head > nthr) ? head + nthr : tl;
-            team_sync<Block>();
-            if (Block) {
-                tail = sc.tail[0];
-                __syncthreads();  // every wave has the tail before the next round appends
-            }
+            csize = tail;
         }
-        const int csize = tail;
         // energy change of flipping the cluster C in each replica: boundary
         // edges (p in C, j not in C) and the field terms of C
         for (int idx = tid; idx < csize; idx += nthr) {

@@ -134,7 +134,50 @@ Status Problem::finalize() {
     abs_sum += wsum2 / 2;
     if (abs_sum > (1ll << 30)) return Status::err(PTX_ERR_UNSUPPORTED, "energy range exceeds 32-bit engine energies");
     if (fmax > 8192) return Status::err(PTX_ERR_UNSUPPORTED, "local field bound above 8192 (threshold table too large)");
+    detect_lattice();
     return Status::ok();
+}
+
+void Problem::detect_lattice() {
+    lat_dim = 0;
+    lat_L = 0;
+    for (int32_t D = 2; D <= 3; ++D) {
+        int32_t L = (int32_t)std::lround(std::pow((double)n, 1.0 / D));
+        int64_t nn = 1;
+        for (int32_t d = 0; d < D; ++d) nn *= L;
+        if (L < 3 || nn != n) continue;
+        bool ok = true;
+        for (int32_t i = 0; i < n && ok; ++i) {
+            if (rowptr[i + 1] - rowptr[i] != 2 * D) {
+                ok = false;
+                break;
+            }
+            // the wrapped neighbours of i along each axis (stride 1, L, L*L)
+            int32_t want[6];
+            int64_t stride = 1;
+            for (int32_t d = 0; d < D; ++d, stride *= L) {
+                const int32_t c = (int32_t)((i / stride) % L);
+                const int32_t base = i - (int32_t)(c * stride);
+                want[2 * d] = base + (int32_t)(((c + 1) % L) * stride);
+                want[2 * d + 1] = base + (int32_t)(((c + L - 1) % L) * stride);
+            }
+            for (int32_t q = rowptr[i]; q < rowptr[i + 1] && ok; ++q) {
+                int32_t hits = 0;
+                for (int32_t k = 0; k < 2 * D; ++k) hits += want[k] == col[q];
+                ok = hits == 1;
+            }
+            // 2D entries, each naming exactly one wanted neighbour, and the
+            // wanted neighbours distinct (L >= 3): the row is that set
+            for (int32_t q = rowptr[i]; q < rowptr[i + 1] && ok; ++q)
+                for (int32_t q2 = q + 1; q2 < rowptr[i + 1]; ++q2)
+                    if (col[q] == col[q2]) ok = false;
+        }
+        if (ok) {
+            lat_dim = D;
+            lat_L = L;
+            return;
+        }
+    }
 }
 
 int64_t Problem::energy(const uint8_t* s) const {

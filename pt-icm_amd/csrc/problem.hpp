@@ -32,12 +32,17 @@ struct Problem {
     int32_t max_deg = 0;
     int64_t abs_sum = 0;  // sum over edges |w| + sum |h|
     bool has_field = false;
+    // periodic hypercubic lattice in index order (i = x + L*y [+ L*L*z]),
+    // every row exactly the 2*lat_dim wrapped axis neighbours: cluster moves
+    // may then grow clusters as bit sets (lat_dim 0: not such a lattice)
+    int32_t lat_dim = 0, lat_L = 0;
 
     int64_t nnz() const { return rowptr.empty() ? 0 : rowptr[n]; }
     int64_t energy(const uint8_t* s) const;
     int64_t maxcut(const uint8_t* s) const;
     int64_t delta_e(const uint8_t* s, int64_t* dE) const;
-    Status finalize();  // colouring, positions, bounds, limits
+    Status finalize();  // colouring, positions, bounds, limits, lattice detection
+    void detect_lattice();
 };
 
 // src/problem.cc:183-247

@@ -31,7 +31,8 @@ class PtxError(RuntimeError):
 class ProblemInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int64), ("nnz", C.c_int64), ("sparse", C.c_int32),
                 ("char_scale", C.c_double), ("n_colours", C.c_int32), ("fmax", C.c_int32),
-                ("e_min_bound", C.c_int64), ("e_max_bound", C.c_int64), ("has_field", C.c_int32)]
+                ("e_min_bound", C.c_int64), ("e_max_bound", C.c_int64), ("has_field", C.c_int32),
+                ("lattice_dim", C.c_int32), ("lattice_L", C.c_int32)]
 
 
 class EngineConfig(C.Structure):

@@ -169,6 +169,34 @@ def test_icm_all_slots_parity(ptx, oracle, mode):
         compare(eng, oe, what=f"step {s}")
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("dim,L", [(2, 33), (2, 40), (2, 64), (3, 5), (3, 12)])
+def test_lattice_cluster_growth_parity(ptx, oracle, dim, L, mode):
+    """Periodic lattices grow clusters as bit sets (split and cooperative
+    kernels; one row per word and rows of several words, odd L with three
+    colour classes, 3D): the oracle's breadth-first clusters bit for bit,
+    with ICM on every slot (global flips included)."""
+    if dim == 2:
+        n, u, v, w = instances.torus_2d(L, seed=L)
+    else:
+        n, u, v, w = instances.ea_3d(L, seed=L)
+    p, g = ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)
+    assert (p.info.lattice_dim, p.info.lattice_L) == (dim, L)
+    nt = 8
+    betas = oracle.beta_linspace(0.4, 3.0, nt, 0)
+    say("lattice", dim, L, "mode", mode)
+    eng = ptx.Engine(p, betas, seed=L, flags=ptx.FLAG_ICM, nc=nt, mode=mode, steps_per_launch=2)
+    assert eng.info().mode == mode
+    oe = oracle.Engine(g, betas, chains=2, seed=L, flags=oracle.FLAG_ICM, nc=nt)
+    for s_ in range(6):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"L={L} dim={dim} step {s_}")
+    eng.step(10)
+    oe.step(10)
+    compare(eng, oe, what=f"L={L} dim={dim} bulk")
+
+
 @pytest.mark.parametrize("kind,nc", [("G039", 8), ("pm_torus20", -1), ("field_random", 4)])
 @pytest.mark.parametrize("icm_waves", [1, 2, 4])
 def test_icm_waves_independence(ptx, oracle, kind, nc, icm_waves):

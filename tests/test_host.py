@@ -4,11 +4,15 @@ error behaviour at the boundary."""
 import math
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
 
 from conftest import REPO, graph_path
+
+sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+import instances  # noqa: E402
 
 GRAPHS = ["lattice_2d_5.gset", "lattice_2d_10.gset", "lattice_2d_20.gset", "G022", "G039"]
 
@@ -205,3 +209,43 @@ def test_engine_without_device_fails_loudly(ptx):
     with pytest.raises(ptx.PtxError) as ei:
         ptx.Engine(p, ptx.beta_linspace(0.5, 3.0, 4))
     assert ei.value.code == -4
+
+
+def _lattice_edges(dim, L, seed=0):
+    idx = np.arange(L ** dim).reshape((L,) * dim)
+    e = np.concatenate([np.stack([idx.ravel(), np.roll(idx, -1, axis=a).ravel()], 1) for a in range(dim)])
+    w = np.where(np.random.default_rng(seed).integers(0, 2, len(e)) == 1, 1.0, -1.0)
+    return L ** dim, e[:, 0].astype(np.int32), e[:, 1].astype(np.int32), w
+
+
+@pytest.mark.parametrize("dim,L", [(2, 3), (2, 5), (2, 20), (2, 33), (2, 64), (3, 3), (3, 6), (3, 11)])
+def test_lattice_detection(ptx, dim, L):
+    """Periodic L^dim lattices numbered like their axes are recognised in any
+    edge order and with zero weights; anything else is not."""
+    n, u, v, w = _lattice_edges(dim, L, seed=L)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(len(u))
+    flip = rng.integers(0, 2, len(u)) == 1
+    uu = np.where(flip, v, u)[perm]
+    vv = np.where(flip, u, v)[perm]
+    ww = w[perm].copy()
+    ww[:3] = 0.0  # zero-weight entries stay edges of a sparse instance
+    p = ptx.Problem.from_coo(n, uu, vv, ww)
+    assert (p.info.lattice_dim, p.info.lattice_L) == (dim, L)
+    # a missing bond, a doubled bond, an extra bond, a relabelled site: no lattice
+    for uu2, vv2 in [(uu[1:], vv[1:]), (np.append(uu, uu[0]), np.append(vv, vv[0])),
+                     (np.append(uu, 0), np.append(vv, n // 2 + 1))]:
+        q = ptx.Problem.from_coo(n, uu2, vv2, np.ones(len(uu2)))
+        assert q.info.lattice_dim == 0
+    relabel = np.arange(n)
+    relabel[[0, 1]] = relabel[[1, 0]]
+    q = ptx.Problem.from_coo(n, relabel[uu], relabel[vv], ww)
+    assert q.info.lattice_dim == 0
+
+
+def test_non_lattices_are_not_detected(ptx):
+    n, u, v, w = instances.erdos_renyi(400, 0.01, seed=2, pm=True)
+    assert ptx.Problem.from_coo(n, u, v, w).info.lattice_dim == 0
+    n, u, v, w = _lattice_edges(2, 2)  # L = 2: doubled bonds
+    assert ptx.Problem.from_coo(n, u, v, w).info.lattice_dim == 0
+    assert ptx.Problem.read(graph_path("G022")).info.lattice_dim == 0
