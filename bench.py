@@ -30,7 +30,7 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
 # scaled by the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1):
 # k_sweep of the split kernels (one wave per replica), and k_fused with 16
 # waves and 16 steps per launch.
-BATCHED_HBM_BYTES = {"k_sweep": (2 * 71281.9 + 41891.6) * 1024,
+BATCHED_HBM_BYTES = {"k_sweep": (2 * 71276.8 + 38424.8) * 1024,
                      "k_fused": (2 * 69305.5 + 475067.6) * 1024}
 
 
