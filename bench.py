@@ -229,12 +229,18 @@ def main():
                                                     "surrogate_accepted", "ladder_s", "run_s", "ms_per_step",
                                                     "spin_updates_per_s", "mean_slots", "mean_icm_slots", "mode",
                                                     "waves_per_group", "autotune_s")},
+            "batched_surrogate_judged_in_run": {
+                k: r["batched_surrogate_judged_in_run"][k]
+                for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
+                          "done_in_validation", "fallback", "ladder_s", "judge_at_s", "fallback_ready_s",
+                          "validate_steps", "wall_s")},
             "batched_multihistogram": {k: r["batched_multihistogram"][k]
                                        for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                                  "ladder_s", "run_s")},
             "plain": {k: pl[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                          "mean_preprocess_s")},
-            "instances_per_hour_ratio": r["instances_per_hour_ratio"]}
+            "instances_per_hour_ratio": r["instances_per_hour_ratio"],
+            "instances_per_hour_ratio_judged_in_run": r["instances_per_hour_ratio_judged_in_run"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
     print(json.dumps(out))

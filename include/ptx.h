@@ -129,7 +129,7 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
 #define PTX_MODE_AUTO 0
 #define PTX_MODE_SPLIT 1 /* sweep / exchange / ICM kernels per step */
 #define PTX_MODE_FUSED 2 /* one workgroup per instance runs whole PT steps */
-#define PTX_MODE_COOP 3  /* one instance: one workgroup per replica, grid barrier per step */
+#define PTX_MODE_COOP 3  /* one workgroup per replica of every instance, all resident, grid barrier per step */
 
 #define PTX_CSR_AUTO 0   /* adjacency in LDS when it fits, else read through L2 */
 #define PTX_CSR_LDS 1
@@ -149,6 +149,9 @@ typedef struct {
     int32_t spins_per_lane;  /* cooperative mode: 1 or 4 positions per lane (0 = auto) */
     int32_t csr_placement;   /* fused mode: PTX_CSR_AUTO, PTX_CSR_LDS or PTX_CSR_GLOBAL */
     int32_t icm_waves;       /* split mode: waves per ICM slot pair, 1, 2 or 4 (0 = 1) */
+    int32_t small_batch_coop;/* split mode: while every replica of the active instances fits one
+                                cooperative launch (for example the last instances of a batch whose
+                                others were retired), run them with the cooperative kernel */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 
@@ -194,15 +197,17 @@ ptx_status ptx_engine_sync(ptx_engine* e);
 typedef struct {
     int32_t n_instances;
     int64_t steps_done;
-    int32_t mode;            /* resolved mode */
+    int32_t mode;            /* resolved mode (after steps: the mode of the latest launches) */
     int32_t waves_per_group; /* resolved */
     int32_t steps_per_launch;
     int64_t total_replicas;
     int64_t spin_updates_per_step; /* sum over replicas of n * sweeps */
     int64_t lds_bytes_per_step;    /* algorithmic LDS bytes of the sweep (4 d_i + 6 per update) */
-    int32_t csr_in_lds;            /* fused mode: 1 if the adjacency is staged in LDS */
+    int32_t csr_in_lds;            /* fused and cooperative modes: 1 if the adjacency is staged in LDS
+                                      (cooperative: in the latest launch) */
     int32_t icm_waves;             /* waves that grow one cluster (split: per pair; fused and
                                       cooperative: 1) */
+    int64_t coop_capacity;         /* workgroups (replicas) one cooperative launch can hold; 0: none */
 } ptx_engine_info;
 ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out);
 

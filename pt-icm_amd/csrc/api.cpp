@@ -237,6 +237,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->spins_per_lane = 0;
     c->csr_placement = PTX_CSR_AUTO;
     c->icm_waves = 0;
+    c->small_batch_coop = 0;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -312,7 +313,7 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
             const double us =
                 std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count() / steps;
             if (trials && cnt < max_trials)
-                trials[cnt] = ptx_tune_trial{sh.mode, e.mode == PTX_MODE_SPLIT ? e.W : e.Wf, sh.spl, sh.csr,
+                trials[cnt] = ptx_tune_trial{sh.mode, e.mode == PTX_MODE_SPLIT ? e.W : (e.mode == PTX_MODE_COOP ? e.coop_W : e.Wf), sh.spl, sh.csr,
                                              e.mode == PTX_MODE_SPLIT ? e.icm_waves_now() : 0, us};
             ++cnt;
             if (!found || us < best_us) {
@@ -347,14 +348,16 @@ ptx_status ptx_engine_info_get(const ptx_engine* e, ptx_engine_info* out) {
     const ptx::Engine& E = e->e;
     out->n_instances = E.ninst;
     out->steps_done = E.steps_done;
-    out->mode = E.mode;
-    out->waves_per_group = E.mode == PTX_MODE_SPLIT ? E.W : E.Wf;
-    out->steps_per_launch = E.mode == PTX_MODE_SPLIT ? 1 : E.K;
+    out->mode = E.last_mode;
+    out->waves_per_group = E.last_mode == PTX_MODE_SPLIT ? E.W : (E.last_mode == PTX_MODE_COOP ? E.coop_W : E.Wf);
+    out->steps_per_launch = E.last_mode == PTX_MODE_SPLIT ? 1 : (E.last_mode == PTX_MODE_COOP ? E.coop_K : E.K);
     out->total_replicas = E.total_rep;
     out->spin_updates_per_step = E.updates_per_step;
     out->lds_bytes_per_step = E.lds_bytes_per_step;
-    out->csr_in_lds = E.mode == PTX_MODE_SPLIT ? 0 : (E.fused_global ? 0 : 1);
-    out->icm_waves = E.mode == PTX_MODE_SPLIT ? E.icm_waves_now() : 1;
+    out->csr_in_lds = E.last_mode == PTX_MODE_COOP ? (E.coop_last_form == 0 ? 1 : 0)
+                      : (E.last_mode == PTX_MODE_SPLIT ? 0 : (E.fused_global ? 0 : 1));
+    out->icm_waves = E.last_mode == PTX_MODE_SPLIT ? E.icm_waves_now() : 1;
+    out->coop_capacity = E.coop_capacity;
     return PTX_OK;
 }
 

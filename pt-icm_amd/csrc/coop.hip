@@ -1,5 +1,6 @@
-// coop.hip -- persistent single-instance kernel: one workgroup per replica,
-// all replicas co-resident (cooperative launch), several PT steps per launch.
+// coop.hip -- persistent kernel for one instance or a small batch: one
+// workgroup per replica of every instance in the launch, all co-resident
+// (cooperative launch), several PT steps per launch.
 //
 // Per PT step each workgroup sweeps its replica with all of its waves, then
 // publishes its spin words and energy to a double-buffered area, then waits
@@ -15,15 +16,17 @@
 namespace ptx {
 
 struct CoopArgs {
-    uint32_t* pub_spins;        // [2][nrep][nw]
-    int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
-    int32_t lat_stage;          // 1: the lattice tables are copied into LDS
-    int32_t* pub_E;             // [2][nrep]
+    uint32_t* pub_spins;        // [2][spin words of all instances], at each instance's spin offset
+    int32_t* pub_E;             // [2][replicas of all instances], at each instance's replica offset
+    const int2* items;          // (instance, replica) of each workgroup
     unsigned long long* bar;    // grid barrier counter (monotonic)
     uint32_t* err;              // set to 1 when a barrier wait times out
-    unsigned long long bar_base;  // barriers completed before this launch
+    unsigned long long bar_base;  // barrier arrivals before this launch
+    int64_t pub_words;          // spin words of all instances (the stride of pub_spins' halves)
+    int32_t pub_reps;           // replicas of all instances (the stride of pub_E's halves)
     int32_t nblocks;
-    int32_t inst;
+    int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
+    int32_t lat_stage;          // 1: the lattice tables are copied into LDS
 };
 
 // Metropolis sweep of one replica by a whole workgroup (W waves).  Lane gl of
@@ -144,9 +147,11 @@ struct CoopLayout {
 
 // pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
 // lat_words > 0 two lattice-order bit sets for cluster growth, and with
-// lat_stage = 1 also a copy of the lattice tables (wpos, obit).
+// lat_stage = 1 also a copy of the lattice tables (wpos, obit); csr_global
+// = 1 leaves the adjacency in global memory (read through L2), so that more
+// workgroups fit a CU.
 __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax, int pre_rnd,
-                                                  int lat_words, int lat_stage) {
+                                                  int lat_words, int lat_stage, int csr_global) {
     CoopLayout L;
     int o = 0;
     auto take = [&](int bytes) {
@@ -154,8 +159,8 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
         o += (bytes + 15) & ~15;
         return at;
     };
-    L.info = take(4 * n);
-    L.adj = take(16 * (adj_len16 > 0 ? adj_len16 : 1));
+    L.info = take(csr_global ? 0 : 4 * n);
+    L.adj = take(csr_global ? 0 : 16 * (adj_len16 > 0 ? adj_len16 : 1));
     L.thr = take(4 * nt * (fmax > 0 ? fmax : 1));
     L.beta = take(8 * nrep);
     L.logu = take(8 * nrep);
@@ -181,15 +186,17 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     return L;
 }
 
-template <int Fmt, int W, int SPL>
+template <int Fmt, int W, int SPL, int G>
 __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int nsteps) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const int nthr = W * kWave;
-    const InstDesc d = a.desc[c.inst];
+    const int2 item = c.items[blockIdx.x];
+    const int inst = item.x;
+    const InstDesc d = a.desc[inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
-    const int r = blockIdx.x;  // this workgroup's replica
-    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage);
+    const int r = item.y;  // this workgroup's replica of its instance
+    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage, G);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -219,8 +226,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     sc.Vo = sc.Do + d.lat_words;
 
     // ---- stage the instance and this replica ----
-    for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
-    {
+    if (!G) {  // G = 1: the adjacency is read through L2
+        for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
         const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
         uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
         for (int q = tid; q < (int)d.adj_len16; q += nthr) dst[q] = src[q];
@@ -263,19 +270,22 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             slot.lu[1] = x.lu[1];
         }
     };
-    if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[c.inst], wave - 1);
+    if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[inst], wave - 1);
     // random words of the first sweep of this launch (later ones are drawn
     // while the grid barrier is pending)
     uint32_t* pre_rnd = c.pre_rnd ? reinterpret_cast<uint32_t*>(smem + L.rnd) : nullptr;
     const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
-    if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[c.inst] * (uint32_t)a.sweeps, tid, nthr);
+    if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[inst] * (uint32_t)a.sweeps, tid, nthr);
     __syncthreads();
-    const Csr<Fmt> csr{info, smem + L.adj};
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16} : Csr<Fmt>{info, smem + L.adj};
     const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
-    const uint32_t step0 = a.inst_step[c.inst];
+    const uint32_t step0 = a.inst_step[inst];
+    // this instance's halves of the published states and energies
+    uint32_t* pub_sp = c.pub_spins + d.spin_off;
+    int32_t* pub_e = c.pub_E + ro;
     const int64_t gr = ro + r;
     uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r * nw;
     SweepState st;
@@ -307,8 +317,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         }
         phase_done(0);
         // ---- publish and wait ----
-        pack_bits(sp, nw, c.pub_spins + ((int64_t)par * nrep + r) * nw, tid, nthr);
-        if (tid == 0) c.pub_E[par * nrep + r] = st.E;
+        pack_bits(sp, nw, pub_sp + par * c.pub_words + (int64_t)r * nw, tid, nthr);
+        if (tid == 0) pub_e[(int64_t)par * c.pub_reps + r] = st.E;
         grid_arrive(c.bar);
         // while thread 0 waits: waves 1 and 2 prepare the next step's
         // exchange, and waves 1.. draw the next step's first-sweep random
@@ -317,10 +327,10 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (pre_rnd && wave > 0 && it + 1 < nsteps)
             fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
                                nthr - kWave);
-        if (!grid_wait(c.bar, c.err, (c.bar_base + (unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
+        if (!grid_wait(c.bar, c.err, c.bar_base + ((unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
                        flag))
             break;
-        for (int s = tid; s < nrep; s += nthr) E[s] = c.pub_E[par * nrep + s];
+        for (int s = tid; s < nrep; s += nthr) E[s] = pub_e[(int64_t)par * c.pub_reps + s];
         __syncthreads();
         phase_done(3);
         // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
@@ -352,8 +362,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                     long long dE1 = 0, dE2 = 0;
                     icm_team<Fmt, false>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
                                          d.has_field != 0, n, nw, t, step, a.seed, c3i,
-                                         c.pub_spins + ((int64_t)par * nrep + r1) * nw,
-                                         c.pub_spins + ((int64_t)par * nrep + r2) * nw, sc, dE1, dE2, lane, kWave,
+                                         pub_sp + par * c.pub_words + (int64_t)r1 * nw,
+                                         pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, lane, kWave,
                                          false, lv);
                     if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
                 }
@@ -383,18 +393,19 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (wave == 0) {
             const unsigned long long tot = (unsigned long long)wave_sum_i64((long long)nsw_total);
             if (lane == 0) {
-                a.swaps[c.inst] += tot;
-                a.inst_step[c.inst] = step0 + (uint32_t)it;
+                a.swaps[inst] += tot;
+                a.inst_step[inst] = step0 + (uint32_t)it;
             }
         }
     }
 }
 
 #define PTX_COOP_INST(F)                                         \
-    template __global__ void k_coop<F, 4, 1>(DevArgs, CoopArgs, int);   \
-    template __global__ void k_coop<F, 8, 1>(DevArgs, CoopArgs, int);   \
-    template __global__ void k_coop<F, 4, 4>(DevArgs, CoopArgs, int);   \
-    template __global__ void k_coop<F, 8, 4>(DevArgs, CoopArgs, int);
+    template __global__ void k_coop<F, 4, 1, 0>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 8, 1, 0>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 4, 4, 0>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 8, 4, 0>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 4, 1, 1>(DevArgs, CoopArgs, int);
 PTX_COOP_INST(kFmtWide)
 PTX_COOP_INST(kFmtUnit)
 

@@ -251,38 +251,23 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     active.assign((size_t)ninst, 1);
 
     // ---- execution mode ----
+    spin_words = spin_off;
     mode = PTX_MODE_SPLIT;
-    if ((cfg.mode == PTX_MODE_COOP || cfg.mode == PTX_MODE_AUTO) && ninst == 1) {
-        int dev = 0, ncu = 0, coopok = 0;
-        PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
-        PTX_TRY(hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
-        PTX_TRY(hip_check(hipDeviceGetAttribute(&coopok, hipDeviceAttributeCooperativeLaunch, dev), "coop attr"));
-        // with the next sweep's random words in LDS when they fit, else without
-        // with the lattice tables and the next sweep's random words in LDS
-        // when they fit, else without (tables first to go)
-        const InstDesc& d0 = desc[0];
-        auto lay = [&](int pre, int stage) {
-            return coop_layout(d0.n, (int)d0.adj_len16, d0.nrep, d0.nt, d0.fmax, pre, d0.lat_words, stage);
-        };
-        coop_pre_rnd = 1;
-        coop_lat_stage = 1;
-        if (lay(1, 1).total > 160 * 1024) coop_lat_stage = 0;
-        if (lay(1, 0).total > 160 * 1024) coop_pre_rnd = 0;
-        const CoopLayout L = lay(coop_pre_rnd, coop_lat_stage);
-        const bool fits = coopok && L.total <= 160 * 1024 && desc[0].nrep <= ncu;
-        if (fits) {
-            mode = PTX_MODE_COOP;
-            coop_lds = L.total;
-            Wf = cfg.waves_per_group == 8 ? 8 : 4;
-            spl = cfg.spins_per_lane == 4 ? 4 : 1;
-            K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 256;
-        } else if (cfg.mode == PTX_MODE_COOP) {
-            return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode needs one instance with at most one replica "
-                                                    "per CU and its adjacency within 160 KiB of LDS");
-        }
-    } else if (cfg.mode == PTX_MODE_COOP) {
-        return Status::err(PTX_ERR_UNSUPPORTED, "cooperative mode runs a single instance");
+    if (cfg.mode == PTX_MODE_COOP) {
+        coop_W = cfg.waves_per_group == 8 ? 8 : 4;
+        spl = cfg.spins_per_lane == 4 ? 4 : 1;
+        coop_K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 256;
+        PTX_TRY(setup_coop(true));
+        mode = PTX_MODE_COOP;
+    } else if (cfg.mode == PTX_MODE_AUTO && ninst == 1) {
+        coop_W = 4;
+        spl = cfg.spins_per_lane == 4 ? 4 : 1;
+        coop_K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 256;
+        PTX_TRY(setup_coop(false));
+        if (coop_capacity >= total_rep) mode = PTX_MODE_COOP;
     }
+    Wf = coop_W;
+    if (mode == PTX_MODE_COOP) K = coop_K;
     // The fused kernel runs when asked for (or chosen by the autotuner): on
     // the batches measured here the split kernels, which spread every
     // instance's replicas and cluster moves over the whole GPU, are faster.
@@ -331,13 +316,19 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
-    PTX_TRY(d_step.alloc(1));
-    if (mode == PTX_MODE_COOP) {
-        PTX_TRY(d_pub_spins.alloc((size_t)2 * desc[0].nrep * desc[0].nw));
-        PTX_TRY(d_pub_E.alloc((size_t)2 * desc[0].nrep));
+    if (mode == PTX_MODE_SPLIT && cfg.small_batch_coop && (cfg.mode == PTX_MODE_AUTO || cfg.mode == PTX_MODE_SPLIT)) {
+        coop_W = 4;
+        spl = 1;
+        coop_K = 256;
+        PTX_TRY(setup_coop(false));
+    }
+    if (coop_capacity > 0) {
+        PTX_TRY(d_pub_spins.alloc((size_t)(2 * spin_words)));
+        PTX_TRY(d_pub_E.alloc((size_t)(2 * total_rep)));
         PTX_TRY(d_bar.alloc(1));
         PTX_TRY(d_err.alloc(1));
     }
+    last_mode = mode;
     PTX_TRY(d_rowinfo.upload(rowinfo, stream));
     PTX_TRY(d_adj.upload(adjw, stream));
     PTX_TRY(d_cls.upload(cls, stream));
@@ -394,7 +385,6 @@ static DevArgs build_args(Engine& e) {
     a.lat = e.d_lat.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.inst_step = e.d_inst_step.p;
-    a.split_step = e.d_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
     a.sweeps = e.cfg.sweeps;
@@ -498,7 +488,9 @@ Status Engine::launch_step() {
             }
         }));
     }
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream, d_step.p);
+    const int na = (int)inst_list.size();
+    hipLaunchKernelGGL(k_step_advance, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, d_order.p, na,
+                       d_inst_step.p);
     return hip_check(hipGetLastError(), "k_step_advance");
 }
 
@@ -554,65 +546,130 @@ Status Engine::launch_fused(int nsteps) {
     }
 }
 
+// The cooperative kernel of the engine's format, width and positions per
+// lane, with the adjacency in LDS (g = 0) or read through L2 (g = 1: four
+// waves, one position per lane).
+const void* Engine::coop_kernel(int g) const {
+    if (fmt == kFmtUnit) {
+        if (g) return reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 1, 1>);
+        if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtUnit, 8, 4, 0>)
+                                         : reinterpret_cast<const void*>(k_coop<kFmtUnit, 8, 1, 0>);
+        return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 4, 0>)
+                        : reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 1, 0>);
+    }
+    if (g) return reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 1, 1>);
+    if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtWide, 8, 4, 0>)
+                                     : reinterpret_cast<const void*>(k_coop<kFmtWide, 8, 1, 0>);
+    return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 4, 0>)
+                    : reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 1, 0>);
+}
+
+// Sizes the cooperative launches, for the adjacency in LDS and (with four
+// waves and one position per lane) read through L2: the LDS layout of the
+// largest instance (with the lattice tables and the next sweep's random
+// words when every instance still fits 160 KiB, tables first to go) and the
+// number of workgroups the device keeps resident at once.  required: an
+// error when the whole batch fits neither form; coop_capacity = 0 says no
+// cooperative launch is possible.
+Status Engine::setup_coop(bool required) {
+    coop_capacity = 0;
+    int dev = 0, ncu = 0, coopok = 0;
+    PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+    PTX_TRY(hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"));
+    PTX_TRY(hip_check(hipDeviceGetAttribute(&coopok, hipDeviceAttributeCooperativeLaunch, dev), "coop attr"));
+    const int lds_limit = 160 * 1024;
+    for (int g = 0; g < 2; ++g) {
+        CoopForm& cf = coop_form[g];
+        cf = CoopForm{};
+        if (!coopok || (g == 1 && (coop_W != 4 || spl != 1))) continue;
+        auto max_total = [&](int pre, int stage) {
+            int mx = 0;
+            for (const InstDesc& d : desc)
+                mx = std::max(mx, coop_layout(d.n, (int)d.adj_len16, d.nrep, d.nt, d.fmax, pre, d.lat_words, stage, g)
+                                      .total);
+            return mx;
+        };
+        cf.pre_rnd = 1;
+        cf.lat_stage = 1;
+        if (max_total(1, 1) > lds_limit) cf.lat_stage = 0;
+        if (max_total(1, 0) > lds_limit) cf.pre_rnd = 0;
+        cf.lds = max_total(cf.pre_rnd, cf.lat_stage);
+        if (cf.lds > lds_limit) continue;
+        const void* f = coop_kernel(g);
+        if (cf.lds > 64 * 1024)
+            PTX_TRY(hip_check(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, cf.lds),
+                              "hipFuncSetAttribute"));
+        int bpc = 0;
+        PTX_TRY(hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, f, coop_W * kWave, (size_t)cf.lds),
+                          "occupancy"));
+        cf.capacity = (int64_t)bpc * ncu;
+        coop_capacity = std::max(coop_capacity, cf.capacity);
+    }
+    if (required && coop_capacity < total_rep)
+        return Status::err(PTX_ERR_UNSUPPORTED,
+                           "cooperative mode needs every replica of the batch resident at once (" +
+                               std::to_string(total_rep) + " workgroups, room for " + std::to_string(coop_capacity) +
+                               ") and each instance within 160 KiB of LDS");
+    return Status::ok();
+}
+
 Status Engine::launch_coop(int nsteps) {
+    // the adjacency in LDS while the active replicas fit that form, else through L2
+    const int g = (int64_t)coop_items.size() <= coop_form[0].capacity ? 0 : 1;
+    const CoopForm& cf = coop_form[g];
+    coop_lds = cf.lds;
+    coop_last_form = g;
     DevArgs a = build_args(*this);
     CoopArgs c{};
     c.pub_spins = d_pub_spins.p;
     c.pub_E = d_pub_E.p;
+    c.items = d_coop_items.p;
     c.bar = d_bar.p;
     c.err = d_err.p;
-    c.bar_base = barriers_done;
-    c.nblocks = desc[0].nrep;
-    c.inst = 0;
-    c.pre_rnd = coop_pre_rnd;
-    c.lat_stage = coop_lat_stage;
-    const dim3 grid((unsigned)desc[0].nrep), block((unsigned)(Wf * kWave));
-    const unsigned lds = (unsigned)coop_lds;
-    auto go = [&](auto kern) -> Status {
-        const void* f = reinterpret_cast<const void*>(kern);
-        if (lds > 64 * 1024)
-            PTX_TRY(hip_check(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                              "hipFuncSetAttribute"));
-        void* args[] = {&a, &c, &nsteps};
-        Timed t;
-        if (timing) {
-            t.kind = 3;
-            PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
-            PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
-            PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
-        }
-        PTX_TRY(hip_check(hipLaunchCooperativeKernel(f, grid, block, args, lds, stream), "k_coop launch"));
-        if (timing) {
-            PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
-            pending.push_back(t);
-        }
-        return Status::ok();
-    };
-    Status st;
-    if (fmt == kFmtUnit) {
-        if (Wf == 8) st = spl == 4 ? go(k_coop<kFmtUnit, 8, 4>) : go(k_coop<kFmtUnit, 8, 1>);
-        else st = spl == 4 ? go(k_coop<kFmtUnit, 4, 4>) : go(k_coop<kFmtUnit, 4, 1>);
-    } else {
-        if (Wf == 8) st = spl == 4 ? go(k_coop<kFmtWide, 8, 4>) : go(k_coop<kFmtWide, 8, 1>);
-        else st = spl == 4 ? go(k_coop<kFmtWide, 4, 4>) : go(k_coop<kFmtWide, 4, 1>);
+    c.bar_base = bar_arrivals;
+    c.pub_words = spin_words;
+    c.pub_reps = (int32_t)total_rep;
+    c.nblocks = (int32_t)coop_items.size();
+    c.pre_rnd = cf.pre_rnd;
+    c.lat_stage = cf.lat_stage;
+    const dim3 grid((unsigned)coop_items.size()), block((unsigned)(coop_W * kWave));
+    const unsigned lds = (unsigned)cf.lds;
+    const void* f = coop_kernel(g);
+    void* args[] = {&a, &c, &nsteps};
+    Timed t;
+    if (timing) {
+        t.kind = 3;
+        PTX_TRY(hip_check(hipEventCreate(&t.a), "hipEventCreate"));
+        PTX_TRY(hip_check(hipEventCreate(&t.b), "hipEventCreate"));
+        PTX_TRY(hip_check(hipEventRecord(t.a, stream), "hipEventRecord"));
     }
-    if (st) barriers_done += (unsigned long long)nsteps;
-    return st;
+    PTX_TRY(hip_check(hipLaunchCooperativeKernel(f, grid, block, args, lds, stream), "k_coop launch"));
+    if (timing) {
+        PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
+        pending.push_back(t);
+    }
+    bar_arrivals += (unsigned long long)nsteps * coop_items.size();
+    return Status::ok();
 }
 
 Status Engine::step(int64_t n) {
     if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
     if (inst_list.empty()) return Status::ok();  // every instance retired
     if (steps_done + n > 0xFFFFFFFFll) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^32 PT steps");
-    if (mode == PTX_MODE_COOP) {
+    // cooperative launches: the cooperative mode, or a split-mode engine
+    // with small_batch_coop whose active replicas fit one launch
+    if (mode == PTX_MODE_COOP || (mode == PTX_MODE_SPLIT && cfg.small_batch_coop && coop_capacity > 0 &&
+                                  active_reps <= coop_capacity)) {
+        last_mode = PTX_MODE_COOP;
         while (n > 0) {
-            const int k = (int)std::min<int64_t>(n, K);
+            const int k = (int)std::min<int64_t>(n, coop_K);
             PTX_TRY(launch_coop(k));
             steps_done += k;
             n -= k;
         }
         return Status::ok();
     }
+    last_mode = mode;
     if (mode == PTX_MODE_FUSED) {
         const int64_t G = cfg.use_graph > 0 ? cfg.use_graph : 0;
         while (n > 0) {
@@ -716,9 +773,16 @@ Status Engine::build_lists() {
         for (int32_t i : act)
             for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
     }
+    coop_items.clear();
+    active_reps = 0;
+    for (int32_t i : act) {
+        active_reps += desc[i].nrep;
+        for (int32_t r = 0; r < desc[i].nrep; ++r) coop_items.push_back(make_int2(i, r));
+    }
     PTX_TRY(put_list(d_items, items, stream));
     PTX_TRY(put_list(d_pairs, pairs, stream));
     PTX_TRY(put_list(d_order, inst_list, stream));
+    PTX_TRY(put_list(d_coop_items, coop_items, stream));
     return hip_check(hipStreamSynchronize(stream), "work lists");
 }
 
@@ -759,7 +823,7 @@ Status Engine::build_graph() {
 
 Status Engine::sync() {
     PTX_TRY(hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"));
-    if (mode == PTX_MODE_COOP && d_err.p) {
+    if (d_err.p) {
         uint32_t err = 0;
         PTX_TRY(hip_check(hipMemcpy(&err, d_err.p, sizeof(err), hipMemcpyDeviceToHost), "read error flag"));
         if (err) return Status::err(PTX_ERR_DEVICE, "cooperative grid barrier timed out");

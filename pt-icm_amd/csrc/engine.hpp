@@ -91,17 +91,31 @@ struct Engine {
     DevBuf<int2> d_items, d_pairs;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
-    DevBuf<uint32_t> d_step;     // split mode: the PT step the next launches run
     DevBuf<unsigned long long> d_phase;
-    // cooperative mode
+    // cooperative launches (cooperative mode, or small batches of a split-mode
+    // engine with small_batch_coop): one workgroup per active replica
     DevBuf<uint32_t> d_pub_spins, d_err;
     DevBuf<int32_t> d_pub_E;
     DevBuf<unsigned long long> d_bar;
-    unsigned long long barriers_done = 0;
+    DevBuf<int2> d_coop_items;
+    std::vector<int2> coop_items;  // (instance, replica) per workgroup, active instances
+    unsigned long long bar_arrivals = 0;  // grid-barrier arrivals of all cooperative launches so far
+    int64_t spin_words = 0;      // spin words of all instances
+    int64_t active_reps = 0;     // replicas of the active instances
+    int64_t coop_capacity = 0;   // co-resident cooperative workgroups (0: no cooperative launches)
+    int32_t coop_W = 4;          // waves per cooperative workgroup
+    int32_t coop_K = 256;        // PT steps per cooperative launch
+    int32_t last_mode = PTX_MODE_SPLIT;  // mode of the latest launches
     int32_t spl = 1;             // positions per lane (cooperative mode)
-    int32_t coop_pre_rnd = 0;    // cooperative mode: next sweep's random words drawn during the grid barrier
-    int32_t coop_lat_stage = 0;  // cooperative mode: lattice tables copied into LDS
-    int32_t coop_lds = 0;
+    struct CoopForm {            // a cooperative launch form: adjacency in LDS (0) or read through L2 (1)
+        int32_t lds = 0;         // dynamic LDS bytes per workgroup
+        int32_t pre_rnd = 0;     // next sweep's random words drawn during the grid barrier
+        int32_t lat_stage = 0;   // lattice tables copied into LDS
+        int64_t capacity = 0;    // co-resident workgroups (0: form not available)
+    };
+    CoopForm coop_form[2];
+    int32_t coop_lds = 0;        // LDS bytes of the latest cooperative launch
+    int32_t coop_last_form = 0;  // form of the latest cooperative launch
     bool phase_on = false;
     // fused mode with use_graph = G > 0: G launches of K steps captured once
     // and replayed (the kernels read their step counter from device memory)
@@ -130,6 +144,8 @@ struct Engine {
     Status launch_step();
     Status launch_fused(int nsteps);
     Status launch_coop(int nsteps);
+    Status setup_coop(bool required);
+    const void* coop_kernel(int g) const;
     Status build_graph();
     Status build_lists();
 

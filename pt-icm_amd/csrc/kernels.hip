@@ -71,11 +71,11 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
 // ---------------------------------------------------------------------------
 template <int Fmt, int W>
 __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
-    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int2 it = a.items[blockIdx.x];
     if (it.x < 0) return;  // idle item padding an XCD's list
+    const uint32_t step = a.inst_step[it.x];
     const InstDesc d = a.desc[it.x];
     const int r = it.y + wave;
     if (r >= d.nrep) return;
@@ -125,10 +125,10 @@ __device__ __forceinline__ void record_histogram(const DevArgs& a, const InstDes
 // k_exchange: one wavefront per instance (LDS: 48 bytes per slot).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, int nrep_max) {
-    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int inst = a.order[blockIdx.x];  // the active instances
+    const uint32_t step = a.inst_step[inst];
     const InstDesc d = a.desc[inst];
     const int nrep = d.nrep;
     double* beta = reinterpret_cast<double*>(smem);
@@ -174,11 +174,11 @@ __global__ __launch_bounds__(kWave) void k_exchange(DevArgs a, int nrep_max) {
 // ---------------------------------------------------------------------------
 template <int Fmt, int W>
 __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat_words_max) {
-    const uint32_t step = *a.split_step;
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x;
     const int2 pr = a.pairs[blockIdx.x];
     if (pr.x < 0) return;  // idle item padding an XCD's list
+    const uint32_t step = a.inst_step[pr.x];
     const InstDesc d = a.desc[pr.x];
     const int t = pr.y;
     IcmScratch sc;
@@ -415,9 +415,12 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
     }
 }
 
-// Ends a split-mode PT step: the next launches read the next step index.
-// (A kernel rather than a host copy, so that graph replays advance it too.)
-__global__ void k_step_advance(uint32_t* step) { *step += 1u; }
+// Ends a split-mode PT step: every active instance's step index advances
+// (a kernel rather than a host copy, so that graph replays advance it too).
+__global__ void k_step_advance(const int32_t* order, int n, uint32_t* inst_step) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) inst_step[order[t]] += 1u;
+}
 
 // ---------------------------------------------------------------------------
 // Self-tests.

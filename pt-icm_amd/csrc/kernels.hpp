@@ -45,8 +45,7 @@ struct DevArgs {
     uint32_t* swap_succ;
     unsigned long long* swaps;  // per instance
     uint32_t* hist;
-    uint32_t* inst_step;     // per-instance PT step counter (fused mode)
-    const uint32_t* split_step;  // split mode: the PT step of the current launches (advanced by k_step_advance)
+    uint32_t* inst_step;     // per-instance PT step counter (every mode; split mode advances it with k_step_advance)
     const int2* items;       // (instance, first replica) per sweep workgroup
     const int2* pairs;       // (instance, slot t) per ICM wavefront
     const int32_t* order;    // fused mode: instance per workgroup

@@ -39,6 +39,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -116,7 +117,8 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate
     t_tune = time.perf_counter()
     kw, _ = ptx.autotune(probs, lads, seed=seed + 10, steps=32, flags=ptx.FLAG_ICM)
     t_tune = time.perf_counter() - t_tune
-    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM, use_graph=(50 if kw["mode"] == 1 else 4), **kw)
+    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM, use_graph=(50 if kw["mode"] == 1 else 4),
+                     small_batch_coop=1, **kw)
     steps = 0
     t0 = time.perf_counter()
     # instances that reached E* are retired from the batch at every poll; the
@@ -151,6 +153,116 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate
             "waves_per_group": info.waves_per_group, "autotune_s": t_tune,
             "mean_slots": float(np.mean([len(l) for l in lads])),
             "mean_icm_slots": float(np.mean([eng.instance_info(i).nc for i in range(len(specs))]))}
+
+
+def _rates(eng, inst):
+    """Adjacent-pair swap acceptance of instance inst (both chains pooled)."""
+    nt = eng.instance_info(inst).nt
+    a, s, _ = eng.swap_stats(inst)
+    att = a[: nt - 1].astype(np.float64) + a[nt: 2 * nt - 1]
+    suc = s[: nt - 1].astype(np.float64) + s[nt: 2 * nt - 1]
+    return suc / np.maximum(att, 1)
+
+
+def run_batched_judged_in_run(ptx, specs, targets, max_steps, poll, seed, log=print, validate_steps=2000):
+    """Batched + surrogate with the judge inside the run: every instance starts
+    the main run (ICM on) on its network ladder right after the 8-slot pilot,
+    and the first validate_steps steps of that run are the validation run.
+    Then an instance whose measured adjacent swap rates are not all within
+    TARGET +- TOL, and that has not reached E* yet, leaves the run and falls
+    back to the multihistogram preprocess ladder (computed on a worker thread
+    while the accepted instances run on), with the remaining step budget
+    max_steps - validate_steps.  Instances that reached E* during the
+    validation window are done whatever their rates."""
+    probs, stats = build_problems(ptx, specs)
+    targets = np.asarray(targets)
+    n = len(specs)
+    excluded = [0.0]  # the one-off autotune is reported, not counted
+    t_start = time.perf_counter()
+
+    def now():
+        return time.perf_counter() - t_start - excluded[0]
+
+    sur = S.Surrogate(ptx)
+    X, bs, be = S.short_features(ptx, probs, stats, T0, T1, 500, 1000, seed)
+    lads = [np.sort(l) for l in sur.ladders(X, bs, be)]
+    t_lad = now()
+    t0 = time.perf_counter()
+    kw, _ = ptx.autotune(probs, lads, seed=seed + 10, steps=32, flags=ptx.FLAG_ICM)
+    excluded[0] += time.perf_counter() - t0
+    graph = 50 if kw["mode"] == 1 else 4
+    eng = ptx.Engine(probs, lads, seed=seed + 10, flags=ptx.FLAG_ICM, use_graph=graph, small_batch_coop=1, **kw)
+    done = np.zeros(n, bool)
+    t_hit = np.full(n, np.inf)
+
+    def poll_once(e, idx, k):
+        e.step(k)
+        m = e.min_energies()
+        t = now()
+        newly = (m <= targets[idx]) & ~done[idx]
+        if newly.any():
+            t_hit[idx[newly]] = t
+            done[idx[newly]] = True
+        return newly.any()
+
+    all_idx = np.arange(n)
+    steps1 = 0
+    while steps1 < validate_steps and done.mean() < 0.99:
+        k = min(poll, validate_steps - steps1)
+        if poll_once(eng, all_idx, k):
+            eng.set_active(~done)
+        steps1 += k
+    ok = np.array([S.in_band(_rates(eng, i)) for i in range(n)])
+    fb = np.nonzero(~done & ~ok)[0]
+    in_run = ~done & ok
+    eng.set_active(in_run)
+    t_judge = now()
+    res = {}
+
+    def fallback():
+        ft = {}
+        mh, _, _ = S.multihistogram_ladders(ptx, [probs[i] for i in fb], S.N_OUT, T0, T1, 5000, 2000, seed + 2,
+                                            timing=ft)
+        res["lads"] = [np.sort(l) for l in mh]
+        res["timing"] = ft
+        res["t_ready"] = now()
+
+    th = threading.Thread(target=fallback) if len(fb) else None
+    if th:
+        th.start()
+    eng2, steps2, budget2 = None, 0, max_steps - validate_steps
+    while done.mean() < 0.99:
+        busy = False
+        if steps1 < max_steps and (in_run & ~done).any():
+            k = min(poll, max_steps - steps1)
+            if poll_once(eng, all_idx, k):
+                eng.set_active(in_run & ~done)
+            steps1 += k
+            busy = True
+        if th is not None and eng2 is None and not th.is_alive():
+            th.join()
+            eng2 = ptx.Engine([probs[i] for i in fb], res["lads"], seed=seed + 20, flags=ptx.FLAG_ICM,
+                              use_graph=graph, small_batch_coop=1, **kw)
+        if eng2 is not None and steps2 < budget2 and (~done[fb]).any():
+            k = min(poll, budget2 - steps2)
+            if poll_once(eng2, fb, k):
+                eng2.set_active(~done[fb])
+            steps2 += k
+            busy = True
+        if not busy:
+            if th is not None and eng2 is None:
+                th.join()  # nothing left to run until the fallback ladders exist
+                continue
+            break
+    wall = now()
+    ttt = t_hit
+    return {"instances": n, "hit_rate": float(done.mean()), "p50_ttt_s": float(np.median(ttt)), "wall_s": wall,
+            "ladder_s": t_lad, "judge_at_s": t_judge, "validate_steps": validate_steps,
+            "surrogate_accepted": int(ok.sum()), "done_in_validation": int(n - len(fb) - int(in_run.sum())),
+            "fallback": int(len(fb)), "fallback_ready_s": res.get("t_ready"),
+            "fallback_timing_s": {k: round(v, 3) for k, v in res.get("timing", {}).items()},
+            "steps_accepted_run": steps1, "steps_fallback_run": steps2,
+            "instances_per_hour": n * 3600.0 / wall, "autotune_s": excluded[0], "mode": kw["mode"]}
 
 
 def run_plain(ptx, specs, targets, max_steps, poll, seed, log=print):
@@ -191,6 +303,8 @@ def run(ptx, n_plain=8, seed=2024, log=print, path=TARGETS):
     cfg = load_targets(path)
     specs, targets = cfg["specs"], np.array(cfg["targets"], np.int64)
     b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
+    bj = run_batched_judged_in_run(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
+    log(f"judged in the run: {bj}")
     bm = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log, surrogate=False)
     # an even sample of both families (specs alternate torus, ER)
     stride = max(2, 2 * (len(specs) // max(n_plain, 1) // 2))
@@ -198,8 +312,9 @@ def run(ptx, n_plain=8, seed=2024, log=print, path=TARGETS):
                  list(range(1, len(specs), stride))[: n_plain // 2])
     pl = run_plain(ptx, [specs[i] for i in idx], targets[idx], cfg["max_steps"], cfg["poll"], seed + 7, log)
     return {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "batched_surrogate": b,
-            "batched_multihistogram": bm, "plain": pl,
-            "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"]}
+            "batched_surrogate_judged_in_run": bj, "batched_multihistogram": bm, "plain": pl,
+            "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"],
+            "instances_per_hour_ratio_judged_in_run": bj["instances_per_hour"] / pl["instances_per_hour"]}
 
 
 def main():

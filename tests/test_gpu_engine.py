@@ -102,7 +102,7 @@ def test_autotune_single_instance_includes_cooperative(ptx):
     _same(_state(a, 1), _state(b, 1))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_retired_instances_freeze_and_the_rest_match(ptx, mode):
     probs, lads = _batch(ptx, k=12, seed=2)
     ref = ptx.Engine(probs, lads, seed=4, flags=ptx.FLAG_ICM, mode=mode, steps_per_launch=1)
@@ -128,3 +128,31 @@ def test_retired_instances_freeze_and_the_rest_match(ptx, mode):
     with pytest.raises(ptx.PtxError):
         eng.set_active(np.zeros(len(probs), bool))
         eng.set_spins(eng.spins(0), 0)
+
+
+def test_small_batch_coop_switches_without_changing_results(ptx):
+    """A split-mode batch too large for one cooperative launch runs the split
+    kernels; once retirement leaves few enough replicas it runs the
+    cooperative kernel -- with the same trajectories as a split-only engine
+    (every kernel draws from the instance's own step counter)."""
+    _, coos = instances.ea_batch(96, 7, sizes=(4, 6))
+    probs = [ptx.Problem.from_coo(*c) for c in coos]
+    lad = ptx.beta_linspace(0.2, 3.0, 16, True)
+    lads = [lad] * len(probs)
+    ref = ptx.Engine(probs, lads, seed=8, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT)
+    eng = ptx.Engine(probs, lads, seed=8, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT, small_batch_coop=1, use_graph=2)
+    ref.step(6)
+    eng.step(6)
+    assert eng.info().mode == ptx.MODE_SPLIT  # 3072 workgroups: more than any MI355X keeps resident
+    say("96 x 32 replicas: split")
+    _same(_state(ref, len(probs)), _state(eng, len(probs)))
+    mask = np.array([i % 12 == 0 for i in range(len(probs))])  # 8 x 32 replicas left
+    ref.set_active(mask)
+    eng.set_active(mask)
+    ref.step(9)
+    eng.step(9)
+    assert eng.info().mode == ptx.MODE_COOP
+    _same(_state(ref, len(probs)), _state(eng, len(probs)))
+    for i in np.nonzero(mask)[0]:
+        sp = eng.spins(i)
+        assert np.array_equal(eng.energies(i), [probs[i].energy(x) for x in sp])

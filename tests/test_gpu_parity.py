@@ -259,10 +259,11 @@ def test_pilot_histograms_match_oracle_logs(ptx, oracle, mode):
 
 
 # ---------------------------------------------------------------- batched ---
-@pytest.mark.parametrize("mode,csr", [(1, 0), (2, 1), (2, 2)])
+@pytest.mark.parametrize("mode,csr", [(1, 0), (2, 1), (2, 2), (3, 0)])
 @pytest.mark.parametrize("with_field", [False, True])
 def test_batched_instances_match_single_oracles(ptx, oracle, mode, csr, with_field):
-    """csr 1: adjacency staged in LDS; csr 2: the fused kernel reads it through L2."""
+    """csr 1: adjacency staged in LDS; csr 2: the fused kernel reads it through
+    L2; mode 3: one cooperative launch for the whole batch."""
     kinds = ["lattice_2d_10.gset", "ea3d_6", "pm_torus20", "lattice_2d_5.gset", "er_small"]
     if with_field:
         kinds.append("field_random")  # forces the 32-bit adjacency format in the fused kernel
@@ -281,6 +282,28 @@ def test_batched_instances_match_single_oracles(ptx, oracle, mode, csr, with_fie
             oe.step(1)
         for i, oe in enumerate(oes):
             compare(eng, oe, inst=i, what=f"inst {i} step {s}")
+
+
+def test_cooperative_batch_reads_adjacency_through_l2_when_lds_is_full(ptx, oracle):
+    """Six G022-sized ER graphs: one instance's adjacency fills most of a
+    CU's LDS, so 288 resident workgroups need the form that reads the
+    adjacency through L2 -- bit-exact with the oracle like the LDS form."""
+    pairs = []
+    for k in range(6):
+        n, u, v, w = instances.erdos_renyi(2000, 0.01, seed=40 + k, pm=k % 2 == 1)
+        pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    lad = oracle.beta_linspace(0.5, 3.0, 24, 0)
+    eng = ptx.Engine([p for p, _ in pairs], [lad] * 6, seed=13, flags=ptx.FLAG_ICM, nc=6, mode=ptx.MODE_COOP,
+                     steps_per_launch=2)
+    oes = [oracle.Engine(g, lad, chains=2, seed=13, instance=i, flags=oracle.FLAG_ICM, nc=6)
+           for i, (_, g) in enumerate(pairs)]
+    for s_ in range(3):
+        eng.step(1)
+        for oe in oes:
+            oe.step(1)
+        for i, oe in enumerate(oes):
+            compare(eng, oe, inst=i, what=f"inst {i} step {s_}")
+    assert eng.info().csr_in_lds == 0
 
 
 def test_batched_3d_ea_invariants(ptx, oracle):
