@@ -77,6 +77,52 @@ def cpu_baseline(problem_path, betas, seconds, seed=0):
                       f"{steps} PT steps in {dt:.1f} s on 1 host core"}
 
 
+def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024):
+    """Throughput of the batched engine at the scale of the batched + surrogate
+    configuration: half 2D +-J tori L = 100 (10,000 spins), half ER +-1 graphs
+    G(5000, 0.004); 64 inverse temperatures linear in beta over T 0.5..3.0
+    (ascending, so the slot rule gives the cold slots ICM), 2 chains, ICM.
+    The launch shape comes from the autotuner (8 steps per trial)."""
+    sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+    import instances
+    from concurrent.futures import ThreadPoolExecutor
+
+    def make(i):
+        if i % 2 == 0:
+            c = instances.torus_2d(100, seed=seed * 100003 + i)
+        else:
+            c = instances.erdos_renyi(5000, 0.004, seed=seed * 100003 + i, pm=True)
+        return ptx.Problem.from_coo(*c)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(8) as ex:
+        probs = list(ex.map(make, range(n_inst)))
+    t_build = time.perf_counter() - t0
+    lad = np.sort(ptx.beta_linspace(0.5, 3.0, 64, False))
+    t0 = time.perf_counter()
+    kw, trials = ptx.autotune(probs, [lad] * n_inst, seed=seed, steps=8, flags=ptx.FLAG_ICM)
+    t_tune = time.perf_counter() - t0
+    eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, use_graph=4, **kw)
+    eng.step(4)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    info = eng.info()
+    icm = [eng.instance_info(i).nc for i in range(2)]
+    return {"workload": f"{n_inst // 2} 2D +-J tori L=100 + {n_inst - n_inst // 2} ER +-1 G(5000, 0.004); "
+                        "64 betas linear over T 0.5..3.0, 2 chains, ICM",
+            "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
+            "ms_per_step": dt * 1e3 / steps, "steps": steps, "replicas": info.total_replicas,
+            "icm_slots": {"torus": icm[0], "er": icm[1]},
+            "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
+                             "icm_waves": info.icm_waves},
+            "build_s": round(t_build, 2), "autotune_s": round(t_tune, 2),
+            "autotune_trials": [{k: (round(v, 1) if isinstance(v, float) else v) for k, v in t.items()}
+                                for t in trials]}
+
+
 def batched_ea(ptx, steps, seed, n_inst=4096):
     """The batched workload: 4096 independent 3D Edwards-Anderson +-J instances,
     L uniform in {6, 8, 10, 12}, 32 inverse temperatures 0.2..3.0 x 2 chains,
@@ -144,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0,
                     help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
+    ap.add_argument("--scale-steps", type=int, default=8,
+                    help="timed PT steps of the 1024-instance mixed batch (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
     ap.add_argument("--ttt-plain", type=int, default=8,
@@ -162,7 +210,14 @@ def main():
     t_pre = time.perf_counter() - t0
     nt = len(betas)
     log(f"preprocess: {nt} temperatures in {t_pre:.2f} s; beta {betas[0]:.4f} .. {betas[-1]:.4f}")
-    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM, mode=args.mode, steps_per_launch=500)
+    tune = None
+    if args.mode == 0:  # launch shape from the autotuner (the trajectories do not depend on it)
+        kw, trials = ptx.autotune(p, betas, seed=args.seed, steps=500, flags=ptx.FLAG_ICM, steps_per_launch=500)
+        tune = {"chosen": kw, "trials": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in t.items()}
+                                         for t in trials]}
+    else:
+        kw = {"mode": args.mode}
+    eng = ptx.Engine(p, betas, seed=args.seed, flags=ptx.FLAG_ICM, steps_per_launch=500, **kw)
     nc = eng.instance_info().nc
     eng.step(args.warmup)
     eng.sync()
@@ -203,7 +258,9 @@ def main():
         "config": {"workload": "plain PT-ICM single instance: G022, multihistogram ladder (64 requested, "
                                f"{nt} chosen), 2 chains, ICM slots {nc}, 1 sweep/step",
                    "n": p.n, "replicas": 2 * nt, "icm_slots": nc, "preprocess_s": round(t_pre, 3),
-                   "best_energy": best, "best_cut": (19990 - best) // 2},
+                   "best_energy": best, "best_cut": (19990 - best) // 2,
+                   "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
+                                    "steps_per_launch": info.steps_per_launch}, "autotune": tune},
         "kernel_ms": {"sweep": tm.sweep_ms / max(tm.sweep_launches, 1),
                       "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
                       "icm": tm.icm_ms / max(tm.icm_launches, 1), "fused": tm.fused_ms / max(tm.fused_launches, 1)},
@@ -216,6 +273,8 @@ def main():
     del eng
     if args.batched_steps > 0:
         out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
+    if args.scale_steps > 0:
+        out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed)
         out["batched"]["roofline"]["peak"] = lds_peak
         out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
     if args.ttt_plain > 0 and os.path.exists(args.ttt_targets):

@@ -188,9 +188,9 @@ ptx_status ptx_engine_step(ptx_engine* e, int64_t n_steps);
  * active ones, with their work lists rebuilt, and retired instances keep
  * their state.  A batch whose instances finish at different times (for
  * example on reaching a target energy) stops paying for the finished ones.
- * No counterpart in the reference (one instance per run).  In split mode the
- * step index of the random streams is the engine's, so an instance
- * reactivated after a pause draws from the steps it rejoins at. */
+ * No counterpart in the reference (one instance per run).  Every instance
+ * keeps its own PT step counter (the index of its random streams), so a
+ * reactivated instance continues exactly where it stopped. */
 ptx_status ptx_engine_set_active(ptx_engine* e, const uint8_t* active);
 ptx_status ptx_engine_sync(ptx_engine* e);
 

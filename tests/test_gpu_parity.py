@@ -170,11 +170,12 @@ def test_icm_all_slots_parity(ptx, oracle, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("dim,L", [(2, 33), (2, 40), (2, 64), (3, 5), (3, 12)])
+@pytest.mark.parametrize("dim,L", [(2, 33), (2, 40), (2, 64), (2, 100), (3, 5), (3, 12), (3, 24)])
 def test_lattice_cluster_growth_parity(ptx, oracle, dim, L, mode):
     """Periodic lattices grow clusters as bit sets (split and cooperative
-    kernels; one row per word and rows of several words, odd L with three
-    colour classes, 3D): the oracle's breadth-first clusters bit for bit,
+    kernels; rows of one or several words, held in registers, in registers
+    and LDS, or updated in place in LDS (2D L = 100, 3D L = 24); odd L with
+    three colour classes): the oracle's breadth-first clusters bit for bit,
     with ICM on every slot (global flips included)."""
     if dim == 2:
         n, u, v, w = instances.torus_2d(L, seed=L)
