@@ -141,8 +141,8 @@ __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err
 }
 
 struct CoopLayout {
-    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, lat_tab, acc,
-        flag, rnd, total;
+    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, red, lat_tab,
+        acc, flag, rnd, total;
 };
 
 // pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
@@ -178,6 +178,7 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     L.sp = take(spin_bytes(nw));
     L.icm = take(icm_scratch_bytes(n));
     L.lat = take(8 * lat_words);
+    L.red = take(8 * kIcmRedSlots);
     L.lat_tab = take(lat_stage && lat_words > 0 ? 4 * (16 * lat_words + n) : 0);
     L.acc = take(4 * 2 * 2 * kMaxCoopWaves);
     L.flag = take(4);
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     sc.V = sc.D + (nw + 1);
     sc.tail = reinterpret_cast<int32_t*>(sc.V + (nw + 1));
     sc.Q = reinterpret_cast<uint16_t*>(sc.tail + 4);
-    sc.red = nullptr;  // one wave grows the cluster: no workgroup sums
+    sc.red = reinterpret_cast<long long*>(smem + L.red);
     sc.Do = reinterpret_cast<uint32_t*>(smem + L.lat);
     sc.Vo = sc.Do + d.lat_words;
 
@@ -356,21 +357,36 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             const int t = r2s[r] % nt;
             if (t >= nt - d.nc) {
                 const int r1 = s2r[t], r2 = s2r[t + nt];
-                // one wave grows the cluster (its frontiers are mostly narrower
-                // than a wave, so more waves would only add barriers)
-                if (wave == 0) {
+                const int maxch = (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk;
+                if (d.lat_dim == 0 && d.deg_mean >= kIcmBlockDeg) {
+                    // wide levels: the whole workgroup grows the cluster and sums
+                    // its boundary; every thread gets both sums
                     long long dE1 = 0, dE2 = 0;
-                    icm_team<Fmt, false>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
-                                         d.has_field != 0, n, nw, t, step, a.seed, c3i,
-                                         pub_sp + par * c.pub_words + (int64_t)r1 * nw,
-                                         pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, lane, kWave,
-                                         false, lv);
-                    if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
+                    icm_team<Fmt, true>(csr, maxch, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                                        pub_sp + par * c.pub_words + (int64_t)r1 * nw,
+                                        pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, tid, nthr,
+                                        false);
+                    st.E += (int32_t)(r == r1 ? dE1 : dE2);
+                    expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
+                    __syncthreads();
+                } else {
+                    // lattices and sparse graphs: one wave grows the cluster (its
+                    // frontiers are mostly narrower than a wave, so more waves
+                    // would only add barriers)
+                    if (wave == 0) {
+                        long long dE1 = 0, dE2 = 0;
+                        icm_team<Fmt, false>(csr, maxch, hf,
+                                             d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                                             pub_sp + par * c.pub_words + (int64_t)r1 * nw,
+                                             pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, lane, kWave,
+                                             false, lv);
+                        if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
+                    }
+                    __syncthreads();
+                    st.E = flag[0];  // the new energy, shared with every wave
+                    expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
+                    __syncthreads();
                 }
-                __syncthreads();
-                st.E = flag[0];  // the new energy, shared with every wave
-                expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
-                __syncthreads();
             }
         }
         phase_done(2);

@@ -122,6 +122,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.fmax = P.fmax;
         d.has_field = P.has_field ? 1 : 0;
         d.max_deg = P.max_deg;
+        d.deg_mean = (int32_t)(P.nnz() / std::max(P.n, 1));
         int32_t nc = 0;
         if (cfg.chains == 2 && (cfg.flags & PTX_FLAG_ICM)) {
             nc = cfg.nc >= 0 ? std::min(cfg.nc, nt) : icm_count(ladders[i].data(), nt, P.char_scale);

@@ -197,9 +197,16 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
     const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
     long long dE1 = 0, dE2 = 0;
-    icm_team<Fmt, (W > 1)>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, a.hfield + d.pos_off,
-                           d.has_field != 0, d.n, d.nw, t, step, a.seed, ctr3(kTagIcm | a.tag_base, d.inst_id), g1,
-                           g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{});
+    const int maxch = (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk;
+    const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
+    if (W > 1 && d.lat_dim > 0) {  // lattices grow as bit sets on the first wave
+        if (tid >= kWave) return;
+        icm_team<Fmt, false>(csr, maxch, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i, g1,
+                             g2, sc, dE1, dE2, tid, kWave, true, lat_view(a, d));
+    } else {
+        icm_team<Fmt, (W > 1)>(csr, maxch, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i,
+                               g1, g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{});
+    }
     if (tid == 0) {
         a.E[ro + r1] += (int32_t)dE1;
         a.E[ro + r2] += (int32_t)dE2;

@@ -895,6 +895,12 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
 
 // Per-wave slots of the workgroup reductions in the ICM team code.
 constexpr int kIcmRedSlots = 16;
+// Clusters of graphs with at least this many neighbours per spin on average
+// (and no lattice form) grow with a whole workgroup where one is at hand:
+// their breadth-first levels are wide (an ER graph of mean degree 20 reaches
+// a few hundred sites in 4-5 levels), and one wave would issue every
+// instruction of them alone.
+constexpr int kIcmBlockDeg = 8;
 
 template <bool Block>
 __device__ __forceinline__ void team_sync() {

@@ -216,6 +216,7 @@ struct InstDesc {
     int32_t e_base;      // minimum possible energy (histogram bin 0)
     uint32_t inst_id;    // random-stream instance id
     int32_t max_deg;
+    int32_t deg_mean;    // adjacency entries per spin, rounded down
     int32_t lat_dim;     // 2 or 3: periodic lattice (bit-set cluster growth), else 0
     int32_t lat_L;
     int32_t lat_R;       // 32-bit words per lattice row (rows padded to whole words)
