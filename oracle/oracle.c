@@ -16,7 +16,7 @@
  *   or_delta_e, or_incremental_de  the invariant of src/problem.cc:64-100
  *                      (dE_i = E(flip i) - E for every i and every i,j pair).
  *   or_colour          proper colouring (no intra-class edge), 2 classes on
- *                      even tori.
+ *                      even tori and bipartite graphs (DSatur is exact there).
  *   or_beta_linspace   the reference's printed ladder in
  *                      scripts/grab_results2csv.py:31.
  *   engine sweep       exact Boltzmann distribution by enumeration (chi2) on
@@ -406,24 +406,91 @@ void or_incremental_de(const or_graph* g, int64_t* dE, const uint8_t* s, int32_t
     }
 }
 
+/* DSatur colouring (Brelaz 1979): repeatedly colour the uncoloured vertex
+ * with the most distinct colours among its neighbours (ties: larger degree,
+ * then smaller index) with the smallest colour none of its neighbours has.
+ * Every adjacency entry counts as an edge (zero weights included, as the
+ * cluster moves follow them).  A binary max-heap with lazy deletion holds
+ * (saturation, degree, vertex); the key order is total, so the result does
+ * not depend on the heap. */
+typedef struct {
+    int32_t sat, deg, v;
+} or_dsat;
+
+static int or_dsat_better(or_dsat a, or_dsat b) {
+    if (a.sat != b.sat) return a.sat > b.sat;
+    if (a.deg != b.deg) return a.deg > b.deg;
+    return a.v < b.v;
+}
+
+static void or_heap_push(or_dsat* h, int64_t* len, or_dsat e) {
+    int64_t i = (*len)++;
+    h[i] = e;
+    while (i > 0) {
+        int64_t p = (i - 1) / 2;
+        if (!or_dsat_better(h[i], h[p])) break;
+        or_dsat t = h[i];
+        h[i] = h[p];
+        h[p] = t;
+        i = p;
+    }
+}
+
+static or_dsat or_heap_pop(or_dsat* h, int64_t* len) {
+    or_dsat top = h[0];
+    h[0] = h[--(*len)];
+    int64_t i = 0;
+    for (;;) {
+        int64_t l = 2 * i + 1, r = l + 1, b = i;
+        if (l < *len && or_dsat_better(h[l], h[b])) b = l;
+        if (r < *len && or_dsat_better(h[r], h[b])) b = r;
+        if (b == i) break;
+        or_dsat t = h[i];
+        h[i] = h[b];
+        h[b] = t;
+        i = b;
+    }
+    return top;
+}
+
 int32_t or_colour(const or_graph* g, int32_t* colour) {
     int32_t n = g->n, ncls = 0;
     int32_t maxdeg = 0;
     for (int32_t i = 0; i < n; ++i)
         if (g->rowptr[i + 1] - g->rowptr[i] > maxdeg) maxdeg = g->rowptr[i + 1] - g->rowptr[i];
-    int32_t* mark = (int32_t*)malloc(((size_t)maxdeg + 2) * sizeof(int32_t));
-    for (int32_t c = 0; c < maxdeg + 2; ++c) mark[c] = -1;
+    const int32_t words = (maxdeg + 2 + 63) / 64;
+    uint64_t* nb = (uint64_t*)calloc((size_t)n * words, sizeof(uint64_t)); /* neighbour colours */
+    int32_t* sat = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+    const int64_t cap = (int64_t)n + g->rowptr[n] + 1;
+    or_dsat* heap = (or_dsat*)malloc((size_t)cap * sizeof(or_dsat));
+    int64_t len = 0;
     for (int32_t i = 0; i < n; ++i) {
-        for (int32_t p = g->rowptr[i]; p < g->rowptr[i + 1]; ++p) {
-            int32_t j = g->col[p];
-            if (j < i && colour[j] <= maxdeg) mark[colour[j]] = i;
-        }
-        int32_t c = 0;
-        while (mark[c] == i) ++c;
-        colour[i] = c;
-        if (c + 1 > ncls) ncls = c + 1;
+        colour[i] = -1;
+        or_dsat e = {0, g->rowptr[i + 1] - g->rowptr[i], i};
+        or_heap_push(heap, &len, e);
     }
-    free(mark);
+    for (int32_t k = 0; k < n; ++k) {
+        or_dsat e;
+        do e = or_heap_pop(heap, &len);
+        while (colour[e.v] >= 0 || e.sat != sat[e.v]);
+        const int32_t v = e.v;
+        int32_t c = 0;
+        while (nb[(size_t)v * words + c / 64] >> (c % 64) & 1u) ++c;
+        colour[v] = c;
+        if (c + 1 > ncls) ncls = c + 1;
+        for (int32_t p = g->rowptr[v]; p < g->rowptr[v + 1]; ++p) {
+            const int32_t j = g->col[p];
+            uint64_t* w = &nb[(size_t)j * words + c / 64];
+            if (colour[j] >= 0 || (*w >> (c % 64) & 1u)) continue;
+            *w |= (uint64_t)1 << (c % 64);
+            ++sat[j];
+            or_dsat f = {sat[j], g->rowptr[j + 1] - g->rowptr[j], j};
+            or_heap_push(heap, &len, f);
+        }
+    }
+    free(heap);
+    free(sat);
+    free(nb);
     return ncls;
 }
 

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <queue>
 #include <sstream>
 
 #include "../../include/ptx.h"
@@ -91,21 +92,48 @@ Status assemble(int32_t n, int64_t m_header, bool sparse, const std::vector<RawE
 Status Problem::finalize() {
     if (n <= 0) return Status::err(PTX_ERR_INVALID, "empty problem");
     if (n > 65536) return Status::err(PTX_ERR_UNSUPPORTED, "more than 65536 spins");
-    // greedy colouring in index order: smallest colour unused by lower-index neighbours
-    colour.assign(n, 0);
+    // DSatur colouring (Brelaz 1979): the uncoloured vertex with the most
+    // distinct neighbour colours (ties: larger degree, then smaller index) takes
+    // the smallest colour its neighbours lack.  Fewer classes mean fewer
+    // dependent phases per sweep (G022: 8 instead of 12 for greedy index
+    // order).  A max-heap with lazy deletion; the key order is total.
     max_deg = 0;
     for (int32_t i = 0; i < n; ++i) max_deg = std::max(max_deg, rowptr[i + 1] - rowptr[i]);
-    std::vector<int32_t> seen(max_deg + 2, -1);
-    ncls = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-            const int32_t j = col[q];
-            if (j < i && colour[j] <= max_deg) seen[colour[j]] = i;
+    colour.assign(n, -1);
+    {
+        struct Key {
+            int32_t sat, deg, v;
+            bool operator<(const Key& o) const {  // "worse than": the heap's top is the best key
+                if (sat != o.sat) return sat < o.sat;
+                if (deg != o.deg) return deg < o.deg;
+                return v > o.v;
+            }
+        };
+        const int32_t words = (max_deg + 2 + 63) / 64;
+        std::vector<uint64_t> nb((size_t)n * words, 0);  // neighbour colours per vertex
+        std::vector<int32_t> sat(n, 0);
+        std::priority_queue<Key> heap;
+        for (int32_t i = 0; i < n; ++i) heap.push({0, rowptr[i + 1] - rowptr[i], i});
+        ncls = 0;
+        for (int32_t k = 0; k < n; ++k) {
+            Key e;
+            do {
+                e = heap.top();
+                heap.pop();
+            } while (colour[e.v] >= 0 || e.sat != sat[e.v]);
+            const int32_t v = e.v;
+            int32_t c = 0;
+            while ((nb[(size_t)v * words + c / 64] >> (c % 64)) & 1u) ++c;
+            colour[v] = c;
+            ncls = std::max(ncls, c + 1);
+            for (int32_t q = rowptr[v]; q < rowptr[v + 1]; ++q) {
+                const int32_t j = col[q];
+                uint64_t& w = nb[(size_t)j * words + c / 64];
+                if (colour[j] >= 0 || ((w >> (c % 64)) & 1u)) continue;
+                w |= (uint64_t)1 << (c % 64);
+                heap.push({++sat[j], rowptr[j + 1] - rowptr[j], j});
+            }
         }
-        int32_t c = 0;
-        while (seen[c] == i) ++c;
-        colour[i] = c;
-        ncls = std::max(ncls, c + 1);
     }
     class_start.assign(ncls + 1, 0);
     for (int32_t i = 0; i < n; ++i) class_start[colour[i] + 1]++;

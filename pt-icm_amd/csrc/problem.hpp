@@ -1,5 +1,5 @@
 // problem.hpp -- host-side instance model: gset/.ising reader, integer CSR,
-// energy/maxcut/dE, greedy colouring and the colour-sorted position order the
+// energy/maxcut/dE, DSatur colouring and the colour-sorted position order the
 // kernels sweep in.  Restates pticm::Problem (include/problem.hh:7-98,
 // src/problem.cc:1-247) with integer weights.
 #pragma once
@@ -25,7 +25,7 @@ struct Problem {
     double char_scale = 0.0;
     // adjacency in index order (each undirected entry stored at both ends)
     std::vector<int32_t> rowptr, col, w, h;
-    // colour classes (greedy, index order) and the position order
+    // colour classes (DSatur) and the position order
     int32_t ncls = 0;
     std::vector<int32_t> colour, pos2orig, orig2pos, class_start;
     int32_t fmax = 0;
