@@ -77,7 +77,7 @@ def cpu_baseline(problem_path, betas, seconds, seed=0):
                       f"{steps} PT steps in {dt:.1f} s on 1 host core"}
 
 
-def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024):
+def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
     """Throughput of the batched engine at the scale of the batched + surrogate
     configuration: half 2D +-J tori L = 100 (10,000 spins), half ER +-1 graphs
     G(5000, 0.004); 64 inverse temperatures linear in beta over T 0.5..3.0
@@ -103,7 +103,7 @@ def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024):
     kw, trials = ptx.autotune(probs, [lad] * n_inst, seed=seed, steps=8, flags=ptx.FLAG_ICM)
     t_tune = time.perf_counter() - t0
     eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, use_graph=4, **kw)
-    eng.step(4)
+    eng.step(warmup)  # past the first steps from random states, whose cluster moves span half the lattice
     eng.sync()
     t0 = time.perf_counter()
     eng.step(steps)
@@ -114,7 +114,7 @@ def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024):
     return {"workload": f"{n_inst // 2} 2D +-J tori L=100 + {n_inst - n_inst // 2} ER +-1 G(5000, 0.004); "
                         "64 betas linear over T 0.5..3.0, 2 chains, ICM",
             "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
-            "ms_per_step": dt * 1e3 / steps, "steps": steps, "replicas": info.total_replicas,
+            "ms_per_step": dt * 1e3 / steps, "steps": steps, "warmup": warmup, "replicas": info.total_replicas,
             "icm_slots": {"torus": icm[0], "er": icm[1]},
             "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
                              "icm_waves": info.icm_waves},
@@ -190,7 +190,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0,
                     help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
-    ap.add_argument("--scale-steps", type=int, default=8,
+    ap.add_argument("--scale-steps", type=int, default=16,
                     help="timed PT steps of the 1024-instance mixed batch (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
