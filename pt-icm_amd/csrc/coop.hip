@@ -110,14 +110,14 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
 
 // Grid barrier over all workgroups of the launch, in two halves so that
 // waves other than the first can do independent work while thread 0 waits.
-// grid_wait returns false when the wait exceeded its bound (the error flag is
-// then set and every workgroup leaves the step loop).
+// Arrival is a release and the wait an acquire, both at agent scope (the
+// workgroup barriers order the other threads' accesses around thread 0's):
+// one L2 write-back per arrival and one invalidation per wait, instead of
+// both on each side.  grid_wait returns false when the wait exceeded its
+// bound (the error flag is then set and every workgroup leaves the step loop).
 __device__ __forceinline__ void grid_arrive(unsigned long long* bar) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        atomicAdd(bar, 1ull);
-    }
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err, unsigned long long target,
@@ -125,7 +125,9 @@ __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err
     if (threadIdx.x == 0) {
         int ok = 1;
         unsigned long long spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        // relaxed polls, one acquire fence once the count is reached (an
+        // acquire per poll would invalidate caches on every iteration)
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(2);
             if (++spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 atomicOr(err, 1u);
@@ -133,7 +135,7 @@ __device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *lds_flag = ok;
     }
     __syncthreads();
