@@ -149,8 +149,9 @@ struct Csr {
         if (Fmt == kFmtUnit) {
             // Padding records read the zero sentinel byte with sign 0, so every
             // gather of a chunk is issued unconditionally and none is masked.
-            // Rows of more than one chunk load up to four chunks before any
-            // gather, so their latencies overlap.
+            // Longer rows go two chunks at a time, the second only on lanes
+            // whose row has it: masked lanes issue no LDS requests, so the
+            // gathers cost the row's chunks, not the wave's longest row.
             const int nch = (d + 7) >> 3;
             int agree = 0;  // neighbours with s_j * sign(w_pj) = +1
             if (nch <= 1) {
@@ -163,24 +164,26 @@ struct Csr {
                 for (int h = 0; h < 8; ++h) agree += (int)(b[h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
                 return d > 0 ? 2 * agree - d : 0;
             }
-            for (int c0 = 0; c0 < nch; c0 += 4) {
-                uint4 v[4];
+            for (int c0 = 0; c0 < nch; c0 += 2) {
+                const bool two = c0 + 1 < nch;
+                const uint4 v0 = r[c0];
+                const uint4 v1 = two ? r[c0 + 1] : make_uint4(0u, 0u, 0u, 0u);
+                const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w};
+                const uint32_t w1[4] = {v1.x, v1.y, v1.z, v1.w};
+                uint32_t b0[8], b1[8];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = r[min(c0 + q, nch - 1)];
-                uint32_t b[4][8];
+                for (int h = 0; h < 8; ++h) b0[h] = sp[(w0[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                if (two) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t wv[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+                    for (int h = 0; h < 8; ++h) b1[h] = sp[(w1[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                } else {
 #pragma unroll
-                    for (int h = 0; h < 8; ++h) b[q][h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                    for (int h = 0; h < 8; ++h) b1[h] = 0u;  // with w1 = 0: sign 0, agree 0
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const uint32_t wv[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-                    int a = 0;
-#pragma unroll
-                    for (int h = 0; h < 8; ++h) a += (int)(b[q][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
-                    agree += c0 + q < nch ? a : 0;
+                for (int h = 0; h < 8; ++h) {
+                    agree += (int)(b0[h] ^ ((w0[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                    agree += (int)(b1[h] ^ ((w1[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
                 }
             }
             return 2 * agree - d;
