@@ -129,7 +129,7 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
 #define PTX_MODE_AUTO 0
 #define PTX_MODE_SPLIT 1 /* sweep / exchange / ICM kernels per step */
 #define PTX_MODE_FUSED 2 /* one workgroup per instance runs whole PT steps */
-#define PTX_MODE_COOP 3  /* one workgroup per replica of every instance, all resident, grid barrier per step */
+#define PTX_MODE_COOP 3  /* one workgroup per replica of every instance, all resident, one wait per step */
 
 #define PTX_CSR_AUTO 0   /* adjacency in LDS when it fits, else read through L2 */
 #define PTX_CSR_LDS 1

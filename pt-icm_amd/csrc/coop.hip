@@ -3,28 +3,27 @@
 // (cooperative launch), several PT steps per launch.
 //
 // Per PT step each workgroup sweeps its replica with all of its waves, then
-// publishes its spin words and energy to a double-buffered area, then waits
-// at one grid barrier.  After the barrier every workgroup computes the whole
-// (NN)_a exchange itself from the published energies (the computation is
-// deterministic, so every copy of the permutation is identical), and the two
-// workgroups of each ICM slot pair both grow the same cluster from the
-// published states and each keeps its own replica's half.  No second barrier
-// is needed: a buffer is rewritten only two steps later, after every
-// workgroup has passed the next barrier.
+// publishes its spin words and tagged energy to a double-buffered area, then
+// waits until every replica of its instance has published the step.  After
+// that every workgroup computes the whole (NN)_a exchange itself from the
+// published energies (the computation is deterministic, so every copy of
+// the permutation is identical), and the two workgroups of each ICM slot
+// pair both grow the same cluster from the published states and each keeps
+// its own replica's half.  No second barrier is needed: a buffer is
+// rewritten only two steps later, after every replica of the instance has
+// published the step in between.
 #include "kernels.hpp"
 
 namespace ptx {
 
 struct CoopArgs {
     uint32_t* pub_spins;        // [2][spin words of all instances], at each instance's spin offset
-    int32_t* pub_E;             // [2][replicas of all instances], at each instance's replica offset
+    unsigned long long* pub_E;  // [2][replicas of all instances]: (step + 1) << 32 | energy, at each
+                                // instance's replica offset; a slot showing this step is the arrival
     const int2* items;          // (instance, replica) of each workgroup
-    unsigned long long* bar;    // grid barrier counter (monotonic)
-    uint32_t* err;              // set to 1 when a barrier wait times out
-    unsigned long long bar_base;  // barrier arrivals before this launch
+    uint32_t* err;              // set to 1 when a wait times out
     int64_t pub_words;          // spin words of all instances (the stride of pub_spins' halves)
     int32_t pub_reps;           // replicas of all instances (the stride of pub_E's halves)
-    int32_t nblocks;
     int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
     int32_t lat_stage;          // 1: the lattice tables are copied into LDS
 };
@@ -108,35 +107,55 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
     }
 }
 
-// Grid barrier over all workgroups of the launch, in two halves so that
-// waves other than the first can do independent work while thread 0 waits.
-// Arrival is a release and the wait an acquire, both at agent scope (the
-// workgroup barriers order the other threads' accesses around thread 0's):
-// one L2 write-back per arrival and one invalidation per wait, instead of
-// both on each side.  grid_wait returns false when the wait exceeded its
-// bound (the error flag is then set and every workgroup leaves the step loop).
-__device__ __forceinline__ void grid_arrive(unsigned long long* bar) {
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+// Per-step barrier among the workgroups of one instance (other instances
+// need not wait: no data crosses instances), in two halves so that waves
+// other than the first can do independent work meanwhile.  Arrival: after
+// the workgroup has published its spin words, thread 0 stores the tagged
+// energy (step + 1) << 32 | E with release semantics at agent scope.  Wait:
+// the first wave polls the instance's slots with relaxed loads (lane s reads
+// slot s) until every one shows this step, takes the energies from the
+// values it polled, and then one acquire fence makes the published spins
+// visible.  The slots are double-buffered by step parity: a slot is
+// rewritten two steps later, after every replica of the instance has
+// published the step in between, which it does only after reading this one.
+// coop_wait returns false when the wait exceeded its bound (the error flag is
+// then set and every workgroup leaves the step loop).
+__device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t step, int32_t E) {
+    __syncthreads();  // the spin words are published
+    if (threadIdx.x == 0)
+        __hip_atomic_store(slot, ((unsigned long long)(step + 1u) << 32) | (uint32_t)E, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool grid_wait(unsigned long long* bar, uint32_t* err, unsigned long long target,
-                                          int* lds_flag) {
-    if (threadIdx.x == 0) {
+__device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int nrep, uint32_t step, uint32_t* err,
+                                          int32_t* E, int* lds_flag) {
+    const int tid = threadIdx.x;
+    if (tid < kWave) {
+        const unsigned long long want = (unsigned long long)(step + 1u);
         int ok = 1;
         unsigned long long spins = 0;
-        // relaxed polls, one acquire fence once the count is reached (an
-        // acquire per poll would invalidate caches on every iteration)
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                atomicOr(err, 1u);
-                ok = 0;
-                break;
+        for (int s0 = 0; s0 < nrep; s0 += kWave) {
+            const int s = s0 + tid;
+            unsigned long long v = 0;
+            while (true) {
+                bool ready = true;
+                if (s < nrep) {
+                    v = __hip_atomic_load(slots + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ready = (v >> 32) == want;
+                }
+                if (__ballot(!ready) == 0ull) break;
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    if (tid == 0) atomicOr(err, 1u);
+                    ok = 0;
+                    break;
+                }
             }
+            if (!ok) break;
+            if (s < nrep) E[s] = (int32_t)(uint32_t)v;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *lds_flag = ok;
+        if (tid == 0) *lds_flag = ok;
     }
     __syncthreads();
     return *lds_flag != 0;
@@ -275,7 +294,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     };
     if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[inst], wave - 1);
     // random words of the first sweep of this launch (later ones are drawn
-    // while the grid barrier is pending)
+    // while the step's wait is pending)
     uint32_t* pre_rnd = c.pre_rnd ? reinterpret_cast<uint32_t*>(smem + L.rnd) : nullptr;
     const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[inst] * (uint32_t)a.sweeps, tid, nthr);
@@ -288,7 +307,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const uint32_t step0 = a.inst_step[inst];
     // this instance's halves of the published states and energies
     uint32_t* pub_sp = c.pub_spins + d.spin_off;
-    int32_t* pub_e = c.pub_E + ro;
+    unsigned long long* pub_e = c.pub_E + ro;
     const int64_t gr = ro + r;
     uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r * nw;
     SweepState st;
@@ -321,8 +340,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         phase_done(0);
         // ---- publish and wait ----
         pack_bits(sp, nw, pub_sp + par * c.pub_words + (int64_t)r * nw, tid, nthr);
-        if (tid == 0) pub_e[(int64_t)par * c.pub_reps + r] = st.E;
-        grid_arrive(c.bar);
+        coop_arrive(pub_e + (int64_t)par * c.pub_reps + r, step, st.E);
         // while thread 0 waits: waves 1 and 2 prepare the next step's
         // exchange, and waves 1.. draw the next step's first-sweep random
         // words (this step's sweep has finished reading them)
@@ -330,11 +348,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (pre_rnd && wave > 0 && it + 1 < nsteps)
             fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
                                nthr - kWave);
-        if (!grid_wait(c.bar, c.err, c.bar_base + ((unsigned long long)it + 1ull) * (unsigned long long)c.nblocks,
-                       flag))
-            break;
-        for (int s = tid; s < nrep; s += nthr) E[s] = pub_e[(int64_t)par * c.pub_reps + s];
-        __syncthreads();
+        if (!coop_wait(pub_e + (int64_t)par * c.pub_reps, nrep, step, c.err, E, flag)) break;
         phase_done(3);
         // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
         //      wave 1 meanwhile prepares the next step's exchange ----

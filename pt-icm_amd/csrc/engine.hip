@@ -326,7 +326,6 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (coop_capacity > 0) {
         PTX_TRY(d_pub_spins.alloc((size_t)(2 * spin_words)));
         PTX_TRY(d_pub_E.alloc((size_t)(2 * total_rep)));
-        PTX_TRY(d_bar.alloc(1));
         PTX_TRY(d_err.alloc(1));
     }
     last_mode = mode;
@@ -625,12 +624,9 @@ Status Engine::launch_coop(int nsteps) {
     c.pub_spins = d_pub_spins.p;
     c.pub_E = d_pub_E.p;
     c.items = d_coop_items.p;
-    c.bar = d_bar.p;
     c.err = d_err.p;
-    c.bar_base = bar_arrivals;
     c.pub_words = spin_words;
     c.pub_reps = (int32_t)total_rep;
-    c.nblocks = (int32_t)coop_items.size();
     c.pre_rnd = cf.pre_rnd;
     c.lat_stage = cf.lat_stage;
     const dim3 grid((unsigned)coop_items.size()), block((unsigned)(coop_W * kWave));
@@ -649,7 +645,6 @@ Status Engine::launch_coop(int nsteps) {
         PTX_TRY(hip_check(hipEventRecord(t.b, stream), "hipEventRecord"));
         pending.push_back(t);
     }
-    bar_arrivals += (unsigned long long)nsteps * coop_items.size();
     return Status::ok();
 }
 
@@ -827,7 +822,7 @@ Status Engine::sync() {
     if (d_err.p) {
         uint32_t err = 0;
         PTX_TRY(hip_check(hipMemcpy(&err, d_err.p, sizeof(err), hipMemcpyDeviceToHost), "read error flag"));
-        if (err) return Status::err(PTX_ERR_DEVICE, "cooperative grid barrier timed out");
+        if (err) return Status::err(PTX_ERR_DEVICE, "cooperative step wait timed out");
     }
     return Status::ok();
 }

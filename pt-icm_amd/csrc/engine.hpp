@@ -95,11 +95,9 @@ struct Engine {
     // cooperative launches (cooperative mode, or small batches of a split-mode
     // engine with small_batch_coop): one workgroup per active replica
     DevBuf<uint32_t> d_pub_spins, d_err;
-    DevBuf<int32_t> d_pub_E;
-    DevBuf<unsigned long long> d_bar;
+    DevBuf<unsigned long long> d_pub_E;  // tagged energies (step + 1) << 32 | E, two per replica
     DevBuf<int2> d_coop_items;
     std::vector<int2> coop_items;  // (instance, replica) per workgroup, active instances
-    unsigned long long bar_arrivals = 0;  // grid-barrier arrivals of all cooperative launches so far
     int64_t spin_words = 0;      // spin words of all instances
     int64_t active_reps = 0;     // replicas of the active instances
     int64_t coop_capacity = 0;   // co-resident cooperative workgroups (0: no cooperative launches)
@@ -109,7 +107,7 @@ struct Engine {
     int32_t spl = 1;             // positions per lane (cooperative mode)
     struct CoopForm {            // a cooperative launch form: adjacency in LDS (0) or read through L2 (1)
         int32_t lds = 0;         // dynamic LDS bytes per workgroup
-        int32_t pre_rnd = 0;     // next sweep's random words drawn during the grid barrier
+        int32_t pre_rnd = 0;     // next sweep's random words drawn during the step wait
         int32_t lat_stage = 0;   // lattice tables copied into LDS
         int64_t capacity = 0;    // co-resident workgroups (0: form not available)
     };
