@@ -58,3 +58,42 @@ def test_measured_swap_rates_match_multihistogram_prediction(ptx):
     meas = suc / att
     print(np.round(pred, 3), np.round(meas, 3), flush=True)
     assert np.all(np.abs(meas - pred) < 0.02), (pred, meas)
+
+
+def test_ea_l6_ground_state_hit_rates_follow_the_reference_curve(ptx):
+    """3D +-J EA L = 6 with the reference's settings (Nt = 10, beta 0.1..1.6,
+    ICM): the fraction of runs whose best energy is the ground state after
+    20, 54, 148, 403, 1096 and 2980 PT steps, against the reference's
+    measured curve (data/pt_icm_ea_l6.csv: 0.12, 0.34, 0.78, 0.92, 0.96,
+    1.00 for Nt = 10).  Its instances are not available, so this takes 10
+    fresh instances x 20 runs (ground states from 20 x 20,000-step runs
+    each).  Measured here: 0.375, 0.685, 0.885, 0.92, 0.95, 0.99 -- the same
+    plateau from 403 steps on, reached sooner (colour-class sweeps and the
+    cluster moves on the four coldest slots; the reference's early points
+    rest on 50 runs of other instances).  Asserted: never more than 0.1
+    below the reference, and within 0.1 of it from 403 steps on."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
+    import instances
+    ref = {20: 0.12, 54: 0.34, 148: 0.78, 403: 0.92, 1096: 0.96, 2980: 1.00}
+    n_inst, runs = 10, 20
+    coos = [instances.ea_3d(6, seed=500 + k) for k in range(n_inst)]
+    probs = [ptx.Problem.from_coo(*c) for c in coos for _ in range(runs)]
+    lad = ptx.beta_linspace(0.1, 1.6, 10, True)
+    long = ptx.Engine(probs, [lad] * len(probs), seed=77, flags=ptx.FLAG_ICM)
+    long.step(20000)
+    gs = long.min_energies().reshape(n_inst, runs).min(axis=1)
+    eng = ptx.Engine(probs, [lad] * len(probs), seed=78, flags=ptx.FLAG_ICM)
+    eng.step(max(ref))
+    first = np.full(len(probs), np.iinfo(np.int64).max)
+    for i in range(len(probs)):
+        be, bs, _ = eng.best(i, with_state=False)
+        hit = be == gs[i // runs]
+        if hit.any():
+            first[i] = bs[hit].min()
+    for steps, want in ref.items():
+        got = float(np.mean(first < steps))
+        print(steps, round(got, 3), want, flush=True)
+        assert got >= want - 0.1, (steps, got, want)
+        if steps >= 403:
+            assert abs(got - want) <= 0.1, (steps, got, want)
