@@ -194,7 +194,7 @@ def main():
                     help="timed PT steps of the 1024-instance mixed batch (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
-    ap.add_argument("--ttt-plain", type=int, default=8,
+    ap.add_argument("--ttt-plain", type=int, default=16,
                     help="instances of the plain path timed for time-to-target (0 = skip the time-to-target part)")
     ap.add_argument("--ttt-targets", default=os.path.join(REPO, "pt-icm_amd", "data", "ttt_targets.json"))
     args = ap.parse_args()

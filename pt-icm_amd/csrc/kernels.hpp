@@ -779,6 +779,87 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
         }
         if (own) Vo[lane] = v;
         wave_sync();
+    } else if (lv.dim == 2 && R <= 4 && rows <= 2 * kWave) {
+        // 2D, rows of up to 4 words and up to 128 rows: a lane holds rows
+        // lane and lane + 64 whole, in registers.  A row closes along x in
+        // registers (shifts carried across its words, wrapped at bit L-1);
+        // rows y +- 1 come by lane shuffles (from either register slot).
+        uint32_t d[2][4], v[2][4];
+        int up[2], dn[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int row = lane + kWave * j;
+            const bool own = row < rows;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool in = own && k < R;
+                d[j][k] = in ? Do[row * R + k] : 0u;
+                v[j][k] = in ? Vo[row * R + k] : 0u;
+            }
+            up[j] = own ? (row + 1 == rows ? 0 : row + 1) : 0;
+            dn[j] = own ? (row == 0 ? rows - 1 : row - 1) : 0;
+        }
+        // bit x of the row takes bits x - 1 and x + 1 (wrapped) where d allows,
+        // until the row no longer changes
+        auto close_row = [&](uint32_t* rv, const uint32_t* rd) {
+            while (true) {
+                uint32_t nx[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k >= R) {
+                        nx[k] = 0u;
+                        continue;
+                    }
+                    const uint32_t cin = k > 0 ? (rv[k - 1] >> 31) : ((rv[R - 1] >> (lb - 1)) & 1u);
+                    const uint32_t cout = k < R - 1 ? ((rv[k + 1] & 1u) << 31) : ((rv[0] & 1u) << (lb - 1));
+                    uint32_t x = (rv[k] << 1) | cin | (rv[k] >> 1) | cout;
+                    if (k == R - 1) x &= lastmask;
+                    nx[k] = rv[k] | (x & rd[k]);
+                }
+                bool same = true;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    same &= nx[k] == rv[k];
+                    rv[k] = nx[k];
+                }
+                if (same) return;
+            }
+        };
+        while (true) {
+            bool grew = false;
+            uint32_t nv[2][4];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // the same word of rows up/dn, from whichever slot holds them
+                    const uint32_t u0 = __shfl(v[0][k], up[j] & (kWave - 1), kWave);
+                    const uint32_t u1 = __shfl(v[1][k], up[j] & (kWave - 1), kWave);
+                    const uint32_t w0 = __shfl(v[0][k], dn[j] & (kWave - 1), kWave);
+                    const uint32_t w1 = __shfl(v[1][k], dn[j] & (kWave - 1), kWave);
+                    const uint32_t ny = (up[j] >= kWave ? u1 : u0) | (dn[j] >= kWave ? w1 : w0);
+                    nv[j][k] = v[j][k] | (ny & d[j][k]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                close_row(nv[j], d[j]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    grew |= nv[j][k] != v[j][k];
+                    v[j][k] = nv[j][k];
+                }
+            }
+            if (__ballot(grew) == 0ull) break;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int row = lane + kWave * j;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (row < rows && k < R) Vo[row * R + k] = v[j][k];
+        }
+        wave_sync();
     } else if (NW <= kLatSlots * kWave) {
         // up to kLatSlots words per lane kept in registers; each round reads every
         // neighbour word first, then writes the grown words
