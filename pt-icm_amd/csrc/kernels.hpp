@@ -111,17 +111,6 @@ struct Csr {
     static constexpr int kPerChunk = Fmt == kFmtUnit ? 8 : 4;
     __device__ __forceinline__ int deg(int p) const { return (int)(info[p] & (uint32_t)kMaxDegree); }
     __device__ __forceinline__ const uint8_t* row(int p) const { return base + (size_t)(info[p] >> 9) * 16u; }
-    __device__ __forceinline__ void entry(const uint8_t* row, int j, int& c, int& w) const {
-        if (Fmt == kFmtUnit) {
-            const uint32_t r = reinterpret_cast<const uint16_t*>(row)[j];
-            c = (int)(r & 0x7FFFu);
-            w = (r & 0x8000u) ? -1 : 1;
-        } else {
-            const uint32_t r = reinterpret_cast<const uint32_t*>(row)[j];
-            c = (int)(r & 0xFFFFu);
-            w = (int32_t)r >> 16;
-        }
-    }
     // Records of 16-byte chunk ch of a row (padding records read position 0).
     __device__ __forceinline__ void chunk(const uint8_t* row, int ch, int* c, int* w) const {
         const uint4 v = reinterpret_cast<const uint4*>(row)[ch];
