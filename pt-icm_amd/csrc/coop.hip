@@ -208,7 +208,10 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     return L;
 }
 
-template <int Fmt, int W, int SPL, int G>
+// IC = 0: no instance of the batch has cluster-move slots, and the kernel
+// carries no cluster-growth code (whose registers would otherwise be
+// allocated around the sweep as well).
+template <int Fmt, int W, int SPL, int G, int IC>
 __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int nsteps) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         __syncthreads();
         phase_done(1);
         // ---- cluster move of this replica's slot pair (both partners compute it) ----
-        if (d.nc > 0) {
+        if (IC && d.nc > 0) {
             const int t = r2s[r] % nt;
             if (t >= nt - d.nc) {
                 const int r1 = s2r[t], r2 = s2r[t + nt];
@@ -432,13 +435,15 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     }
 }
 
-#define PTX_COOP_INST(F)                                         \
-    template __global__ void k_coop<F, 4, 1, 0>(DevArgs, CoopArgs, int); \
-    template __global__ void k_coop<F, 8, 1, 0>(DevArgs, CoopArgs, int); \
-    template __global__ void k_coop<F, 4, 4, 0>(DevArgs, CoopArgs, int); \
-    template __global__ void k_coop<F, 8, 4, 0>(DevArgs, CoopArgs, int); \
-    template __global__ void k_coop<F, 4, 1, 1>(DevArgs, CoopArgs, int);
-PTX_COOP_INST(kFmtWide)
-PTX_COOP_INST(kFmtUnit)
+#define PTX_COOP_INST(F, IC)                                          \
+    template __global__ void k_coop<F, 4, 1, 0, IC>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 8, 1, 0, IC>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 4, 4, 0, IC>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 8, 4, 0, IC>(DevArgs, CoopArgs, int); \
+    template __global__ void k_coop<F, 4, 1, 1, IC>(DevArgs, CoopArgs, int);
+PTX_COOP_INST(kFmtWide, 0)
+PTX_COOP_INST(kFmtWide, 1)
+PTX_COOP_INST(kFmtUnit, 0)
+PTX_COOP_INST(kFmtUnit, 1)
 
 }  // namespace ptx

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "kernels.hip"
 #include "coop.hip"
@@ -128,6 +129,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             nc = cfg.nc >= 0 ? std::min(cfg.nc, nt) : icm_count(ladders[i].data(), nt, P.char_scale);
         }
         d.nc = nc;
+        if (nc > 0) coop_icm = true;
         d.inst_id = (uint32_t)i;
         // adjacency rows by position, each padded to 16 bytes (at least one
         // chunk); padding records name the sentinel position 32 * nw
@@ -548,20 +550,23 @@ Status Engine::launch_fused(int nsteps) {
 
 // The cooperative kernel of the engine's format, width and positions per
 // lane, with the adjacency in LDS (g = 0) or read through L2 (g = 1: four
-// waves, one position per lane).
+// waves, one position per lane), and with cluster moves compiled in when
+// any instance has cluster-move slots.
 const void* Engine::coop_kernel(int g) const {
-    if (fmt == kFmtUnit) {
-        if (g) return reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 1, 1>);
-        if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtUnit, 8, 4, 0>)
-                                         : reinterpret_cast<const void*>(k_coop<kFmtUnit, 8, 1, 0>);
-        return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 4, 0>)
-                        : reinterpret_cast<const void*>(k_coop<kFmtUnit, 4, 1, 0>);
-    }
-    if (g) return reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 1, 1>);
-    if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtWide, 8, 4, 0>)
-                                     : reinterpret_cast<const void*>(k_coop<kFmtWide, 8, 1, 0>);
-    return spl == 4 ? reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 4, 0>)
-                    : reinterpret_cast<const void*>(k_coop<kFmtWide, 4, 1, 0>);
+    auto pick = [&](auto fmt_tag, auto ic_tag) -> const void* {
+        constexpr int F = decltype(fmt_tag)::value, I = decltype(ic_tag)::value;
+        if (g) return reinterpret_cast<const void*>(k_coop<F, 4, 1, 1, I>);
+        if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<F, 8, 4, 0, I>)
+                                         : reinterpret_cast<const void*>(k_coop<F, 8, 1, 0, I>);
+        return spl == 4 ? reinterpret_cast<const void*>(k_coop<F, 4, 4, 0, I>)
+                        : reinterpret_cast<const void*>(k_coop<F, 4, 1, 0, I>);
+    };
+    using U = std::integral_constant<int, kFmtUnit>;
+    using Wd = std::integral_constant<int, kFmtWide>;
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    if (fmt == kFmtUnit) return coop_icm ? pick(U{}, I1{}) : pick(U{}, I0{});
+    return coop_icm ? pick(Wd{}, I1{}) : pick(Wd{}, I0{});
 }
 
 // Sizes the cooperative launches, for the adjacency in LDS and (with four

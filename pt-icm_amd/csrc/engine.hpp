@@ -114,6 +114,7 @@ struct Engine {
     CoopForm coop_form[2];
     int32_t coop_lds = 0;        // LDS bytes of the latest cooperative launch
     int32_t coop_last_form = 0;  // form of the latest cooperative launch
+    bool coop_icm = false;       // some instance has cluster-move slots (kernel with cluster moves)
     bool phase_on = false;
     // fused mode with use_graph = G > 0: G launches of K steps captured once
     // and replayed (the kernels read their step counter from device memory)
