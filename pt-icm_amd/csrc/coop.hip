@@ -63,28 +63,60 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
         if (SPL == 1) {
-            for (int p = cs + gl; p < ce; p += nthr) {
-                const int f = csr.field(p, sp) + (has_field ? hf[p] : 0);
-                const int dE = sp[p] ? -2 * f : 2 * f;
-                bool accept = dE <= 0;
-                if (!accept) {
-                    const uint32_t u = pre ? pre[p]
-                                           : pick(philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1),
-                                                  (uint32_t)p);
-                    accept = u < thr_row[(dE >> 1) - 1];
+            // T lanes per position when the class has at most nthr / T of
+            // them (T = 2, 4): each lane gathers every T-th chunk of the row
+            // and a quad permutation sums the parts, so short classes use
+            // every wave and each lane's gather chain is shorter.
+            auto update = [&](auto tc) {
+                constexpr int T = decltype(tc)::value;
+                const int sub = gl & (T - 1);
+                for (int p = cs + gl / T; p < ce; p += nthr / T) {
+                    // the position's own byte and random word are read before
+                    // the gathers, off the decision's chain of LDS reads
+                    const uint32_t b = sp[p];
+                    const uint32_t u0 = pre ? pre[p] : 0u;
+                    int f;
+                    if (T == 1) {
+                        f = csr.field(p, sp);
+                    } else {
+                        int d;
+                        const int part = group_sum_i32<T>(csr.template field_part<T>(p, sp, sub, d));
+                        f = Fmt == kFmtUnit ? 2 * part - d : part;
+                    }
+                    f += has_field ? hf[p] : 0;
+                    bool took = false;
+                    if (sub == 0) {
+                        const int dE = b ? -2 * f : 2 * f;
+                        bool accept = dE <= 0;
+                        if (!accept) {
+                            const uint32_t u =
+                                pre ? u0
+                                    : pick(philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1),
+                                           (uint32_t)p);
+                            accept = u < thr_row[(dE >> 1) - 1];
+                        }
+                        if (accept) {
+                            sp[p] = b ^ 1u;
+                            dsum += dE;
+                            took = true;
+                        }
+                    }
+                    nacc += (int)__popcll(__ballot(took));  // wave-uniform count, no reduction needed
                 }
-                if (accept) {
-                    sp[p] ^= 1u;
-                    dsum += dE;
-                    ++nacc;
-                }
-            }
+            };
+            const int csz = ce - cs;
+            if (4 * csz <= nthr)
+                update(std::integral_constant<int, 4>{});
+            else if (2 * csz <= nthr)
+                update(std::integral_constant<int, 2>{});
+            else
+                update(std::integral_constant<int, 1>{});
         } else {
             for (int g = (cs >> 2) + gl; 4 * g < ce; g += nthr)
                 group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc, pre);
         }
         dsum = wave_sum_i32(dsum);
-        nacc = wave_sum_i32(nacc);
+        if (SPL != 1) nacc = wave_sum_i32(nacc);
         int* a = acc + 2 * kMaxCoopWaves * (int)(cc & 1u);
         if (lane == 0) {
             a[2 * wv] = dsum;
@@ -115,18 +147,23 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
 // the first wave polls the instance's slots with relaxed loads (lane s reads
 // slot s) until every one shows this step, takes the energies from the
 // values it polled, and then one acquire fence makes the published spins
-// visible.  The slots are double-buffered by step parity: a slot is
+// visible.  Without cluster moves no workgroup reads another's spins, and
+// the 64-bit tagged word carries the energy itself: REL = ACQ = false then
+// store the slot relaxed, with no barrier before it and no fence after the
+// wait.  The slots are double-buffered by step parity: a slot is
 // rewritten two steps later, after every replica of the instance has
 // published the step in between, which it does only after reading this one.
 // coop_wait returns false when the wait exceeded its bound (the error flag is
 // then set and every workgroup leaves the step loop).
+template <bool REL>
 __device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t step, int32_t E) {
-    __syncthreads();  // the spin words are published
+    if (REL) __syncthreads();  // the spin words are published
     if (threadIdx.x == 0)
-        __hip_atomic_store(slot, ((unsigned long long)(step + 1u) << 32) | (uint32_t)E, __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(slot, ((unsigned long long)(step + 1u) << 32) | (uint32_t)E,
+                           REL ? __ATOMIC_RELEASE : __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <bool ACQ>
 __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int nrep, uint32_t step, uint32_t* err,
                                           int32_t* E, int* lds_flag) {
     const int tid = threadIdx.x;
@@ -144,8 +181,10 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
                     ready = (v >> 32) == want;
                 }
                 if (__ballot(!ready) == 0ull) break;
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __builtin_amdgcn_s_sleep(1);
+                // the error flag and the bound every 32 polls (a poll is one memory round trip)
+                if ((++spins & 31u) == 0 &&
+                    (spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                     if (tid == 0) atomicOr(err, 1u);
                     ok = 0;
                     break;
@@ -154,7 +193,7 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
             if (!ok) break;
             if (s < nrep) E[s] = (int32_t)(uint32_t)v;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (ACQ) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (tid == 0) *lds_flag = ok;
     }
     __syncthreads();
@@ -162,8 +201,8 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
 }
 
 struct CoopLayout {
-    int info, adj, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, red, lat_tab,
-        acc, flag, rnd, total;
+    int info, adj, cls, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, red,
+        lat_tab, acc, flag, rnd, total;
 };
 
 // pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
@@ -171,8 +210,8 @@ struct CoopLayout {
 // lat_stage = 1 also a copy of the lattice tables (wpos, obit); csr_global
 // = 1 leaves the adjacency in global memory (read through L2), so that more
 // workgroups fit a CU.
-__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep, int nt, int fmax, int pre_rnd,
-                                                  int lat_words, int lat_stage, int csr_global) {
+__host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int ncls, int nrep, int nt, int fmax,
+                                                  int pre_rnd, int lat_words, int lat_stage, int csr_global) {
     CoopLayout L;
     int o = 0;
     auto take = [&](int bytes) {
@@ -180,8 +219,11 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
         o += (bytes + 15) & ~15;
         return at;
     };
+    const int nw = (n + 31) / 32;
+    L.sp = take(spin_bytes(nw));  // at LDS address 0: a gather's address is the record itself
     L.info = take(csr_global ? 0 : 4 * n);
     L.adj = take(csr_global ? 0 : 16 * (adj_len16 > 0 ? adj_len16 : 1));
+    L.cls = take(4 * (ncls + 1));
     L.thr = take(4 * nt * (fmax > 0 ? fmax : 1));
     L.beta = take(8 * nrep);
     L.logu = take(8 * nrep);
@@ -195,8 +237,6 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int nrep
     L.tpos = take(4 * nrep);
     L.level = take(4 * nrep);
     L.pre = take(2 * kWave * (int)sizeof(ExchPre));  // prepared exchange of the next step, two buffers
-    const int nw = (n + 31) / 32;
-    L.sp = take(spin_bytes(nw));
     L.icm = take(icm_scratch_bytes(n));
     L.lat = take(8 * lat_words);
     L.red = take(8 * kIcmRedSlots);
@@ -221,7 +261,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const InstDesc d = a.desc[inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
     const int r = item.y;  // this workgroup's replica of its instance
-    const CoopLayout L = coop_layout(n, (int)d.adj_len16, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage, G);
+    const CoopLayout L =
+        coop_layout(n, (int)d.adj_len16, d.ncls, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage, G);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -258,6 +299,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         for (int q = tid; q < (int)d.adj_len16; q += nthr) dst[q] = src[q];
     }
     for (int m = tid; m < nt * d.fmax; m += nthr) thr[m] = a.thr[d.thr_off + m];
+    int32_t* cls = reinterpret_cast<int32_t*>(smem + L.cls);  // class bounds, read at every class
+    for (int m = tid; m <= d.ncls; m += nthr) cls[m] = a.cls[d.cls_off + m];
     LatView lv = lat_view(a, d);
     if (c.lat_stage && d.lat_dim > 0) {  // the tables of this lattice shape, read by every cluster move
         uint32_t* tab = reinterpret_cast<uint32_t*>(smem + L.lat_tab);
@@ -302,9 +345,9 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[inst] * (uint32_t)a.sweeps, tid, nthr);
     __syncthreads();
-    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16} : Csr<Fmt>{info, smem + L.adj};
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg}
+                           : Csr<Fmt>{info, smem + L.adj, d.neg};
     const int32_t* hf = a.hfield + d.pos_off;
-    const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
     const uint32_t step0 = a.inst_step[inst];
@@ -342,8 +385,11 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         }
         phase_done(0);
         // ---- publish and wait ----
-        pack_bits(sp, nw, pub_sp + par * c.pub_words + (int64_t)r * nw, tid, nthr);
-        coop_arrive(pub_e + (int64_t)par * c.pub_reps + r, step, st.E);
+        // the published states are read only by cluster moves; without them
+        // the tagged energy slot is all a replica publishes (a relaxed
+        // agent-scope store: no release, no acquire after the wait)
+        if (IC) pack_bits(sp, nw, pub_sp + par * c.pub_words + (int64_t)r * nw, tid, nthr);
+        coop_arrive<IC != 0>(pub_e + (int64_t)par * c.pub_reps + r, step, st.E);
         // while thread 0 waits: waves 1 and 2 prepare the next step's
         // exchange, and waves 1.. draw the next step's first-sweep random
         // words (this step's sweep has finished reading them)
@@ -351,7 +397,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (pre_rnd && wave > 0 && it + 1 < nsteps)
             fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
                                nthr - kWave);
-        if (!coop_wait(pub_e + (int64_t)par * c.pub_reps, nrep, step, c.err, E, flag)) break;
+        if (!coop_wait<IC != 0>(pub_e + (int64_t)par * c.pub_reps, nrep, step, c.err, E, flag)) break;
         phase_done(3);
         // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
         //      wave 1 meanwhile prepares the next step's exchange ----

@@ -122,6 +122,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.ncls = P.ncls;
         d.fmax = P.fmax;
         d.has_field = P.has_field ? 1 : 0;
+        d.neg = std::any_of(P.w.begin(), P.w.end(), [](int32_t w) { return w < 0; }) ? 1 : 0;
         d.max_deg = P.max_deg;
         d.deg_mean = (int32_t)(P.nnz() / std::max(P.n, 1));
         int32_t nc = 0;
@@ -590,8 +591,9 @@ Status Engine::setup_coop(bool required) {
         auto max_total = [&](int pre, int stage) {
             int mx = 0;
             for (const InstDesc& d : desc)
-                mx = std::max(mx, coop_layout(d.n, (int)d.adj_len16, d.nrep, d.nt, d.fmax, pre, d.lat_words, stage, g)
-                                      .total);
+                mx = std::max(
+                    mx, coop_layout(d.n, (int)d.adj_len16, d.ncls, d.nrep, d.nt, d.fmax, pre, d.lat_words, stage, g)
+                            .total);
             return mx;
         };
         cf.pre_rnd = 1;

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
     wave_sync();
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg};
     const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     for (int sw = 0; sw < a.sweeps; ++sw) {
         const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
@@ -315,8 +315,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
         succ[s] = 0;
     }
     __syncthreads();
-    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16}
-                           : Csr<Fmt>{info, smem + L.adj};
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg}
+                           : Csr<Fmt>{info, smem + L.adj, d.neg};
     const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);

@@ -81,6 +81,15 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
     v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
     return __builtin_amdgcn_readlane(v, 63);
 }
+// Sum over aligned groups of T lanes (T = 1, 2 or 4) of a quad with DPP quad
+// permutations; every lane of a group gets the group's sum.  The lanes of a
+// group must be active together.
+template <int T>
+__device__ __forceinline__ int group_sum_i32(int v) {
+    if (T >= 2) v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    if (T >= 4) v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    return v;
+}
 __device__ __forceinline__ long long wave_sum_i64(long long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
@@ -108,6 +117,7 @@ template <int Fmt>
 struct Csr {
     const uint32_t* info;
     const uint8_t* base;
+    int neg = 1;  // 0: no coupling is negative (unit format: every sign bit is 0, agree = up neighbours)
     static constexpr int kPerChunk = Fmt == kFmtUnit ? 8 : 4;
     __device__ __forceinline__ int deg(int p) const { return (int)(info[p] & (uint32_t)kMaxDegree); }
     __device__ __forceinline__ const uint8_t* row(int p) const { return base + (size_t)(info[p] >> 9) * 16u; }
@@ -130,50 +140,57 @@ struct Csr {
             }
         }
     }
+    // Neighbour bytes of up to R chunks of a row at once: chunk c0 + k * step
+    // (k < R) when below nch, else nothing (masked lanes issue no LDS
+    // request).  NEG: records carry a sign bit (agree = byte ^ sign), else a
+    // record is the neighbour's position itself.  Returns the number of
+    // agreeing neighbours; padding records read the zero sentinel byte with
+    // sign 0 and add nothing.
+    template <int R, bool NEG>
+    __device__ __forceinline__ int agree_chunks(const uint4* r, int c0, int step, int nch, const uint8_t* sp) const {
+        uint4 v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = c0 + k * step < nch ? r[c0 + k * step] : make_uint4(0u, 0u, 0u, 0u);
+        uint32_t b[R][8];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            if (c0 + k * step < nch) {
+#pragma unroll
+                for (int h = 0; h < 8; ++h)
+                    b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu]
+                                  : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
+            } else {
+#pragma unroll
+                for (int h = 0; h < 8; ++h) b[k][h] = 0u;
+            }
+        }
+        int agree = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                agree += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u)) : (int)b[k][h];
+        }
+        return agree;
+    }
     // sum_j w_pj s_j with s = 2b - 1, from byte spins
     __device__ __forceinline__ int field(int p, const uint8_t* sp) const {
         const uint32_t ri = info[p];
         const int d = (int)(ri & (uint32_t)kMaxDegree);
         const uint4* r = reinterpret_cast<const uint4*>(base + (size_t)(ri >> 9) * 16u);
         if (Fmt == kFmtUnit) {
-            // Padding records read the zero sentinel byte with sign 0, so every
-            // gather of a chunk is issued unconditionally and none is masked.
-            // Longer rows go two chunks at a time, the second only on lanes
-            // whose row has it: masked lanes issue no LDS requests, so the
+            // Rows of up to four chunks (32 records) load every chunk and
+            // issue every gather in one round, longer rows four chunks per
+            // round; a lane's chunks beyond its row are masked, so the
             // gathers cost the row's chunks, not the wave's longest row.
             const int nch = (d + 7) >> 3;
             int agree = 0;  // neighbours with s_j * sign(w_pj) = +1
-            if (nch <= 1) {
-                const uint4 v = r[0];
-                const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
-                uint32_t b[8];
-#pragma unroll
-                for (int h = 0; h < 8; ++h) b[h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
-#pragma unroll
-                for (int h = 0; h < 8; ++h) agree += (int)(b[h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
-                return d > 0 ? 2 * agree - d : 0;
-            }
-            for (int c0 = 0; c0 < nch; c0 += 2) {
-                const bool two = c0 + 1 < nch;
-                const uint4 v0 = r[c0];
-                const uint4 v1 = two ? r[c0 + 1] : make_uint4(0u, 0u, 0u, 0u);
-                const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w};
-                const uint32_t w1[4] = {v1.x, v1.y, v1.z, v1.w};
-                uint32_t b0[8], b1[8];
-#pragma unroll
-                for (int h = 0; h < 8; ++h) b0[h] = sp[(w0[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
-                if (two) {
-#pragma unroll
-                    for (int h = 0; h < 8; ++h) b1[h] = sp[(w1[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
-                } else {
-#pragma unroll
-                    for (int h = 0; h < 8; ++h) b1[h] = 0u;  // with w1 = 0: sign 0, agree 0
-                }
-#pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    agree += (int)(b0[h] ^ ((w0[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
-                    agree += (int)(b1[h] ^ ((w1[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
-                }
+            if (neg) {
+                for (int c0 = 0; c0 < nch; c0 += 4) agree += agree_chunks<4, true>(r, c0, 1, nch, sp);
+            } else {
+                for (int c0 = 0; c0 < nch; c0 += 4) agree += agree_chunks<4, false>(r, c0, 1, nch, sp);
             }
             return 2 * agree - d;
         }
@@ -188,6 +205,39 @@ struct Csr {
             }
         }
         return f;
+    }
+    // Part of the local field of p from the row's chunks sub, sub + T, ...
+    // (lane sub of T lanes sharing the position): unit format the number of
+    // agreeing neighbours (field = 2 * total - deg), wide format the partial
+    // sum of w_pj s_j.  Padding records add nothing to either.
+    template <int T>
+    __device__ __forceinline__ int field_part(int p, const uint8_t* sp, int sub, int& deg) const {
+        const uint32_t ri = info[p];
+        const int d = (int)(ri & (uint32_t)kMaxDegree);
+        deg = d;
+        const uint4* r = reinterpret_cast<const uint4*>(base + (size_t)(ri >> 9) * 16u);
+        int acc = 0;
+        if (Fmt == kFmtUnit) {
+            // 4 / T chunks per lane per round: one round for rows of up to 32 records
+            constexpr int R = T >= 4 ? 1 : 4 / T;
+            const int nch = (d + 7) >> 3;
+            if (neg) {
+                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks<R, true>(r, c0, T, nch, sp);
+            } else {
+                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks<R, false>(r, c0, T, nch, sp);
+            }
+            return acc;
+        }
+        for (int c = sub; kPerChunk * c < d; c += T) {
+            const uint4 v = r[c];
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const int w = (int32_t)wv[h] >> 16;  // 0 on padding records
+                acc += sp[wv[h] & 0xFFFFu] ? w : -w;
+            }
+        }
+        return acc;
     }
     // Local fields of the four consecutive positions p0..p0+3 (positions
     // outside [cs, ce) get 0).  The row words, chunks and neighbour bytes of
@@ -227,8 +277,14 @@ struct Csr {
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
                     int a = 0;
+                    if (neg) {
 #pragma unroll
-                    for (int h = 0; h < 8; ++h) a += (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                        for (int h = 0; h < 8; ++h)
+                            a += (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
+                    } else {
+#pragma unroll
+                        for (int h = 0; h < 8; ++h) a += (int)b[k][h];
+                    }
                     f[k] += ch < nch[k] ? a : 0;
                 }
             } else {
