@@ -244,6 +244,14 @@ struct Csr {
     // all four are loaded before any is used, so their latencies overlap;
     // padding records read the zero sentinel byte and need no mask.
     __device__ __forceinline__ void group_field(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
+        // the sign decision hoisted out of the loops: one loop body per case
+        if (Fmt == kFmtUnit && !neg)
+            group_field_t<false>(p0, cs, ce, sp, f);
+        else
+            group_field_t<true>(p0, cs, ce, sp, f);
+    }
+    template <bool NEG>
+    __device__ __forceinline__ void group_field_t(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
         uint32_t ri[4];
         int d[4], nch[4];
         int mx = 0;
@@ -271,20 +279,17 @@ struct Csr {
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
 #pragma unroll
-                    for (int h = 0; h < 8; ++h) b[k][h] = sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu];
+                    for (int h = 0; h < 8; ++h)
+                        b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu]
+                                      : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
                 }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
                     int a = 0;
-                    if (neg) {
 #pragma unroll
-                        for (int h = 0; h < 8; ++h)
-                            a += (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u));
-                    } else {
-#pragma unroll
-                        for (int h = 0; h < 8; ++h) a += (int)b[k][h];
-                    }
+                    for (int h = 0; h < 8; ++h)
+                        a += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u)) : (int)b[k][h];
                     f[k] += ch < nch[k] ? a : 0;
                 }
             } else {
