@@ -494,6 +494,17 @@ struct ExchPre {
     double lu[2];
 };
 
+// The value of lane + 1 (lane_next) or lane - 1 (lane_prev) by DPP wave
+// shifts on the VALU (no LDS crossbar round trip); a lane without such a
+// neighbour keeps its own value, as with __shfl_down / __shfl_up by 1.  All
+// 64 lanes must be active.
+__device__ __forceinline__ int lane_next(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+__device__ __forceinline__ int lane_prev(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+
 // only_half >= 0 prepares that half alone (the other keeps tpos -1): two
 // waves can prepare the two halves at once.
 __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, uint64_t seed, uint32_t tag_base,
@@ -533,8 +544,8 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
         // dependency levels: 1 + the highest level of an earlier attempt on k-1 or k+1
         int level = 0;
         while (true) {
-            const int tl = __shfl_up(tpos, 1, kWave), ll = __shfl_up(level, 1, kWave);
-            const int tr = __shfl_down(tpos, 1, kWave), lr = __shfl_down(level, 1, kWave);
+            const int tl = lane_prev(tpos), ll = lane_prev(level);
+            const int tr = lane_next(tpos), lr = lane_next(level);
             int lv = 0;
             if (tpos >= 0) {
                 lv = 1;
@@ -568,8 +579,8 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
         const int level = x.level[h];
         const int maxlv = wave_max_i32(level);
         for (int L = 1; L <= maxlv; ++L) {
-            const int rep_r = __shfl_down(rep, 1, kWave);
-            const int32_t es_r = __shfl_down(es, 1, kWave);
+            const int rep_r = lane_next(rep);
+            const int32_t es_r = lane_next(es);
             int sw = 0;
             if (level == L) {
                 const double lb = (bk - bk1) * (double)(es - es_r);
@@ -580,9 +591,9 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
                     ++nsw;
                 }
             }
-            const int rep_l = __shfl_up(rep, 1, kWave);
-            const int32_t es_l = __shfl_up(es, 1, kWave);
-            const int swl = __shfl_up(sw, 1, kWave);
+            const int rep_l = lane_prev(rep);
+            const int32_t es_l = lane_prev(es);
+            const int swl = lane_prev(sw);
             if (sw) {
                 rep = rep_r;
                 es = es_r;
