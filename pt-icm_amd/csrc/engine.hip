@@ -108,6 +108,21 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         for (int32_t wv : pp->w) unit_ok = unit_ok && (wv == 1 || wv == -1);
     }
     fmt = unit_ok ? kFmtUnit : kFmtWide;
+    // sign table of the split sweep: unit records with a negative coupling
+    // somewhere, every instance's table within kSignTableMax bytes
+    // (and, so that every launch shape that fits without it still fits, 8
+    // waves' tables and threshold rows within 64 KiB)
+    tab = 0;
+    if (fmt == kFmtUnit) {
+        bool any_neg = false;
+        int64_t tmax = 0, fmx = 1;
+        for (const auto& pp : problems) {
+            for (int32_t wv : pp->w) any_neg = any_neg || wv < 0;
+            tmax = std::max<int64_t>(tmax, sign_table_bytes((pp->n + 31) / 32));
+            fmx = std::max<int64_t>(fmx, pp->fmax);
+        }
+        tab = any_neg && tmax <= kSignTableMax && 8 * (align_up(tmax, 16) + 4 * align_up(fmx, 4)) <= 64 * 1024 ? 1 : 0;
+    }
     for (int32_t i = 0; i < ninst; ++i) {
         const Problem& P = *problems[i];
         const int32_t nt = n_temps[i];
@@ -123,6 +138,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.fmax = P.fmax;
         d.has_field = P.has_field ? 1 : 0;
         d.neg = std::any_of(P.w.begin(), P.w.end(), [](int32_t w) { return w < 0; }) ? 1 : 0;
+        d.kbit = tab ? sign_table_bit(d.nw) : 15;
         d.max_deg = P.max_deg;
         d.deg_mean = (int32_t)(P.nnz() / std::max(P.n, 1));
         int32_t nc = 0;
@@ -149,7 +165,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 std::vector<uint16_t> rec((size_t)chunks * 8, (uint16_t)sentinel);
                 for (int32_t j = 0; j < dg; ++j) {
                     const int32_t q = P.rowptr[o] + j;
-                    rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 0x8000u : 0u));
+                    rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << d.kbit : 0u));
                 }
                 for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
                 cur16 += chunks;
@@ -241,7 +257,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
     Wi = cfg.icm_waves;
     if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
-    spin_stride = (int32_t)align_up((int64_t)spin_bytes(nw_max), 16);
+    spin_stride = (int32_t)align_up((int64_t)(tab ? sign_table_bytes(nw_max) : spin_bytes(nw_max)), 16);
     thr_stride = (int32_t)align_up(fmax_max, 4);
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
     if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
@@ -392,6 +408,7 @@ static DevArgs build_args(Engine& e) {
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
     a.sweeps = e.cfg.sweeps;
     a.spin_stride = e.spin_stride;
+    a.tab = e.tab;
     a.thr_stride = e.thr_stride;
     a.pilot = (e.cfg.flags & PTX_FLAG_PILOT) ? 1 : 0;
     a.no_exchange = (e.cfg.flags & PTX_FLAG_NO_EXCHANGE) ? 1 : 0;

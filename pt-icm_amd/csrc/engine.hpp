@@ -56,6 +56,7 @@ struct Engine {
     int32_t Wi = 0;              // waves per ICM slot pair (split mode; 0 = by the number of pairs)
     int32_t mode = PTX_MODE_SPLIT;
     int32_t fmt = 0;             // adjacency record format of the fused kernel
+    int32_t tab = 0;             // unit records carry the sign at the instance's kbit; k_sweep keeps the sign table
     int32_t Wf = 8;              // waves per workgroup (fused mode)
     int32_t K = 16;              // PT steps per fused launch
     int32_t fused_lds = 0;       // bytes of dynamic LDS per fused workgroup

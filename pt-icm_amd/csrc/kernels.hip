@@ -49,7 +49,7 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
         expand_bits(words, d.nw, sp, lane);
         wave_sync();
     }
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit};
     const int32_t E = energy_bytes(csr, d.n, a.hfield + d.pos_off, d.has_field != 0, sp, lane);
     if (lane == 0) {
         a.E[gr] = E;
@@ -93,7 +93,13 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
     wave_sync();
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg};
+    if (a.tab) {  // the sign table's negated bytes at 2^kbit
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(sp + (1 << d.kbit));
+        for (int q = lane; q < spin_bytes(d.nw) / 4; q += kWave) dst[q] = src[q] ^ 0x01010101u;
+        wave_sync();
+    }
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, a.tab};
     const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     for (int sw = 0; sw < a.sweeps; ++sw) {
         const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
@@ -195,7 +201,7 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
     uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * d.nw;
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit};
     long long dE1 = 0, dE2 = 0;
     const int maxch = (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk;
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
@@ -315,8 +321,8 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
         succ[s] = 0;
     }
     __syncthreads();
-    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg}
-                           : Csr<Fmt>{info, smem + L.adj, d.neg};
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit}
+                           : Csr<Fmt>{info, smem + L.adj, d.neg, d.kbit};
     const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);

@@ -51,6 +51,7 @@ struct DevArgs {
     const int32_t* order;    // fused mode: instance per workgroup
     const uint32_t* lat;     // lattice tables (LatView), one per lattice shape
     int32_t fmt;             // adjacency record format
+    int32_t tab;             // 1: unit records carry the sign at bit kbit and k_sweep keeps the sign table
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
@@ -117,7 +118,10 @@ template <int Fmt>
 struct Csr {
     const uint32_t* info;
     const uint8_t* base;
-    int neg = 1;  // 0: no coupling is negative (unit format: every sign bit is 0, agree = up neighbours)
+    int neg = 1;    // 0: no coupling is negative (unit format: every sign bit is 0, agree = up neighbours)
+    int kbit = 15;  // unit format: a record's sign bit (below it the position)
+    int tab = 0;    // 1: sp holds the sign table (negated bytes at 2^kbit): a record addresses b ^ sign
+    __device__ __forceinline__ uint32_t pos_mask() const { return (1u << kbit) - 1u; }
     static constexpr int kPerChunk = Fmt == kFmtUnit ? 8 : 4;
     __device__ __forceinline__ int deg(int p) const { return (int)(info[p] & (uint32_t)kMaxDegree); }
     __device__ __forceinline__ const uint8_t* row(int p) const { return base + (size_t)(info[p] >> 9) * 16u; }
@@ -129,8 +133,8 @@ struct Csr {
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const uint32_t e = (wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
-                c[h] = (int)(e & 0x7FFFu);
-                w[h] = (e & 0x8000u) ? -1 : 1;
+                c[h] = (int)(e & pos_mask());
+                w[h] = ((e >> kbit) & 1u) ? -1 : 1;
             }
         } else {
 #pragma unroll
@@ -158,7 +162,7 @@ struct Csr {
             if (c0 + k * step < nch) {
 #pragma unroll
                 for (int h = 0; h < 8; ++h)
-                    b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu]
+                    b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & pos_mask()]
                                   : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
             } else {
 #pragma unroll
@@ -171,7 +175,7 @@ struct Csr {
             const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
 #pragma unroll
             for (int h = 0; h < 8; ++h)
-                agree += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u)) : (int)b[k][h];
+                agree += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + kbit)) & 1u)) : (int)b[k][h];
         }
         return agree;
     }
@@ -245,7 +249,8 @@ struct Csr {
     // padding records read the zero sentinel byte and need no mask.
     __device__ __forceinline__ void group_field(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
         // the sign decision hoisted out of the loops: one loop body per case
-        if (Fmt == kFmtUnit && !neg)
+        // (with the sign table a signed record reads b ^ sign itself)
+        if (Fmt == kFmtUnit && (!neg || tab))
             group_field_t<false>(p0, cs, ce, sp, f);
         else
             group_field_t<true>(p0, cs, ce, sp, f);
@@ -280,7 +285,7 @@ struct Csr {
                     const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
 #pragma unroll
                     for (int h = 0; h < 8; ++h)
-                        b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & 0x7FFFu]
+                        b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & pos_mask()]
                                       : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
                 }
 #pragma unroll
@@ -289,7 +294,7 @@ struct Csr {
                     int a = 0;
 #pragma unroll
                     for (int h = 0; h < 8; ++h)
-                        a += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + 15)) & 1u)) : (int)b[k][h];
+                        a += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + kbit)) & 1u)) : (int)b[k][h];
                     f[k] += ch < nch[k] ? a : 0;
                 }
             } else {
@@ -325,6 +330,17 @@ struct Csr {
 // padding records read) and the padding bytes past n stay 0.  lane/stride
 // let several waves share the work.
 __host__ __device__ inline int spin_bytes(int nw) { return 32 * (nw + 1); }
+// Sign table of the split sweep: the spin bytes at [0, spin_bytes) and their
+// negations at [2^K, 2^K + spin_bytes), K the smallest bit with 2^K >=
+// spin_bytes(nw) (so every position, the sentinel included, is below 2^K).
+// A unit record pos | sign << K then addresses b_pos ^ sign directly.
+__host__ __device__ inline int sign_table_bit(int nw) {
+    int k = 2;
+    while ((1 << k) < spin_bytes(nw)) ++k;
+    return k;
+}
+__host__ __device__ inline int sign_table_bytes(int nw) { return (1 << sign_table_bit(nw)) + spin_bytes(nw); }
+constexpr int kSignTableMax = 16 * 1024;  // per wave; larger instances keep sign bit 15 and decode it
 
 __device__ __forceinline__ void expand_bits(const uint32_t* __restrict__ words, int nw, uint8_t* sp, int lane,
                                             int stride = kWave) {
@@ -375,7 +391,8 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
                                              const int32_t* __restrict__ hf, bool has_field, uint32_t c3, uint32_t k0,
                                              uint32_t k1, int r, uint32_t gsweep, uint8_t* sp,
                                              const uint32_t* thr_row, int& dsum, int& nacc,
-                                             const uint32_t* pre = nullptr) {
+                                             const uint32_t* pre = nullptr, uint8_t* spn = nullptr) {
+    // spn (optional): the sign table's negated bytes, kept equal to sp ^ 1
     // pre (optional): this sweep's random words, pre[p] = word p&3 of block p>>2
     const int p0 = 4 * g;
     int f[4];
@@ -413,7 +430,10 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
             ++nacc;
         }
     }
-    if (flip) *reinterpret_cast<uint32_t*>(sp + p0) = own ^ flip;
+    if (flip) {
+        *reinterpret_cast<uint32_t*>(sp + p0) = own ^ flip;
+        if (spn) *reinterpret_cast<uint32_t*>(spn + p0) = own ^ flip ^ 0x01010101u;
+    }
 }
 
 template <int Fmt>
@@ -426,8 +446,9 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
     for (int c = 0; c < ncls; ++c) {
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
+        uint8_t* spn = csr.tab ? sp + (1 << csr.kbit) : nullptr;
         for (int g = (cs >> 2) + lane; 4 * g < ce; g += kWave) {
-            group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc);
+            group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc, nullptr, spn);
         }
         wave_sync();
         dsum = wave_sum_i32(dsum);

@@ -204,6 +204,7 @@ struct InstDesc {
     int32_t nc;        // ICM slots
     int32_t has_field; // any h != 0
     int32_t neg;       // any coupling < 0 (0: unit-format records carry no sign, the gathers skip it)
+    int32_t kbit;      // unit format: sign bit of a record (15, or the bit of the split sweep's sign table)
     int64_t pos_off;     // -> rowinfo[n], h[n], pos2orig[n]
     int64_t adj_off16;   // -> first 16-byte unit of the instance's adjacency rows
     int64_t adj_len16;   // 16-byte units of adjacency rows
