@@ -16,7 +16,8 @@ namespace ptx {
 constexpr int kWave = 64;
 
 // Adjacency record formats.  Wide: u32 = position | (int16 weight << 16).
-// Unit: u16 = position | (weight < 0) << 15, for |w| == 1 and n <= 32736.
+// Unit: u16 = position | (weight < 0) << K, for |w| == 1 and n <= 32736, with
+// K = 15 or the instance's sign-table bit (InstDesc::kbit, see sign_table_bit).
 // Rows start on 16-byte boundaries and take at least one 16-byte chunk;
 // padding records name the sentinel position 32 * nw (weight 0 / sign 0),
 // whose spin byte is kept 0, so they add nothing to a local field.
