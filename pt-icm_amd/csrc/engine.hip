@@ -90,6 +90,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     std::vector<uint32_t> adjw;  // padded rows as 32-bit words (both formats)
     std::vector<double> beta;
     std::vector<uint32_t> lat;
+    std::vector<uint32_t> jneg;  // lattice edge signs, per instance
     struct LatShape {
         int32_t dim, L;
         const Problem* prob;
@@ -206,6 +207,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         // (the colouring, and so the position order, depends on the shape only)
         d.lat_dim = d.lat_L = d.lat_R = d.lat_words = 0;
         d.lat_off = 0;
+        d.jn_off = -1;
         if (nc > 0 && P.lat_dim > 0) {
             const int32_t L = P.lat_L, R = (L + 31) / 32;
             int32_t rows = 1;
@@ -236,6 +238,29 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             d.lat_words = R * rows;
             d.lat_off = off;
             lat_words_max = std::max(lat_words_max, d.lat_words);
+            // edge signs for the cut energy on bit sets: rows of one word
+            // (4 <= L <= 32), unit couplings, no field
+            if (R == 1 && L >= 4 && fmt == kFmtUnit && !P.has_field) {
+                d.jn_off = (int64_t)jneg.size();
+                auto weight = [&](int32_t i, int32_t j) {
+                    for (int32_t q = P.rowptr[i]; q < P.rowptr[i + 1]; ++q)
+                        if (P.col[q] == j) return P.w[q];
+                    return 0;
+                };
+                int64_t stride = 1;
+                for (int32_t ax = 0; ax < P.lat_dim; ++ax, stride *= L) {
+                    for (int32_t row = 0; row < rows; ++row) {
+                        uint32_t word = 0;
+                        for (int32_t x = 0; x < L; ++x) {
+                            const int32_t o = row * L + x;
+                            const int32_t c = (int32_t)((o / stride) % L);
+                            const int32_t j = o - (int32_t)(c * stride) + (int32_t)(((c + 1) % L) * stride);
+                            if (weight(o, j) < 0) word |= 1u << x;
+                        }
+                        jneg.push_back(word);
+                    }
+                }
+            }
         }
         d.e_base = (int32_t)(-P.abs_sum);
         d.hist_bins = hist ? (int32_t)(P.abs_sum + 1) : 0;
@@ -355,6 +380,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_p2o.upload(p2o, stream));
     if (lat.empty()) lat.assign(4, 0u);
     PTX_TRY(d_lat.upload(lat, stream));
+    if (jneg.empty()) jneg.assign(4, 0u);
+    PTX_TRY(d_jneg.upload(jneg, stream));
     PTX_TRY(d_thr.upload(thr, stream));
     PTX_TRY(d_beta.upload(beta, stream));
     PTX_TRY(build_lists());
@@ -402,6 +429,7 @@ static DevArgs build_args(Engine& e) {
     a.pairs = e.d_pairs.p;
     a.order = e.d_order.p;
     a.lat = e.d_lat.p;
+    a.jneg = e.d_jneg.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.inst_step = e.d_inst_step.p;
     a.seed = e.seed;

@@ -51,6 +51,7 @@ struct DevArgs {
     const int2* pairs;       // (instance, slot t) per ICM wavefront
     const int32_t* order;    // fused mode: instance per workgroup
     const uint32_t* lat;     // lattice tables (LatView), one per lattice shape
+    const uint32_t* jneg;    // lattice edge signs per instance (InstDesc::jn_off)
     int32_t fmt;             // adjacency record format
     int32_t tab;             // 1: unit records carry the sign at bit kbit and k_sweep keeps the sign table
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
@@ -770,6 +771,7 @@ struct LatView {
     const uint16_t* wpos;
     const uint32_t* obit;
     int dim, L, R, words;
+    const uint32_t* jn = nullptr;  // edge signs [dim][rows] (rows of one word), or none
 };
 
 // lattice words per lane kept in registers during cluster growth
@@ -779,12 +781,13 @@ __device__ __forceinline__ LatView lat_view(const DevArgs& a, const InstDesc& d)
     if (d.lat_dim == 0) return LatView{nullptr, nullptr, 0, 0, 0, 0};
     const uint32_t* t = a.lat + d.lat_off;  // wpos (16-bit) then obit
     return LatView{reinterpret_cast<const uint16_t*>(t), t + 16 * (int64_t)d.lat_words, d.lat_dim, d.lat_L, d.lat_R,
-                   d.lat_words};
+                   d.lat_words, d.jn_off >= 0 ? a.jneg + d.jn_off : nullptr};
 }
 
 // Grows the cluster of seed_pos (a position in D) as a bit set in lattice
 // order until no word changes, then writes it to V (position order) and its
-// positions to Q; returns the cluster size.  One wavefront.  Updates are
+// positions to Q (with edge signs instead S1 in lattice order to the Q area,
+// for lattice_cut_energy); returns the cluster size.  One wavefront.  Updates are
 // in place and only ever add sites reached from the seed through D, so the
 // order of the word updates does not change the fixed point: the connected
 // component of the seed in D.
@@ -796,6 +799,11 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
     uint32_t* Do = sc.Do;
     uint32_t* Vo = sc.Vo;
     const uint32_t* D = sc.D;
+    // With edge signs (the cut energy on bit sets follows) S1 goes to lattice
+    // order in the same pass, into the position-list area, and no position
+    // list is written.
+    const bool cut = lv.jn != nullptr;
+    uint32_t* S1o = reinterpret_cast<uint32_t*>(sc.Q);
     // D in lattice order: the 32 positions of each word are read as four
     // 16-byte loads, then their D bits
     for (int w = lane; w < NW; w += kWave) {
@@ -803,7 +811,7 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
         uint4 c[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) c[q] = src[q];
-        uint32_t word = 0;
+        uint32_t word = 0, sword = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const uint32_t pv[4] = {c[q].x, c[q].y, c[q].z, c[q].w};
@@ -812,11 +820,19 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
                 const uint32_t p0 = pv[h] & 0xFFFFu, p1 = pv[h] >> 16;
                 word |= ((D[p0 >> 5] >> (p0 & 31)) & 1u) << (8 * q + 2 * h);
                 word |= ((D[p1 >> 5] >> (p1 & 31)) & 1u) << (8 * q + 2 * h + 1);
+                if (cut) {
+                    sword |= ((sc.S1[p0 >> 5] >> (p0 & 31)) & 1u) << (8 * q + 2 * h);
+                    sword |= ((sc.S1[p1 >> 5] >> (p1 & 31)) & 1u) << (8 * q + 2 * h + 1);
+                }
             }
         }
-        if (w % R == R - 1) word &= lastmask;
+        if (w % R == R - 1) {
+            word &= lastmask;
+            sword &= lastmask;
+        }
         Do[w] = word;
         Vo[w] = 0u;
+        if (cut) S1o[w] = sword;
     }
     wave_sync();
     if (lane == 0) {
@@ -1050,7 +1066,7 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
             const unsigned long long b = __ballot(in);
             const int below =
                 (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u));
-            if (in) sc.Q[csize + below] = (uint16_t)p;
+            if (in && !cut) sc.Q[csize + below] = (uint16_t)p;
             if (lane == 0) sc.V[p0 >> 5] = (uint32_t)b;
             if (lane == 32 && (p0 >> 5) + 1 < nw) sc.V[(p0 >> 5) + 1] = (uint32_t)(b >> 32);
             csize += __popcll(b);
@@ -1058,6 +1074,48 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
     }
     wave_sync();
     return csize;
+}
+
+// Energy change of flipping the grown cluster (Vo, lattice order) in both
+// replicas of a periodic lattice with one word per row and no field, on bit
+// sets (one wavefront, after lattice_grow).  An edge (p, q) with exactly one
+// end in the cluster is cut; flipping the cluster turns its w s_p s_q =
+// (-1)^(neg_pq ^ b_p ^ b_q) into the negative, so dE = -2 (cut - 2 odd), odd
+// the cut edges whose parity bit is 1.  Each edge is counted once, from the
+// site to its +axis neighbour (the rotated row along x, the next row along y
+// and z, wrapped).  lattice_grow left S1 in lattice order in the
+// position-list area; S2 = S1 ^ D.  Adds this lane's part to dE1, dE2.
+__device__ __forceinline__ void lattice_cut_energy(const LatView& lv, IcmScratch sc, int lane, long long& dE1,
+                                                   long long& dE2) {
+    const int L = lv.L, rows = lv.words;
+    const uint32_t lm = L == 32 ? 0xFFFFFFFFu : ((1u << L) - 1u);
+    const uint32_t* Do = sc.Do;
+    const uint32_t* Vo = sc.Vo;
+    const uint32_t* S1o = reinterpret_cast<const uint32_t*>(sc.Q);
+    auto rot = [&](uint32_t v) { return ((v >> 1) | ((v & 1u) << (L - 1))) & lm; };  // bit x <- bit (x+1) mod L
+    const uint32_t* jx = lv.jn;
+    const uint32_t* jy = jx + rows;
+    const uint32_t* jz = jy + rows;
+    int cut = 0, odd1 = 0, odd2 = 0;
+    for (int w = lane; w < rows; w += kWave) {
+        const uint32_t c = Vo[w], s1 = S1o[w], s2 = s1 ^ Do[w];
+        auto edges = [&](uint32_t cn, uint32_t s1n, uint32_t s2n, uint32_t neg) {
+            const uint32_t x = c ^ cn;
+            cut += __popc(x);
+            odd1 += __popc(x & (neg ^ s1 ^ s1n));
+            odd2 += __popc(x & (neg ^ s2 ^ s2n));
+        };
+        edges(rot(c), rot(s1), rot(s2), jx[w]);
+        const int y = w % L, zb = w - y;
+        const int wy = zb + (y + 1) % L;
+        edges(Vo[wy], S1o[wy], S1o[wy] ^ Do[wy], jy[w]);
+        if (lv.dim == 3) {
+            const int wz = y + (zb + L) % (L * L);
+            edges(Vo[wz], S1o[wz], S1o[wz] ^ Do[wz], jz[w]);
+        }
+    }
+    dE1 += -2ll * (cut - 2 * odd1);
+    dE2 += -2ll * (cut - 2 * odd2);
 }
 
 // Per-wave slots of the workgroup reductions in the ICM team code.
@@ -1199,8 +1257,13 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
         team_sync<Block>();
         constexpr int K = Csr<Fmt>::kPerChunk;
         int csize;
+        bool cut_done = false;  // the energy change came from bit sets
         if (!Block && lv.dim > 0) {
             csize = lattice_grow(lv, n, nw, seed_pos, sc, lane);
+            if (lv.jn) {
+                lattice_cut_energy(lv, sc, lane, dE1, dE2);
+                cut_done = true;
+            }
         } else {
             // Breadth-first growth.  Each lane takes one queued site and walks its
             // row a 16-byte chunk at a time: the D and V words of the chunk's
@@ -1286,7 +1349,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
         }
         // energy change of flipping the cluster C in each replica: boundary
         // edges (p in C, j not in C) and the field terms of C
-        for (int idx = tid; idx < csize; idx += nthr) {
+        for (int idx = tid; idx < (cut_done ? 0 : csize); idx += nthr) {
             const int p = sc.Q[idx];
             const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
             const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;

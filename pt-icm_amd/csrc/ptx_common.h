@@ -224,6 +224,8 @@ struct InstDesc {
     int32_t lat_R;       // 32-bit words per lattice row (rows padded to whole words)
     int32_t lat_words;   // words of a lattice-order bit set: lat_R * L^(lat_dim-1)
     int64_t lat_off;     // -> lat_o2p[n] (index -> position) and lat_obit[n] (position -> word<<5|bit)
+    int64_t jn_off;      // -> edge signs [lat_dim][rows] (bit x of row r: the coupling of the site to its +axis
+                         //    neighbour is negative) for cut energies on bit sets, or -1
 };
 
 }  // namespace ptx
