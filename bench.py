@@ -273,10 +273,10 @@ def main():
     del eng
     if args.batched_steps > 0:
         out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
-    if args.scale_steps > 0:
-        out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed)
         out["batched"]["roofline"]["peak"] = lds_peak
         out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
+    if args.scale_steps > 0:
+        out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed)
     if args.ttt_plain > 0 and os.path.exists(args.ttt_targets):
         sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
         import ttt

@@ -238,9 +238,9 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             d.lat_words = R * rows;
             d.lat_off = off;
             lat_words_max = std::max(lat_words_max, d.lat_words);
-            // edge signs for the cut energy on bit sets: rows of one word
-            // (4 <= L <= 32), unit couplings, no field
-            if (R == 1 && L >= 4 && fmt == kFmtUnit && !P.has_field) {
+            // edge signs for the cut energy on bit sets (L >= 4: the lattice-
+            // order S1 fits the position-list area), unit couplings, no field
+            if (L >= 4 && fmt == kFmtUnit && !P.has_field) {
                 d.jn_off = (int64_t)jneg.size();
                 auto weight = [&](int32_t i, int32_t j) {
                     for (int32_t q = P.rowptr[i]; q < P.rowptr[i + 1]; ++q)
@@ -250,14 +250,16 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 int64_t stride = 1;
                 for (int32_t ax = 0; ax < P.lat_dim; ++ax, stride *= L) {
                     for (int32_t row = 0; row < rows; ++row) {
-                        uint32_t word = 0;
-                        for (int32_t x = 0; x < L; ++x) {
-                            const int32_t o = row * L + x;
-                            const int32_t c = (int32_t)((o / stride) % L);
-                            const int32_t j = o - (int32_t)(c * stride) + (int32_t)(((c + 1) % L) * stride);
-                            if (weight(o, j) < 0) word |= 1u << x;
+                        for (int32_t k = 0; k < R; ++k) {
+                            uint32_t word = 0;
+                            for (int32_t b = 0; b < 32 && 32 * k + b < L; ++b) {
+                                const int32_t o = row * L + 32 * k + b;
+                                const int32_t c = (int32_t)((o / stride) % L);
+                                const int32_t j = o - (int32_t)(c * stride) + (int32_t)(((c + 1) % L) * stride);
+                                if (weight(o, j) < 0) word |= 1u << b;
+                            }
+                            jneg.push_back(word);
                         }
-                        jneg.push_back(word);
                     }
                 }
             }

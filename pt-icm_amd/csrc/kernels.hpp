@@ -1077,41 +1077,56 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
 }
 
 // Energy change of flipping the grown cluster (Vo, lattice order) in both
-// replicas of a periodic lattice with one word per row and no field, on bit
-// sets (one wavefront, after lattice_grow).  An edge (p, q) with exactly one
-// end in the cluster is cut; flipping the cluster turns its w s_p s_q =
-// (-1)^(neg_pq ^ b_p ^ b_q) into the negative, so dE = -2 (cut - 2 odd), odd
-// the cut edges whose parity bit is 1.  Each edge is counted once, from the
-// site to its +axis neighbour (the rotated row along x, the next row along y
-// and z, wrapped).  lattice_grow left S1 in lattice order in the
-// position-list area; S2 = S1 ^ D.  Adds this lane's part to dE1, dE2.
+// replicas of a periodic lattice with no field, on bit sets (one wavefront,
+// after lattice_grow).  An edge (p, q) with exactly one end in the cluster
+// is cut; flipping the cluster turns its w s_p s_q = (-1)^(neg_pq ^ b_p ^
+// b_q) into the negative, so dE = -2 (cut - 2 odd), odd the cut edges whose
+// parity bit is 1.  Each edge is counted once, from the site to its +axis
+// neighbour: along x the row rotated by one site (across its R words, the
+// last word's top site taking the first word's bit 0), along y and z the
+// same word of the next row, wrapped.  A lane takes whole rows.
+// lattice_grow left S1 in lattice order in the position-list area; S2 = S1
+// ^ D.  Adds this lane's part to dE1, dE2.
 __device__ __forceinline__ void lattice_cut_energy(const LatView& lv, IcmScratch sc, int lane, long long& dE1,
                                                    long long& dE2) {
-    const int L = lv.L, rows = lv.words;
-    const uint32_t lm = L == 32 ? 0xFFFFFFFFu : ((1u << L) - 1u);
+    const int L = lv.L, R = lv.R, NW = lv.words, rows = NW / R;
+    const int lb = L - 32 * (R - 1);  // sites in a row's last word (1..32)
+    const uint32_t lastmask = lb == 32 ? 0xFFFFFFFFu : ((1u << lb) - 1u);
     const uint32_t* Do = sc.Do;
     const uint32_t* Vo = sc.Vo;
     const uint32_t* S1o = reinterpret_cast<const uint32_t*>(sc.Q);
-    auto rot = [&](uint32_t v) { return ((v >> 1) | ((v & 1u) << (L - 1))) & lm; };  // bit x <- bit (x+1) mod L
     const uint32_t* jx = lv.jn;
-    const uint32_t* jy = jx + rows;
-    const uint32_t* jz = jy + rows;
+    const uint32_t* jy = jx + NW;
+    const uint32_t* jz = jy + NW;
     int cut = 0, odd1 = 0, odd2 = 0;
-    for (int w = lane; w < rows; w += kWave) {
-        const uint32_t c = Vo[w], s1 = S1o[w], s2 = s1 ^ Do[w];
-        auto edges = [&](uint32_t cn, uint32_t s1n, uint32_t s2n, uint32_t neg) {
-            const uint32_t x = c ^ cn;
-            cut += __popc(x);
-            odd1 += __popc(x & (neg ^ s1 ^ s1n));
-            odd2 += __popc(x & (neg ^ s2 ^ s2n));
-        };
-        edges(rot(c), rot(s1), rot(s2), jx[w]);
-        const int y = w % L, zb = w - y;
-        const int wy = zb + (y + 1) % L;
-        edges(Vo[wy], S1o[wy], S1o[wy] ^ Do[wy], jy[w]);
-        if (lv.dim == 3) {
-            const int wz = y + (zb + L) % (L * L);
-            edges(Vo[wz], S1o[wz], S1o[wz] ^ Do[wz], jz[w]);
+    for (int row = lane; row < rows; row += kWave) {
+        const int y = row % L, zb = row - y;
+        const int ry = zb + (y + 1) % L;
+        const int rz = lv.dim == 3 ? y + (zb + L) % (L * L) : 0;
+        const int w0 = row * R;
+        for (int k = 0; k < R; ++k) {
+            const int w = w0 + k;
+            const uint32_t c = Vo[w], s1 = S1o[w], s2 = s1 ^ Do[w];
+            auto edges = [&](uint32_t cn, uint32_t s1n, uint32_t s2n, uint32_t neg) {
+                const uint32_t x = c ^ cn;
+                cut += __popc(x);
+                odd1 += __popc(x & (neg ^ s1 ^ s1n));
+                odd2 += __popc(x & (neg ^ s2 ^ s2n));
+            };
+            // +x: bit b of the rotated word is site 32 k + b + 1 of the row
+            const bool last = k == R - 1;
+            const int wn = last ? w0 : w + 1;
+            const uint32_t cn = Vo[wn], s1n = S1o[wn], s2n = s1n ^ Do[wn];
+            const int top = last ? lb - 1 : 31;
+            const uint32_t m = last ? lastmask : 0xFFFFFFFFu;
+            edges(((c >> 1) | ((cn & 1u) << top)) & m, ((s1 >> 1) | ((s1n & 1u) << top)) & m,
+                  ((s2 >> 1) | ((s2n & 1u) << top)) & m, jx[w]);
+            const int wy = ry * R + k;
+            edges(Vo[wy], S1o[wy], S1o[wy] ^ Do[wy], jy[w]);
+            if (lv.dim == 3) {
+                const int wz = rz * R + k;
+                edges(Vo[wz], S1o[wz], S1o[wz] ^ Do[wz], jz[w]);
+            }
         }
     }
     dE1 += -2ll * (cut - 2 * odd1);
