@@ -142,7 +142,9 @@ typedef struct {
     int32_t nc;              /* ICM slots; -1 = reference rule per instance */
     int64_t hist_start;      /* first step whose energies enter the histogram */
     int32_t mode;            /* PTX_MODE_* */
-    int32_t waves_per_group; /* replicas per workgroup (0 = autotuned) */
+    int32_t waves_per_group; /* split: replicas (one wave each) per workgroup, 1/2/4/8; fused: waves per
+                                instance, 4/8/16; cooperative: waves per replica, 4, 8 or 16 (16 with one
+                                position per lane and no ICM slot, else 8) (0 = auto) */
     int32_t steps_per_launch;/* fused mode: PT steps per launch (0 = auto) */
     int32_t use_graph;       /* fused mode: G > 0 captures G launches of steps_per_launch steps in a
                                 hipGraph and replays it (0 = eager launches) */

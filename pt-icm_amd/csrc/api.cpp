@@ -281,9 +281,11 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
             int32_t mode, w, spl, csr, wi;
         };
         std::vector<Shape> shapes;
-        if (n_instances == 1)
+        if (n_instances == 1) {
             for (int w : {4, 8})
                 for (int spl : {1, 4}) shapes.push_back({PTX_MODE_COOP, w, spl, PTX_CSR_AUTO, 0});
+            shapes.push_back({PTX_MODE_COOP, 16, 1, PTX_CSR_AUTO, 0});  // taken as 8 with cluster moves
+        }
         for (int w : {4, 8, 16})
             for (int csr : {PTX_CSR_LDS, PTX_CSR_GLOBAL}) shapes.push_back({PTX_MODE_FUSED, w, 0, csr, 0});
         // split: one wave per cluster (the default, 0) or four

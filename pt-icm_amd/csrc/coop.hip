@@ -34,7 +34,7 @@ struct CoopArgs {
 // sweep.  Every position draws word p&3 of the Philox block p>>2, so the
 // result is the same for any SPL and workgroup size.  Each wave's class sums
 // go to its slot in one of two alternating LDS buffers (one barrier per class).
-constexpr int kMaxCoopWaves = 8;  // waves per cooperative workgroup (4 or 8)
+constexpr int kMaxCoopWaves = 16;  // waves per cooperative workgroup (4, 8, or 16 without cluster moves)
 
 // The random words of a whole sweep of replica r: pre[p] = word p&3 of the
 // Philox block (p>>2, r, gsweep) -- the words the sweep would draw for p.
@@ -491,5 +491,9 @@ PTX_COOP_INST(kFmtWide, 0)
 PTX_COOP_INST(kFmtWide, 1)
 PTX_COOP_INST(kFmtUnit, 0)
 PTX_COOP_INST(kFmtUnit, 1)
+// 16 waves (4 per SIMD: more LDS reads in flight per class), one position
+// per lane, adjacency in LDS, no cluster moves
+template __global__ void k_coop<kFmtWide, 16, 1, 0, 0>(DevArgs, CoopArgs, int);
+template __global__ void k_coop<kFmtUnit, 16, 1, 0, 0>(DevArgs, CoopArgs, int);
 
 }  // namespace ptx

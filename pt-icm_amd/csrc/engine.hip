@@ -301,8 +301,11 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     spin_words = spin_off;
     mode = PTX_MODE_SPLIT;
     if (cfg.mode == PTX_MODE_COOP) {
-        coop_W = cfg.waves_per_group == 8 ? 8 : 4;
         spl = cfg.spins_per_lane == 4 ? 4 : 1;
+        // 16 waves only for one position per lane and no cluster moves
+        coop_W = cfg.waves_per_group == 16 && spl == 1 && !coop_icm ? 16
+                 : cfg.waves_per_group >= 8                        ? 8
+                                                                   : 4;
         coop_K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 256;
         PTX_TRY(setup_coop(true));
         mode = PTX_MODE_COOP;
@@ -604,6 +607,7 @@ const void* Engine::coop_kernel(int g) const {
     auto pick = [&](auto fmt_tag, auto ic_tag) -> const void* {
         constexpr int F = decltype(fmt_tag)::value, I = decltype(ic_tag)::value;
         if (g) return reinterpret_cast<const void*>(k_coop<F, 4, 1, 1, I>);
+        if (coop_W == 16) return reinterpret_cast<const void*>(k_coop<F, 16, 1, 0, 0>);
         if (coop_W == 8) return spl == 4 ? reinterpret_cast<const void*>(k_coop<F, 8, 4, 0, I>)
                                          : reinterpret_cast<const void*>(k_coop<F, 8, 1, 0, I>);
         return spl == 4 ? reinterpret_cast<const void*>(k_coop<F, 4, 4, 0, I>)

@@ -229,6 +229,37 @@ def test_launch_shape_independence(ptx, oracle, mode, wpg, spl, lanes):
     compare(eng, oe, what=f"W={wpg}")
 
 
+@pytest.mark.parametrize("kind", ["G022", "field_random", "ea3d_6"])
+def test_cooperative_16_waves_parity(ptx, oracle, kind):
+    """Cooperative kernel with 16 waves per replica (no ICM slot; 4 lanes
+    per position on classes of at most 256 positions, 2 on classes of at
+    most 512) against the oracle."""
+    p, g = make_pair(ptx, oracle, kind)
+    betas = oracle.beta_linspace(0.5, 3.0, 10, 0)
+    say("case", kind, "16 waves")
+    eng = ptx.Engine(p, betas, seed=23, flags=0, mode=ptx.MODE_COOP, waves_per_group=16, steps_per_launch=4)
+    assert eng.info().waves_per_group == 16
+    oe = oracle.Engine(g, betas, chains=2, seed=23, flags=0)
+    for s_ in range(6):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"{kind} step {s_}")
+    eng.step(20)
+    oe.step(20)
+    compare(eng, oe, what=f"{kind} bulk")
+
+
+def test_cooperative_16_waves_fall_back_to_8_with_cluster_moves(ptx, oracle):
+    p, g = make_pair(ptx, oracle, "pm_torus20")
+    betas = oracle.beta_linspace(0.5, 3.0, 12, 0)
+    eng = ptx.Engine(p, betas, seed=4, flags=ptx.FLAG_ICM, mode=ptx.MODE_COOP, waves_per_group=16)
+    assert eng.info().waves_per_group == 8
+    oe = oracle.Engine(g, betas, chains=2, seed=4, flags=oracle.FLAG_ICM)
+    eng.step(10)
+    oe.step(10)
+    compare(eng, oe, what="16 -> 8 waves")
+
+
 def test_write_spins_recomputes_energy(ptx, oracle):
     p, g = make_pair(ptx, oracle, "lattice_2d_20.gset")
     betas = np.linspace(0.5, 1.5, 4)
