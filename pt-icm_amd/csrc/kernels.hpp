@@ -1081,24 +1081,24 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
 // after lattice_grow).  An edge (p, q) with exactly one end in the cluster
 // is cut; flipping the cluster turns its w s_p s_q = (-1)^(neg_pq ^ b_p ^
 // b_q) into the negative, so dE = -2 (cut - 2 odd), odd the cut edges whose
-// parity bit is 1.  Each edge is counted once, from the site to its +axis
+// parity bit is 1 (replica 1; replica 2's change is its negative, as the
+// cluster is a component of D).  Each edge is counted once, from the site to its +axis
 // neighbour: along x the row rotated by one site (across its R words, the
 // last word's top site taking the first word's bit 0), along y and z the
 // same word of the next row, wrapped.  A lane takes whole rows.
-// lattice_grow left S1 in lattice order in the position-list area; S2 = S1
-// ^ D.  Adds this lane's part to dE1, dE2.
+// lattice_grow left S1 in lattice order in the position-list area.  Adds
+// this lane's part to dE1 and its negative to dE2.
 __device__ __forceinline__ void lattice_cut_energy(const LatView& lv, IcmScratch sc, int lane, long long& dE1,
                                                    long long& dE2) {
     const int L = lv.L, R = lv.R, NW = lv.words, rows = NW / R;
     const int lb = L - 32 * (R - 1);  // sites in a row's last word (1..32)
     const uint32_t lastmask = lb == 32 ? 0xFFFFFFFFu : ((1u << lb) - 1u);
-    const uint32_t* Do = sc.Do;
     const uint32_t* Vo = sc.Vo;
     const uint32_t* S1o = reinterpret_cast<const uint32_t*>(sc.Q);
     const uint32_t* jx = lv.jn;
     const uint32_t* jy = jx + NW;
     const uint32_t* jz = jy + NW;
-    int cut = 0, odd1 = 0, odd2 = 0;
+    int cut = 0, odd = 0;
     for (int row = lane; row < rows; row += kWave) {
         const int y = row % L, zb = row - y;
         const int ry = zb + (y + 1) % L;
@@ -1106,31 +1106,30 @@ __device__ __forceinline__ void lattice_cut_energy(const LatView& lv, IcmScratch
         const int w0 = row * R;
         for (int k = 0; k < R; ++k) {
             const int w = w0 + k;
-            const uint32_t c = Vo[w], s1 = S1o[w], s2 = s1 ^ Do[w];
-            auto edges = [&](uint32_t cn, uint32_t s1n, uint32_t s2n, uint32_t neg) {
+            const uint32_t c = Vo[w], s1 = S1o[w];
+            auto edges = [&](uint32_t cn, uint32_t s1n, uint32_t neg) {
                 const uint32_t x = c ^ cn;
                 cut += __popc(x);
-                odd1 += __popc(x & (neg ^ s1 ^ s1n));
-                odd2 += __popc(x & (neg ^ s2 ^ s2n));
+                odd += __popc(x & (neg ^ s1 ^ s1n));
             };
             // +x: bit b of the rotated word is site 32 k + b + 1 of the row
             const bool last = k == R - 1;
             const int wn = last ? w0 : w + 1;
-            const uint32_t cn = Vo[wn], s1n = S1o[wn], s2n = s1n ^ Do[wn];
+            const uint32_t cn = Vo[wn], s1n = S1o[wn];
             const int top = last ? lb - 1 : 31;
             const uint32_t m = last ? lastmask : 0xFFFFFFFFu;
-            edges(((c >> 1) | ((cn & 1u) << top)) & m, ((s1 >> 1) | ((s1n & 1u) << top)) & m,
-                  ((s2 >> 1) | ((s2n & 1u) << top)) & m, jx[w]);
+            edges(((c >> 1) | ((cn & 1u) << top)) & m, ((s1 >> 1) | ((s1n & 1u) << top)) & m, jx[w]);
             const int wy = ry * R + k;
-            edges(Vo[wy], S1o[wy], S1o[wy] ^ Do[wy], jy[w]);
+            edges(Vo[wy], S1o[wy], jy[w]);
             if (lv.dim == 3) {
                 const int wz = rz * R + k;
-                edges(Vo[wz], S1o[wz], S1o[wz] ^ Do[wz], jz[w]);
+                edges(Vo[wz], S1o[wz], jz[w]);
             }
         }
     }
-    dE1 += -2ll * (cut - 2 * odd1);
-    dE2 += -2ll * (cut - 2 * odd2);
+    const long long t = -2ll * (cut - 2 * odd);
+    dE1 += t;
+    dE2 -= t;
 }
 
 // Per-wave slots of the workgroup reductions in the ICM team code.
@@ -1362,13 +1361,14 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
             }
             csize = tail;
         }
-        // energy change of flipping the cluster C in each replica: boundary
-        // edges (p in C, j not in C) and the field terms of C
+        // energy change of flipping the cluster C in replica 1: boundary
+        // edges (p in C, j not in C) and the field terms of C.  Replica 2's
+        // is its negative: C is a component of D = S1 ^ S2, so s2 = -s1 on C
+        // and every neighbour of C outside C lies outside D, where s2 = s1.
         for (int idx = tid; idx < (cut_done ? 0 : csize); idx += nthr) {
             const int p = sc.Q[idx];
             const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
-            const int s2 = ((S2[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
-            int b1 = 0, b2 = 0;
+            int b1 = 0;
             const uint8_t* row = csr.row(p);
             const int dg = csr.deg(p);
             for (int ch = 0; K * ch < dg; ++ch) {
@@ -1381,18 +1381,13 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                     const uint32_t sh = (uint32_t)(j & 31);
                     const uint32_t vin = (V[j >> 5] >> sh) & 1u;
                     const int a1 = ((S1[j >> 5] >> sh) & 1u) ? wgt[h] : -wgt[h];
-                    const int a2 = ((S2[j >> 5] >> sh) & 1u) ? wgt[h] : -wgt[h];
                     const bool use = h < live && !vin;
                     b1 += use ? a1 : 0;
-                    b2 += use ? a2 : 0;
                 }
             }
-            dE1 += -2ll * s1 * b1;
-            dE2 += -2ll * s2 * b2;
-            if (has_field) {
-                dE1 += -2ll * s1 * hf[p];
-                dE2 += -2ll * s2 * hf[p];
-            }
+            const long long t = -2ll * s1 * (b1 + (has_field ? hf[p] : 0));
+            dE1 += t;
+            dE2 -= t;
         }
         team_sync<Block>();
         for (int w = tid; w < nw; w += nthr) {
