@@ -1271,7 +1271,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
         team_sync<Block>();
         constexpr int K = Csr<Fmt>::kPerChunk;
         int csize;
-        bool cut_done = false;  // the energy change came from bit sets
+        bool cut_done = false;  // the energy change is known (bit sets, or sums taken during the growth)
         if (!Block && lv.dim > 0) {
             csize = lattice_grow(lv, n, nw, seed_pos, sc, lane);
             if (lv.jn) {
@@ -1295,6 +1295,10 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                 const int dg = act ? csr.deg(p) : 0;
                 const uint8_t* row = csr.row(p);
                 const int nch = (dg + K - 1) / K;
+                // p's boundary sum, taken while its row is read: a neighbour
+                // of a cluster site is outside the cluster exactly when it
+                // is outside D (the cluster is D's component)
+                int b1 = 0;
                 for (int ch = 0; ch < maxch; ++ch) {
                     const int live = ch < nch ? min(K, dg - ch * K) : 0;
                     // Every load and claim below is issued unconditionally (the
@@ -1303,11 +1307,12 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                     // masks the results.
                     int col[K], wgt[K];
                     csr.chunk(row, min(ch, max(nch - 1, 0)), col, wgt);
-                    uint32_t dwd[K], vwd[K];
+                    uint32_t dwd[K], vwd[K], swd[K];
     #pragma unroll
                     for (int h = 0; h < K; ++h) {
                         dwd[h] = D[col[h] >> 5];
                         vwd[h] = V[col[h] >> 5];
+                        swd[h] = S1[col[h] >> 5];
                     }
                     uint32_t cand = 0;
     #pragma unroll
@@ -1315,6 +1320,8 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                         const uint32_t bit = 1u << (col[h] & 31);
                         const bool c = (h < live) & ((dwd[h] & bit) != 0u) & ((vwd[h] & bit) == 0u);
                         cand |= (c ? 1u : 0u) << h;
+                        const bool out = (h < live) & ((dwd[h] & bit) == 0u);
+                        b1 += out ? ((swd[h] & bit) ? wgt[h] : -wgt[h]) : 0;
                     }
                     // only candidates claim (padding records share one sentinel word,
                     // and same-address atomics serialise); the claims carry no
@@ -1352,6 +1359,12 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                     for (int h = 0; h < K; ++h)
                         if ((claimed >> h) & 1u) sc.Q[pos++] = (uint16_t)col[h];
                 }
+                if (act) {
+                    const int s1 = ((S1[p >> 5] >> (p & 31)) & 1u) ? 1 : -1;
+                    const long long t = -2ll * s1 * (b1 + (has_field ? hf[p] : 0));
+                    dE1 += t;
+                    dE2 -= t;
+                }
                 head = (tl - head > nthr) ? head + nthr : tl;
                 team_sync<Block>();
                 if (Block) {
@@ -1360,6 +1373,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                 }
             }
             csize = tail;
+            cut_done = true;  // boundary sums taken during the growth
         }
         // energy change of flipping the cluster C in replica 1: boundary
         // edges (p in C, j not in C) and the field terms of C.  Replica 2's
