@@ -422,12 +422,11 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             const int t = r2s[r] % nt;
             if (t >= nt - d.nc) {
                 const int r1 = s2r[t], r2 = s2r[t + nt];
-                const int maxch = (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk;
                 if (d.lat_dim == 0 && d.deg_mean >= kIcmBlockDeg) {
                     // wide levels: the whole workgroup grows the cluster and sums
                     // its boundary; every thread gets both sums
                     long long dE1 = 0, dE2 = 0;
-                    icm_team<Fmt, true>(csr, maxch, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
+                    icm_team<Fmt, true>(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                         pub_sp + par * c.pub_words + (int64_t)r1 * nw,
                                         pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, tid, nthr,
                                         false);
@@ -440,7 +439,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                     // would only add barriers)
                     if (wave == 0) {
                         long long dE1 = 0, dE2 = 0;
-                        icm_team<Fmt, false>(csr, maxch, hf,
+                        icm_team<Fmt, false>(csr, hf,
                                              d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                              pub_sp + par * c.pub_words + (int64_t)r1 * nw,
                                              pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, lane, kWave,

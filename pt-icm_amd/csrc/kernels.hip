@@ -203,14 +203,13 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
     const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit};
     long long dE1 = 0, dE2 = 0;
-    const int maxch = (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk;
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
     if (W > 1 && d.lat_dim > 0) {  // lattices grow as bit sets on the first wave
         if (tid >= kWave) return;
-        icm_team<Fmt, false>(csr, maxch, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i, g1,
+        icm_team<Fmt, false>(csr, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i, g1,
                              g2, sc, dE1, dE2, tid, kWave, true, lat_view(a, d));
     } else {
-        icm_team<Fmt, (W > 1)>(csr, maxch, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i,
+        icm_team<Fmt, (W > 1)>(csr, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i,
                                g1, g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{});
     }
     if (tid == 0) {
@@ -397,7 +396,7 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
                 const int t = nt - 1 - j;
                 const int r1 = s2r[t], r2 = s2r[t + nt];
                 long long dE1 = 0, dE2 = 0;
-                icm_team<Fmt, false>(csr, (d.max_deg + Csr<Fmt>::kPerChunk - 1) / Csr<Fmt>::kPerChunk, hf,
+                icm_team<Fmt, false>(csr, hf,
                                      d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                      a.spins + d.spin_off + (int64_t)r1 * nw, a.spins + d.spin_off + (int64_t)r2 * nw,
                                      sc, dE1, dE2, lane, kWave);

@@ -1174,7 +1174,7 @@ __device__ __forceinline__ int team_sum_i32(int v, long long* red, int tid, int 
 }
 
 template <int Fmt, bool Block>
-__device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const int32_t* hf, bool has_field, int n,
+__device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf, bool has_field, int n,
                                          int nw, int t, uint32_t step, uint64_t seed, uint32_t c3,
                                          const uint32_t* g1_in, const uint32_t* g2_in, IcmScratch sc,
                                          long long& dE1_out, long long& dE2_out, int tid, int nthr,
@@ -1299,20 +1299,31 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, int maxch, const i
                 // of a cluster site is outside the cluster exactly when it
                 // is outside D (the cluster is D's component)
                 int b1 = 0;
-                for (int ch = 0; ch < maxch; ++ch) {
+                // chunks up to the longest row among the wave's queued sites
+                // (lanes without a site have none), not the instance's
+                const int wch = wave_max_i32(nch);
+                for (int ch = 0; ch < wch; ++ch) {
                     const int live = ch < nch ? min(K, dg - ch * K) : 0;
-                    // Every load and claim below is issued unconditionally (the
-                    // chunk index is clamped to the row, padding records name the
-                    // in-range sentinel word) so that they overlap; liveness only
-                    // masks the results.
+                    // A lane loads a chunk only while its row has one; the loads
+                    // of a chunk's records are issued together so that they
+                    // overlap, and liveness masks the padding records' results.
                     int col[K], wgt[K];
-                    csr.chunk(row, min(ch, max(nch - 1, 0)), col, wgt);
                     uint32_t dwd[K], vwd[K], swd[K];
+                    if (live > 0) {
+                        csr.chunk(row, ch, col, wgt);
     #pragma unroll
-                    for (int h = 0; h < K; ++h) {
-                        dwd[h] = D[col[h] >> 5];
-                        vwd[h] = V[col[h] >> 5];
-                        swd[h] = S1[col[h] >> 5];
+                        for (int h = 0; h < K; ++h) {
+                            dwd[h] = D[col[h] >> 5];
+                            vwd[h] = V[col[h] >> 5];
+                            swd[h] = S1[col[h] >> 5];
+                        }
+                    } else {
+    #pragma unroll
+                        for (int h = 0; h < K; ++h) {
+                            col[h] = 0;
+                            wgt[h] = 0;
+                            dwd[h] = vwd[h] = swd[h] = 0u;
+                        }
                     }
                     uint32_t cand = 0;
     #pragma unroll
