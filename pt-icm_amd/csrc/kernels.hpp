@@ -960,8 +960,11 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
         }
         wave_sync();
     } else if (NW <= kLatSlots * kWave) {
-        // up to kLatSlots words per lane kept in registers; each round reads every
-        // neighbour word first, then writes the grown words
+        // up to kLatSlots words per lane kept in registers, updated in place
+        // slot by slot (monotone: a neighbour read before or after its own
+        // update this round only changes when a site joins, not whether), so
+        // a later slot already sees the earlier slots' growth; one wave
+        // barrier per round
         const int J = (NW + kWave - 1) / kWave;
         uint32_t dreg[kLatSlots], vreg[kLatSlots];
 #pragma unroll
@@ -972,10 +975,9 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
             vreg[j] = own ? Vo[w] : 0u;
         }
         while (true) {
-            uint32_t nb[kLatSlots];
+            bool grew = false;
 #pragma unroll
             for (int j = 0; j < kLatSlots; ++j) {
-                nb[j] = 0u;
                 const int w = lane + kWave * j;
                 if (j >= J || w >= NW || vreg[j] == dreg[j]) continue;
                 const int row = w / R, k = w - row * R;
@@ -994,17 +996,9 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
                     if (k == R - 1) nx &= lastmask;
                     n |= nx;
                 }
-                nb[j] = n;
-            }
-            wave_sync();  // every read of this round before any write
-            bool grew = false;
-#pragma unroll
-            for (int j = 0; j < kLatSlots; ++j) {
-                const int w = lane + kWave * j;
-                if (j >= J || w >= NW) continue;
-                uint32_t nv = vreg[j] | (nb[j] & dreg[j]);
+                uint32_t nv = v | (n & dreg[j]);
                 if (R == 1) nv = close_x(nv, dreg[j]);
-                if (nv != vreg[j]) {
+                if (nv != v) {
                     vreg[j] = nv;
                     Vo[w] = nv;
                     grew = true;
