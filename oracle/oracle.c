@@ -491,6 +491,32 @@ int32_t or_colour(const or_graph* g, int32_t* colour) {
     free(heap);
     free(sat);
     free(nb);
+    /* balancing pass: a class above cap = ceil(n / ncls) gives vertices, in
+       index order, to the lowest-numbered class with room and none of their
+       neighbours (the product's colouring does the same) */
+    {
+        const int32_t cap = (n + ncls - 1) / ncls;
+        int32_t* size = (int32_t*)calloc((size_t)ncls, sizeof(int32_t));
+        uint8_t* nbc = (uint8_t*)calloc((size_t)ncls, 1);
+        for (int32_t i = 0; i < n; ++i) size[colour[i]]++;
+        for (int32_t c = 0; c < ncls; ++c) {
+            for (int32_t v = 0; v < n && size[c] > cap; ++v) {
+                if (colour[v] != c) continue;
+                for (int32_t p = g->rowptr[v]; p < g->rowptr[v + 1]; ++p) nbc[colour[g->col[p]]] = 1;
+                for (int32_t c2 = 0; c2 < ncls; ++c2) {
+                    if (c2 != c && size[c2] < cap && !nbc[c2]) {
+                        colour[v] = c2;
+                        size[c]--;
+                        size[c2]++;
+                        break;
+                    }
+                }
+                for (int32_t p = g->rowptr[v]; p < g->rowptr[v + 1]; ++p) nbc[colour[g->col[p]]] = 0;
+            }
+        }
+        free(nbc);
+        free(size);
+    }
     return ncls;
 }
 

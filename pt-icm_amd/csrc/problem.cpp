@@ -135,6 +135,32 @@ Status Problem::finalize() {
             }
         }
     }
+    // Balancing pass: a class above cap = ceil(n / ncls) gives vertices, in
+    // index order, to the lowest-numbered class with room and none of
+    // their neighbours (the colouring stays proper, no class empties).
+    // Classes of at most half a workgroup's lanes let two lanes share each
+    // position in the cooperative sweep.
+    {
+        const int32_t cap = (n + ncls - 1) / ncls;
+        std::vector<int32_t> size(ncls, 0);
+        for (int32_t i = 0; i < n; ++i) size[colour[i]]++;
+        std::vector<uint8_t> nbc(ncls, 0);
+        for (int32_t c = 0; c < ncls; ++c) {
+            for (int32_t v = 0; v < n && size[c] > cap; ++v) {
+                if (colour[v] != c) continue;
+                for (int32_t q = rowptr[v]; q < rowptr[v + 1]; ++q) nbc[colour[col[q]]] = 1;
+                for (int32_t c2 = 0; c2 < ncls; ++c2) {
+                    if (c2 != c && size[c2] < cap && !nbc[c2]) {
+                        colour[v] = c2;
+                        size[c]--;
+                        size[c2]++;
+                        break;
+                    }
+                }
+                for (int32_t q = rowptr[v]; q < rowptr[v + 1]; ++q) nbc[colour[col[q]]] = 0;
+            }
+        }
+    }
     class_start.assign(ncls + 1, 0);
     for (int32_t i = 0; i < n; ++i) class_start[colour[i] + 1]++;
     for (int32_t c = 0; c < ncls; ++c) class_start[c + 1] += class_start[c];
