@@ -194,14 +194,15 @@ def test_colouring_proper(oracle):
 def test_colouring_balanced(oracle):
     """The balancing pass after DSatur leaves every class within a few
     positions of ceil(n / k) on the reference graphs (G022: 8 classes of
-    247-253 instead of 127-284), with the class count unchanged."""
-    for name, k_want in [("G022", 8), ("G039", 7)]:
+    247-253 instead of 127-284; G039: 7 classes of 275-291), with the class
+    count unchanged."""
+    for name, k_want, slack in [("G022", 8, 3), ("G039", 7, 5)]:
         g = oracle.Graph.read(graph_path(name))
         k, c = g.colour()
         assert k == k_want
         sizes = np.bincount(c, minlength=k)
         cap = -(-g.n // k)
-        assert sizes.min() > 0 and sizes.max() <= cap + 3, sizes
+        assert sizes.min() > 0 and sizes.max() <= cap + slack, sizes
 
 
 # -------------------------------------------------------------- ladders -----
