@@ -61,7 +61,7 @@ typedef struct {
     int64_t nnz;          /* directed adjacency entries */
     int32_t sparse;       /* is_sparse(), problem.hh:74 */
     double char_scale;    /* get_char_scale(), problem.hh:66 */
-    int32_t n_colours;    /* colour classes (DSatur) */
+    int32_t n_colours;    /* colour classes (DSatur, then sizes balanced towards ceil(n / n_colours)) */
     int32_t fmax;         /* max_i |h_i| + sum_j |w_ij| */
     int64_t e_min_bound;  /* -(sum |w| over edges + sum |h|) */
     int64_t e_max_bound;
