@@ -75,15 +75,16 @@ __device__ __forceinline__ void wave_sync() {
 // Wave-wide sum and maximum with DPP row shifts and row broadcasts (a
 // prefix scan whose last lane holds the total; VALU only, no LDS crossbar).
 // Call with all 64 lanes active.
-__device__ __forceinline__ int wave_sum_i32(int v) {
+__device__ __forceinline__ int wave_prefix_i32(int v) {  // inclusive: lane i gets v_0 + ... + v_i
     v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
     v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
     v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xe, false);  // row_shr:4, banks 1-3
     v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xc, false);  // row_shr:8, banks 2-3
     v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
     v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-    return __builtin_amdgcn_readlane(v, 63);
+    return v;
 }
+__device__ __forceinline__ int wave_sum_i32(int v) { return __builtin_amdgcn_readlane(wave_prefix_i32(v), 63); }
 // Sum over aligned groups of T lanes (T = 1, 2 or 4) of a quad with DPP quad
 // permutations; every lane of a group gets the group's sum.  The lanes of a
 // group must be active together.
@@ -1222,13 +1223,8 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
                 const int w = w0 + lane;
                 const uint32_t dw = w < nw ? D[w] : 0u;
                 const int c = __popc(dw);
-                int incl = c;
-#pragma unroll
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const int y = __shfl_up(incl, o, kWave);
-                    if (lane >= o) incl += y;
-                }
-                const int total = __shfl(incl, kWave - 1, kWave);
+                const int incl = wave_prefix_i32(c);  // DPP scan, no LDS crossbar
+                const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
                 if (k < base + total) {
                     const bool mine = (k < base + incl) && (k >= base + incl - c);
                     const unsigned long long ball = __ballot(mine);
@@ -1252,7 +1248,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
                         }
                         sel = 32 * w + at;
                     }
-                    seed_pos = __shfl(sel, owner, kWave);
+                    seed_pos = __builtin_amdgcn_readlane(sel, owner);  // owner is wave-uniform
                 }
                 base += total;
             }
