@@ -556,7 +556,7 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
             is_pair ? pick(philox4x32_10((uint32_t)kc >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)kc) : 0u;
         int rank = 0;
         for (int i = 0; i < np; ++i) {
-            const uint32_t ki = __shfl(key, i, kWave);  // lane i < np holds pair i of chain 0
+            const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)key, i);  // lane i < np: pair i of chain 0
             rank += (ki < key) || (ki == key && i < kc);
         }
         const int a = h * np + rank;
@@ -594,7 +594,11 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
     int rep = valid ? s2r[lane] : 0;
     int32_t es = valid ? E[rep] : 0;
     const double bk = valid ? beta[lane] : 0.0;
-    const double bk1 = __shfl_down(bk, 1, kWave);  // beta of slot k + 1 (used only at a decision)
+    // beta of slot k + 1 (used only at a decision), by DPP on both halves
+    const unsigned long long bkb = (unsigned long long)__double_as_longlong(bk);
+    const double bk1 = __longlong_as_double(
+        (long long)(((unsigned long long)(uint32_t)lane_next((int)(uint32_t)(bkb >> 32)) << 32) |
+                    (uint32_t)lane_next((int)(uint32_t)bkb)));
     uint32_t natt = 0, nsucc = 0;
     unsigned long long nsw = 0;
 #pragma unroll
@@ -1351,7 +1355,7 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
                     if (Block) {
                         int wbase = 0;
                         if (lane == 0 && total > 0) wbase = atomicAdd(sc.tail, total);
-                        pos = __shfl(wbase, 0, kWave) + excl;
+                        pos = __builtin_amdgcn_readfirstlane(wbase) + excl;
                     } else {
                         pos = tail + excl;
                         tail += total;
