@@ -405,14 +405,15 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
             if (r == 0) record_histogram(a, d, step, E, s2r, lane);
             if (!a.no_exchange) {
                 if (prepared) {
+                    // one lane per slot: r2s written from the lane's own registers
                     const ExchPre x = pre[(step & 1u) * kWave + lane];
-                    nsw_total += exch_apply(nt, nrep, a.pilot != 0, x, beta, E, s2r, att, succ, lane);
+                    nsw_total += exch_apply(nt, nrep, a.pilot != 0, x, beta, E, s2r, att, succ, lane, r2s);
                 } else {
                     nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta,
                                                E, s2r, att, succ, xs, lane);
+                    wave_sync();
+                    for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
                 }
-                wave_sync();
-                for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
             }
         }
         __syncthreads();

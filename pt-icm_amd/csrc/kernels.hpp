@@ -585,9 +585,11 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
     return x;
 }
 
+// r2s (optional): also written, r2s[replica] = slot, from the same registers.
 __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool pilot, const ExchPre& x,
                                                          const double* beta, const int32_t* E, int32_t* s2r,
-                                                         uint32_t* att, uint32_t* succ, int lane) {
+                                                         uint32_t* att, uint32_t* succ, int lane,
+                                                         int32_t* r2s = nullptr) {
     const int nhalf = pilot ? 1 : 2;
     const bool valid = lane < nrep;
     const int kc = valid ? lane % nt : 0;
@@ -633,6 +635,7 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
     }
     if (valid) {
         s2r[lane] = rep;
+        if (r2s) r2s[rep] = lane;
         att[lane] += natt;
         succ[lane] += nsucc;
     }
