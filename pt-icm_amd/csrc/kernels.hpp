@@ -792,6 +792,62 @@ __device__ __forceinline__ LatView lat_view(const DevArgs& a, const InstDesc& d)
                    d.lat_words, d.jn_off >= 0 ? a.jneg + d.jn_off : nullptr};
 }
 
+// x closure of one lattice row of L sites in RR words (bits >= L zero),
+// wrapped at L: v (seeds, a subset of d) becomes the union of the runs of d
+// that hold a seed.  Upward within each run by one addition, a & ~(a + s) |
+// s (the carry from a run's lowest seed clears the run up to its top; a
+// second seed in the run is put back by | s), downward the same on the
+// bit-reversed field, then once more if the run through the wrap joins a
+// filled half: O(RR) instructions instead of one site per iteration.
+template <int RR>
+__device__ __forceinline__ void fill_row(uint32_t* v, const uint32_t* d, int L) {
+    const int sh = 32 * RR - L;      // 0..31: the reversed field's offset
+    const int lb = L - 32 * (RR - 1);  // sites in the last word
+    auto up_fill = [&](const uint32_t* a, const uint32_t* s, uint32_t* f) {
+        uint32_t carry = 0u;
+#pragma unroll
+        for (int k = 0; k < RR; ++k) {
+            const unsigned long long t = (unsigned long long)a[k] + s[k] + carry;
+            carry = (uint32_t)(t >> 32);
+            f[k] = (a[k] & ~(uint32_t)t) | s[k];
+        }
+    };
+    auto rev = [&](const uint32_t* x, uint32_t* y) {  // bit i of the field -> bit L - 1 - i
+        uint32_t t[RR];
+#pragma unroll
+        for (int k = 0; k < RR; ++k) t[k] = __brev(x[RR - 1 - k]);
+#pragma unroll
+        for (int k = 0; k < RR; ++k) {
+            const uint32_t hi = (k + 1 < RR && sh != 0) ? (t[k + 1] << (32 - sh)) : 0u;
+            y[k] = (sh != 0 ? (t[k] >> sh) : t[k]) | hi;
+        }
+    };
+    uint32_t dr[RR];
+    rev(d, dr);
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t fu[RR], sr[RR], fdr[RR], fd[RR];
+        up_fill(d, v, fu);
+        rev(v, sr);
+        up_fill(dr, sr, fdr);
+        rev(fdr, fd);
+#pragma unroll
+        for (int k = 0; k < RR; ++k) v[k] = fu[k] | fd[k];
+        // the wrap: sites L - 1 and 0 are neighbours
+        const bool top = ((v[RR - 1] >> (lb - 1)) & 1u) != 0u, bot = (v[0] & 1u) != 0u;
+        const bool dtop = ((d[RR - 1] >> (lb - 1)) & 1u) != 0u, dbot = (d[0] & 1u) != 0u;
+        bool more = false;
+        if (top && dbot && !bot) {
+            v[0] |= 1u;
+            more = true;
+        }
+        if (bot && dtop && !top) {
+            v[RR - 1] |= 1u << (lb - 1);
+            more = true;
+        }
+        if (!more) break;
+    }
+}
+
 // Grows the cluster of seed_pos (a position in D) as a bit set in lattice
 // order until no word changes, then writes it to V (position order) and its
 // positions to Q (with edge signs instead S1 in lattice order to the Q area,
@@ -889,10 +945,9 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
     } else if (lv.dim == 2 && R <= 4 && rows <= 2 * kWave) {
         // 2D, rows of up to 4 words and up to 128 rows: a lane holds rows
         // lane and lane + 64 whole, in registers.  A row closes along x in
-        // registers (shifts carried across its words, wrapped at bit L-1);
-        // rows y +- 1 come by lane shuffles (from either register slot).
+        // registers (fill_row: additions carried across its words, wrapped
+        // at bit L-1); rows y +- 1 come by DPP wave shifts and readlane.
         uint32_t d[2][4], v[2][4];
-        int up[2], dn[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int row = lane + kWave * j;
@@ -903,53 +958,51 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
                 d[j][k] = in ? Do[row * R + k] : 0u;
                 v[j][k] = in ? Vo[row * R + k] : 0u;
             }
-            up[j] = own ? (row + 1 == rows ? 0 : row + 1) : 0;
-            dn[j] = own ? (row == 0 ? rows - 1 : row - 1) : 0;
         }
-        // bit x of the row takes bits x - 1 and x + 1 (wrapped) where d allows,
-        // until the row no longer changes
+        const bool two = rows > kWave;  // rows 64.. in slot 1
+        const int last = rows - 1;
+        // the row's x closure within d (wrapped), by fill_row
         auto close_row = [&](uint32_t* rv, const uint32_t* rd) {
-            while (true) {
-                uint32_t nx[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    if (k >= R) {
-                        nx[k] = 0u;
-                        continue;
-                    }
-                    const uint32_t cin = k > 0 ? (rv[k - 1] >> 31) : ((rv[R - 1] >> (lb - 1)) & 1u);
-                    const uint32_t cout = k < R - 1 ? ((rv[k + 1] & 1u) << 31) : ((rv[0] & 1u) << (lb - 1));
-                    uint32_t x = (rv[k] << 1) | cin | (rv[k] >> 1) | cout;
-                    if (k == R - 1) x &= lastmask;
-                    nx[k] = rv[k] | (x & rd[k]);
-                }
-                bool same = true;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    same &= nx[k] == rv[k];
-                    rv[k] = nx[k];
-                }
-                if (same) return;
+            switch (R) {
+                case 1: fill_row<1>(rv, rd, L); break;
+                case 2: fill_row<2>(rv, rd, L); break;
+                case 3: fill_row<3>(rv, rd, L); break;
+                default: fill_row<4>(rv, rd, L); break;
             }
         };
         while (true) {
             bool grew = false;
             uint32_t nv[2][4];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    // the same word of rows up/dn, from whichever slot holds them
-                    const uint32_t u0 = __shfl(v[0][k], up[j] & (kWave - 1), kWave);
-                    const uint32_t u1 = __shfl(v[1][k], up[j] & (kWave - 1), kWave);
-                    const uint32_t w0 = __shfl(v[0][k], dn[j] & (kWave - 1), kWave);
-                    const uint32_t w1 = __shfl(v[1][k], dn[j] & (kWave - 1), kWave);
-                    const uint32_t ny = (up[j] >= kWave ? u1 : u0) | (dn[j] >= kWave ? w1 : w0);
-                    nv[j][k] = v[j][k] | (ny & d[j][k]);
+            for (int k = 0; k < 4; ++k) {
+                if (k >= R) {  // wave-uniform
+                    nv[0][k] = nv[1][k] = 0u;
+                    continue;
+                }
+                // the same word of rows y +- 1: lane +- 1 of the same slot by
+                // DPP wave shifts; the rows across a slot or the wrap
+                // (0, 63, 64, rows - 1) by readlane
+                const uint32_t a0 = v[0][k], a1 = v[1][k];
+                const uint32_t r0 = __builtin_amdgcn_readlane(a0, 0);
+                const uint32_t rl = __builtin_amdgcn_readlane(last >= kWave ? a1 : a0, last & (kWave - 1));
+                const uint32_t n0 = (uint32_t)lane_next((int)a0), p0 = (uint32_t)lane_prev((int)a0);
+                uint32_t u = lane == last ? r0 : n0;
+                if (two && lane == kWave - 1) u = __builtin_amdgcn_readlane(a1, 0);
+                const uint32_t w = lane == 0 ? rl : p0;
+                nv[0][k] = a0 | ((u | w) & d[0][k]);
+                if (two) {
+                    const uint32_t r63 = __builtin_amdgcn_readlane(a0, kWave - 1);
+                    const uint32_t n1 = (uint32_t)lane_next((int)a1), p1 = (uint32_t)lane_prev((int)a1);
+                    const uint32_t u1 = lane + kWave == last ? r0 : n1;
+                    const uint32_t w1 = lane == 0 ? r63 : p1;
+                    nv[1][k] = a1 | ((u1 | w1) & d[1][k]);
+                } else {
+                    nv[1][k] = 0u;
                 }
             }
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
+                if (j == 1 && !two) break;  // slot 1 stays empty
                 close_row(nv[j], d[j]);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
