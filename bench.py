@@ -292,7 +292,7 @@ def main():
                 k: r["batched_surrogate_judged_in_run"][k]
                 for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
                           "done_in_validation", "fallback", "ladder_s", "judge_at_s", "fallback_ready_s",
-                          "validate_steps", "wall_s")},
+                          "validate_steps", "steps_accepted_run", "steps_fallback_run", "wall_s")},
             "batched_multihistogram": {k: r["batched_multihistogram"][k]
                                        for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
                                                  "ladder_s", "run_s")},
