@@ -170,13 +170,16 @@ def test_icm_all_slots_parity(ptx, oracle, mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("dim,L", [(2, 33), (2, 40), (2, 64), (2, 100), (3, 5), (3, 12), (3, 24)])
+@pytest.mark.parametrize("dim,L", [(2, 33), (2, 40), (2, 64), (2, 80), (2, 100), (2, 129), (3, 5), (3, 12),
+                                   (3, 24)])
 def test_lattice_cluster_growth_parity(ptx, oracle, dim, L, mode):
     """Periodic lattices grow clusters as bit sets (split and cooperative
-    kernels; rows of one or several words, held in registers, in registers
-    and LDS, or updated in place in LDS (2D L = 100, 3D L = 24); odd L with
-    three colour classes): the oracle's breadth-first clusters bit for bit,
-    with ICM on every slot (global flips included)."""
+    kernels): 2D rows of 2, 3 or 4 words held whole in registers, one or two
+    rows per lane (L = 33 .. 100: carry-add row fills, rows y +- 1 by DPP
+    shifts), rows of one word per lane (3D L = 5), words in registers and
+    LDS (3D L = 12), or updated in place in LDS (2D L = 129, 3D L = 24); odd
+    L with three colour classes.  The oracle's breadth-first clusters bit
+    for bit, with ICM on every slot (global flips included)."""
     if dim == 2:
         n, u, v, w = instances.torus_2d(L, seed=L)
     else:
