@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ptx_common.h"
+#include "rowfill.hpp"
 
 namespace ptx {
 
@@ -790,62 +791,6 @@ __device__ __forceinline__ LatView lat_view(const DevArgs& a, const InstDesc& d)
     const uint32_t* t = a.lat + d.lat_off;  // wpos (16-bit) then obit
     return LatView{reinterpret_cast<const uint16_t*>(t), t + 16 * (int64_t)d.lat_words, d.lat_dim, d.lat_L, d.lat_R,
                    d.lat_words, d.jn_off >= 0 ? a.jneg + d.jn_off : nullptr};
-}
-
-// x closure of one lattice row of L sites in RR words (bits >= L zero),
-// wrapped at L: v (seeds, a subset of d) becomes the union of the runs of d
-// that hold a seed.  Upward within each run by one addition, a & ~(a + s) |
-// s (the carry from a run's lowest seed clears the run up to its top; a
-// second seed in the run is put back by | s), downward the same on the
-// bit-reversed field, then once more if the run through the wrap joins a
-// filled half: O(RR) instructions instead of one site per iteration.
-template <int RR>
-__device__ __forceinline__ void fill_row(uint32_t* v, const uint32_t* d, int L) {
-    const int sh = 32 * RR - L;      // 0..31: the reversed field's offset
-    const int lb = L - 32 * (RR - 1);  // sites in the last word
-    auto up_fill = [&](const uint32_t* a, const uint32_t* s, uint32_t* f) {
-        uint32_t carry = 0u;
-#pragma unroll
-        for (int k = 0; k < RR; ++k) {
-            const unsigned long long t = (unsigned long long)a[k] + s[k] + carry;
-            carry = (uint32_t)(t >> 32);
-            f[k] = (a[k] & ~(uint32_t)t) | s[k];
-        }
-    };
-    auto rev = [&](const uint32_t* x, uint32_t* y) {  // bit i of the field -> bit L - 1 - i
-        uint32_t t[RR];
-#pragma unroll
-        for (int k = 0; k < RR; ++k) t[k] = __brev(x[RR - 1 - k]);
-#pragma unroll
-        for (int k = 0; k < RR; ++k) {
-            const uint32_t hi = (k + 1 < RR && sh != 0) ? (t[k + 1] << (32 - sh)) : 0u;
-            y[k] = (sh != 0 ? (t[k] >> sh) : t[k]) | hi;
-        }
-    };
-    uint32_t dr[RR];
-    rev(d, dr);
-    for (int pass = 0; pass < 2; ++pass) {
-        uint32_t fu[RR], sr[RR], fdr[RR], fd[RR];
-        up_fill(d, v, fu);
-        rev(v, sr);
-        up_fill(dr, sr, fdr);
-        rev(fdr, fd);
-#pragma unroll
-        for (int k = 0; k < RR; ++k) v[k] = fu[k] | fd[k];
-        // the wrap: sites L - 1 and 0 are neighbours
-        const bool top = ((v[RR - 1] >> (lb - 1)) & 1u) != 0u, bot = (v[0] & 1u) != 0u;
-        const bool dtop = ((d[RR - 1] >> (lb - 1)) & 1u) != 0u, dbot = (d[0] & 1u) != 0u;
-        bool more = false;
-        if (top && dbot && !bot) {
-            v[0] |= 1u;
-            more = true;
-        }
-        if (bot && dtop && !top) {
-            v[RR - 1] |= 1u << (lb - 1);
-            more = true;
-        }
-        if (!more) break;
-    }
 }
 
 // Grows the cluster of seed_pos (a position in D) as a bit set in lattice
