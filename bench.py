@@ -32,6 +32,16 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
 # waves and 16 steps per launch.
 BATCHED_HBM_BYTES = {"k_sweep": (2 * 71276.8 + 38424.8) * 1024,
                      "k_fused": (2 * 69305.5 + 475067.6) * 1024}
+# HBM bytes of one k_coop launch on G022 (22 slots x 2 chains, 8 waves) as a
+# function of the PT steps it runs: a per-launch part (staging the adjacency,
+# thresholds and states into each workgroup's LDS) plus a per-step part (the
+# tagged energy slots, written and polled at agent scope, which crosses the
+# XCDs' L2s).  From rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# `microbench.py coop --launches 2 --short 20` (launches of 500 and of 20
+# steps), scaled by the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1):
+# 500 steps 7,800 KiB, 20 steps 1,607 KiB.
+COOP_HBM_KIB_PER_LAUNCH = 1349.0
+COOP_HBM_KIB_PER_STEP = 12.90
 
 
 def load_ptx():
@@ -55,7 +65,7 @@ def preprocess_ladder(ptx, problem, nt, T0, T1, seed, burn_in=5000, samples=2000
                        hist_start=burn_in)
     pilot.step(burn_in + samples)
     counts, e_base = pilot.histograms()
-    del pilot
+    pilot.close()
     return ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_FAST)
 
 
@@ -111,6 +121,9 @@ def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
     dt = time.perf_counter() - t0
     info = eng.info()
     icm = [eng.instance_info(i).nc for i in range(2)]
+    eng.close()
+    for q in probs:
+        q.close()
     return {"workload": f"{n_inst // 2} 2D +-J tori L=100 + {n_inst - n_inst // 2} ER +-1 G(5000, 0.004); "
                         "64 betas linear over T 0.5..3.0, 2 chains, ICM",
             "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
@@ -155,13 +168,17 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
     eng.sync()
     dt_graph = time.perf_counter() - t0
     info = eng.info()
-    if info.mode == 2:
+    if info.mode == 2:  # the timed launches cover `steps` PT steps between them
         kern, avg_ms = "k_fused", tm.fused_ms / max(tm.fused_launches, 1)
-        bpl = info.lds_bytes_per_step * info.steps_per_launch
+        steps_per_launch = steps / max(tm.fused_launches, 1)
     else:
         kern, avg_ms = "k_sweep", tm.sweep_ms / max(tm.sweep_launches, 1)
-        bpl = info.lds_bytes_per_step
+        steps_per_launch = steps / max(tm.sweep_launches, 1)
+    bpl = info.lds_bytes_per_step * steps_per_launch
     achieved = bpl / (avg_ms * 1e-3) / 1e9
+    eng.close()
+    for q in probs:
+        q.close()
     return {"workload": "4096 x 3D EA +-J, L in {6,8,10,12}, 32 betas 0.2..3.0, 2 chains, ICM",
             "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
             "ms_per_step": dt * 1e3 / steps, "steps": steps, "instances_per_hour_at_10k_steps":
@@ -177,7 +194,7 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
                          "traffic": BATCHED_HBM_BYTES["k_sweep"] if info.mode == 1 and info.waves_per_group == 1
                          else (BATCHED_HBM_BYTES["k_fused"] if info.mode == 2 and info.waves_per_group == 16
                                and info.steps_per_launch == 16 else None), "kernel": kern, "avg_launch_ms": avg_ms,
-                         "steps_per_launch": info.steps_per_launch}}
+                         "bytes_per_launch": bpl, "steps_per_timed_launch": steps_per_launch}}
 
 
 def main():
@@ -231,13 +248,21 @@ def main():
     info = eng.info()
     updates = info.spin_updates_per_step * args.steps
     value = updates / dt
-    if info.mode in (2, 3):  # one launch covers steps_per_launch PT steps
+    # algorithmic bytes per launch: the timed launches cover args.steps PT
+    # steps between them (a launch runs at most steps_per_launch of them)
+    if info.mode in (2, 3):
         kern = "k_fused" if info.mode == 2 else "k_coop"
-        avg_ms = tm.fused_ms / max(tm.fused_launches, 1)
-        bytes_per_launch = info.lds_bytes_per_step * info.steps_per_launch
+        launches = max(tm.fused_launches, 1)
+        avg_ms = tm.fused_ms / launches
     else:
-        kern, avg_ms = "k_sweep", tm.sweep_ms / max(tm.sweep_launches, 1)
-        bytes_per_launch = info.lds_bytes_per_step
+        kern, launches = "k_sweep", max(tm.sweep_launches, 1)
+        avg_ms = tm.sweep_ms / launches
+    steps_per_timed_launch = args.steps / launches
+    bytes_per_launch = info.lds_bytes_per_step * steps_per_timed_launch
+    # measured HBM bytes per launch (the counter runs' shape: G022, 22 slots, k_coop with 8 waves)
+    traffic = None
+    if kern == "k_coop" and nt == 22 and info.waves_per_group == 8:
+        traffic = (COOP_HBM_KIB_PER_LAUNCH + COOP_HBM_KIB_PER_STEP * steps_per_timed_launch) * 1024
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     lds_peak = max(ptx.lds_bandwidth_gbs() for _ in range(3))  # pinned by the microbenchmark
     be, bs, _ = eng.best(0, with_state=False)
@@ -265,12 +290,12 @@ def main():
                       "exchange": tm.exchange_ms / max(tm.exchange_launches, 1),
                       "icm": tm.icm_ms / max(tm.icm_launches, 1), "fused": tm.fused_ms / max(tm.fused_launches, 1)},
         "roofline": {"bound": "lds", "achieved": achieved, "peak": lds_peak, "unit": "GB/s",
-                     "frac": achieved / lds_peak, "traffic": None, "peak_nominal": LDS_PEAK_GBS,
+                     "frac": achieved / lds_peak, "traffic": traffic, "peak_nominal": LDS_PEAK_GBS,
                      "peak_source": "k_lds_read microbenchmark (16-byte LDS reads, all CUs)",
                      "kernel": kern, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
-                     "steps_per_launch": info.steps_per_launch},
+                     "timed_launches": launches, "steps_per_timed_launch": steps_per_timed_launch},
     }
-    del eng
+    eng.close()
     if args.batched_steps > 0:
         out["batched"] = batched_ea(ptx, args.batched_steps, args.seed)
         out["batched"]["roofline"]["peak"] = lds_peak
@@ -302,7 +327,8 @@ def main():
             "instances_per_hour_ratio_judged_in_run": r["instances_per_hour_ratio_judged_in_run"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
-    print(json.dumps(out))
+    p.close()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -154,6 +154,12 @@ typedef struct {
     int32_t small_batch_coop;/* split mode: while every replica of the active instances fits one
                                 cooperative launch (for example the last instances of a batch whose
                                 others were retired), run them with the cooperative kernel */
+    int32_t coop_wait_us;    /* cooperative launches: bound of one per-step wait for the other replicas
+                                of an instance, in microseconds (0 = 100,000).  A wait that exceeds it
+                                ends the launch; the next ptx_engine_step, ptx_engine_sync or read then
+                                returns PTX_ERR_DEVICE and no further launch is enqueued.  A negative
+                                value gives up at the first poll that finds a replica missing (tests of
+                                that error path). */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 

@@ -238,6 +238,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->csr_placement = PTX_CSR_AUTO;
     c->icm_waves = 0;
     c->small_batch_coop = 0;
+    c->coop_wait_us = 0;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,

@@ -21,7 +21,10 @@ struct CoopArgs {
     unsigned long long* pub_E;  // [2][replicas of all instances]: (step + 1) << 32 | energy, at each
                                 // instance's replica offset; a slot showing this step is the arrival
     const int2* items;          // (instance, replica) of each workgroup
-    uint32_t* err;              // set to 1 when a wait times out
+    uint32_t* err;              // set to 1 when a wait gives up (read by the other workgroups)
+    uint32_t* fault;            // host-mapped fault words: [0] set with err, for the host to read without a sync
+    uint64_t wait_ticks;        // bound of one step wait in wall-clock ticks; 0: give up at the first
+                                // poll that finds a replica missing (forces the error path in tests)
     int64_t pub_words;          // spin words of all instances (the stride of pub_spins' halves)
     int32_t pub_reps;           // replicas of all instances (the stride of pub_E's halves)
     int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
@@ -153,8 +156,10 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
 // wait.  The slots are double-buffered by step parity: a slot is
 // rewritten two steps later, after every replica of the instance has
 // published the step in between, which it does only after reading this one.
-// coop_wait returns false when the wait exceeded its bound (the error flag is
-// then set and every workgroup leaves the step loop).
+// coop_wait returns false when the wait exceeded its bound of wait_ticks
+// wall-clock ticks (100 MHz), or another workgroup gave up: the error flag is
+// then set, the host-mapped fault word too, and every workgroup leaves the
+// step loop, so the launch always ends.
 template <bool REL>
 __device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t step, int32_t E) {
     if (REL) __syncthreads();  // the spin words are published
@@ -164,13 +169,14 @@ __device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t s
 }
 
 template <bool ACQ>
-__device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int nrep, uint32_t step, uint32_t* err,
+__device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int nrep, uint32_t step, const CoopArgs& c,
                                           int32_t* E, int* lds_flag) {
     const int tid = threadIdx.x;
     if (tid < kWave) {
         const unsigned long long want = (unsigned long long)(step + 1u);
         int ok = 1;
         unsigned long long spins = 0;
+        const uint64_t t0 = wall_clock64();
         for (int s0 = 0; s0 < nrep; s0 += kWave) {
             const int s = s0 + tid;
             unsigned long long v = 0;
@@ -182,10 +188,14 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
                 }
                 if (__ballot(!ready) == 0ull) break;
                 __builtin_amdgcn_s_sleep(1);
-                // the error flag and the bound every 32 polls (a poll is one memory round trip)
-                if ((++spins & 31u) == 0 &&
-                    (spins > (1ull << 22) || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                    if (tid == 0) atomicOr(err, 1u);
+                // the error flag and the time bound every 32 polls (a poll is one memory round trip)
+                if (((++spins & 31u) == 0 || c.wait_ticks == 0) &&
+                    (wall_clock64() - t0 > c.wait_ticks ||
+                     __hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    if (tid == 0) {
+                        atomicOr(c.err, 1u);
+                        __hip_atomic_store(c.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
                     ok = 0;
                     break;
                 }
@@ -290,6 +300,12 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     sc.red = reinterpret_cast<long long*>(smem + L.red);
     sc.Do = reinterpret_cast<uint32_t*>(smem + L.lat);
     sc.Vo = sc.Do + d.lat_words;
+
+    // a wait of an earlier launch gave up: leave at once (one load, shared
+    // through LDS so that every wave takes the same branch)
+    if (tid == 0) *flag = (int)__hip_atomic_load(c.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (*flag) return;
 
     // ---- stage the instance and this replica ----
     if (!G) {  // G = 1: the adjacency is read through L2
@@ -397,7 +413,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (pre_rnd && wave > 0 && it + 1 < nsteps)
             fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
                                nthr - kWave);
-        if (!coop_wait<IC != 0>(pub_e + (int64_t)par * c.pub_reps, nrep, step, c.err, E, flag)) break;
+        if (!coop_wait<IC != 0>(pub_e + (int64_t)par * c.pub_reps, nrep, step, c, E, flag)) break;
         phase_done(3);
         // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
         //      wave 1 meanwhile prepares the next step's exchange ----

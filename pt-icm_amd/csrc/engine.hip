@@ -57,6 +57,19 @@ Engine::~Engine() {
         (void)hipEventDestroy(t.b);
     }
     if (stream) (void)hipStreamDestroy(stream);
+    if (h_fault) (void)hipHostFree(h_fault);
+}
+
+Status Engine::check_fault() const {
+    if (!h_fault) return Status::ok();
+    const uint32_t wait = __atomic_load_n(h_fault, __ATOMIC_ACQUIRE);
+    const uint32_t hist = __atomic_load_n(h_fault + 1, __ATOMIC_ACQUIRE);
+    if (wait)
+        return Status::err(PTX_ERR_DEVICE, "a cooperative step wait gave up (a replica's workgroup did not arrive "
+                                           "within the bound); the engine's state is no longer a valid trajectory");
+    if (hist)
+        return Status::err(PTX_ERR_DEVICE, std::to_string(hist) + " histogram samples fell outside their table");
+    return Status::ok();
 }
 
 namespace {
@@ -81,6 +94,19 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (hipGetDeviceCount(&device_count) != hipSuccess || device_count == 0)
         return Status::err(PTX_ERR_DEVICE, "no HIP device available");
     PTX_TRY(hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate"));
+    PTX_TRY(hip_check(hipHostMalloc((void**)&h_fault, 4 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent),
+                      "hipHostMalloc"));
+    std::memset(h_fault, 0, 4 * sizeof(uint32_t));
+    PTX_TRY(hip_check(hipHostGetDevicePointer((void**)&d_fault, h_fault, 0), "hipHostGetDevicePointer"));
+    {
+        int dev = 0, rate_khz = 0;
+        PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+        PTX_TRY(hip_check(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate"));
+        // default bound 100 ms: a PT step of one replica's workgroup takes microseconds to a
+        // millisecond, so a longer wait means a partner that is not running
+        const int64_t us = cfg.coop_wait_us > 0 ? cfg.coop_wait_us : (cfg.coop_wait_us == 0 ? 100000 : 0);
+        wait_ticks = (uint64_t)us * (uint64_t)(rate_khz > 0 ? rate_khz : 100000) / 1000u;
+    }
 
     // ---- host layout ----
     desc.resize(ninst);
@@ -436,6 +462,7 @@ static DevArgs build_args(Engine& e) {
     a.lat = e.d_lat.p;
     a.jneg = e.d_jneg.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
+    a.fault = e.d_fault;
     a.inst_step = e.d_inst_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
@@ -683,6 +710,8 @@ Status Engine::launch_coop(int nsteps) {
     c.pub_E = d_pub_E.p;
     c.items = d_coop_items.p;
     c.err = d_err.p;
+    c.fault = d_fault;
+    c.wait_ticks = wait_ticks;
     c.pub_words = spin_words;
     c.pub_reps = (int32_t)total_rep;
     c.pre_rnd = cf.pre_rnd;
@@ -708,6 +737,7 @@ Status Engine::launch_coop(int nsteps) {
 
 Status Engine::step(int64_t n) {
     if (n < 0) return Status::err(PTX_ERR_INVALID, "negative step count");
+    PTX_TRY(check_fault());  // no more launches after a reported fault
     if (inst_list.empty()) return Status::ok();  // every instance retired
     if (steps_done + n > 0xFFFFFFFFll) return Status::err(PTX_ERR_UNSUPPORTED, "more than 2^32 PT steps");
     // cooperative launches: the cooperative mode, or a split-mode engine
@@ -717,6 +747,7 @@ Status Engine::step(int64_t n) {
         last_mode = PTX_MODE_COOP;
         while (n > 0) {
             const int k = (int)std::min<int64_t>(n, coop_K);
+            PTX_TRY(check_fault());
             PTX_TRY(launch_coop(k));
             steps_done += k;
             n -= k;
@@ -877,12 +908,7 @@ Status Engine::build_graph() {
 
 Status Engine::sync() {
     PTX_TRY(hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"));
-    if (d_err.p) {
-        uint32_t err = 0;
-        PTX_TRY(hip_check(hipMemcpy(&err, d_err.p, sizeof(err), hipMemcpyDeviceToHost), "read error flag"));
-        if (err) return Status::err(PTX_ERR_DEVICE, "cooperative step wait timed out");
-    }
-    return Status::ok();
+    return check_fault();
 }
 
 Status Engine::collect_timing() {
@@ -936,6 +962,7 @@ static Status check_inst(const Engine& e, int32_t inst) {
 
 Status Engine::read_spins(int32_t inst, uint8_t* out) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     const Problem& P = *probs[inst];
     std::vector<uint32_t> w((size_t)d.nrep * d.nw);
@@ -963,6 +990,7 @@ Status Engine::write_spins(int32_t inst, const uint8_t* in) {
 
 Status Engine::read_energies(int32_t inst, int64_t* out) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     std::vector<int32_t> e(d.nrep);
     PTX_TRY(d_E.download(e.data(), e.size(), (size_t)d.rep_off, stream));
@@ -972,12 +1000,14 @@ Status Engine::read_energies(int32_t inst, int64_t* out) {
 
 Status Engine::read_perm(int32_t inst, int32_t* out) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     return d_slot2rep.download(out, (size_t)d.nrep, (size_t)d.rep_off, stream);
 }
 
 Status Engine::read_swap_stats(int32_t inst, uint32_t* att, uint32_t* succ, uint64_t* swaps) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     PTX_TRY(d_att.download(att, (size_t)d.nrep, (size_t)d.rep_off, stream));
     PTX_TRY(d_succ.download(succ, (size_t)d.nrep, (size_t)d.rep_off, stream));
@@ -989,6 +1019,7 @@ Status Engine::read_swap_stats(int32_t inst, uint32_t* att, uint32_t* succ, uint
 
 Status Engine::read_flips(int32_t inst, uint64_t* out) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     std::vector<unsigned long long> f(d.nrep);
     PTX_TRY(d_flips.download(f.data(), f.size(), (size_t)d.rep_off, stream));
@@ -998,6 +1029,7 @@ Status Engine::read_flips(int32_t inst, uint64_t* out) {
 
 Status Engine::read_best(int32_t inst, int64_t* be, int64_t* bs, uint8_t* st) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     const Problem& P = *probs[inst];
     std::vector<int32_t> e(d.nrep), s(d.nrep);
@@ -1019,12 +1051,14 @@ Status Engine::read_best(int32_t inst, int64_t* be, int64_t* bs, uint8_t* st) {
 
 Status Engine::read_histograms(int32_t inst, uint32_t* out) {
     PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
     const InstDesc& d = desc[inst];
     if (!(cfg.flags & PTX_FLAG_HISTOGRAM)) return Status::err(PTX_ERR_INVALID, "engine built without histograms");
     return d_hist.download(out, (size_t)d.nrep * d.hist_bins, (size_t)d.hist_off, stream);
 }
 
 Status Engine::read_min_energies(int64_t* out) {
+    PTX_TRY(sync());
     std::vector<int32_t> e((size_t)total_rep);
     PTX_TRY(d_bestE.download(e.data(), e.size(), 0, stream));
     for (int32_t i = 0; i < ninst; ++i) {

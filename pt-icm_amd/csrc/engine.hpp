@@ -122,6 +122,12 @@ struct Engine {
     // and replayed (the kernels read their step counter from device memory)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // fault words in pinned, device-mapped host memory, written by kernels
+    // only on a fault and read by the host without a synchronisation:
+    // [0] a cooperative step wait gave up, [1] histogram samples outside their table
+    uint32_t* h_fault = nullptr;
+    uint32_t* d_fault = nullptr;  // the same words as the kernels address them
+    uint64_t wait_ticks = 0;      // bound of one cooperative step wait (wall-clock ticks)
 
     ~Engine();
     Status create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -159,6 +165,9 @@ struct Engine {
     int icm_waves_now() const { return Wi > 0 ? Wi : 1; }
 
     Status collect_timing();
+    // PTX_ERR_DEVICE once a kernel has reported a fault (sticky: the state is
+    // no longer a valid trajectory)
+    Status check_fault() const;
 };
 
 Status hip_check(hipError_t e, const char* what);

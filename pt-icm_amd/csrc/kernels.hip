@@ -122,8 +122,10 @@ __device__ __forceinline__ void record_histogram(const DevArgs& a, const InstDes
     if (!a.hist_on || (int64_t)step < a.hist_start) return;
     for (int s = lane; s < d.nrep; s += kWave) {
         const int bin = (E[s2r[s]] - d.e_base) >> 1;
-        // energies lie in [e_base, -e_base]; a bin outside the table is never written
+        // energies lie in [e_base, -e_base]; a sample outside the table is an
+        // engine fault: it is counted (fault word 1, host-mapped) instead of written
         if (bin >= 0 && bin < d.hist_bins) atomicAdd(&a.hist[d.hist_off + (int64_t)s * d.hist_bins + bin], 1u);
+        else __hip_atomic_fetch_add(a.fault + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

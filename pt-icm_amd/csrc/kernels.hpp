@@ -56,6 +56,8 @@ struct DevArgs {
     int32_t fmt;             // adjacency record format
     int32_t tab;             // 1: unit records carry the sign at bit kbit and k_sweep keeps the sign table
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
+    uint32_t* fault;         // host-mapped fault words: [0] a cooperative wait gave up, [1] histogram
+                             // samples outside their table
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
     int32_t sweeps;

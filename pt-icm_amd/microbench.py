@@ -5,6 +5,7 @@
     python pt-icm_amd/microbench.py copy --mib 256 --reps 5   # for counter calibration
     python pt-icm_amd/microbench.py batched --steps 32        # 4096 3D EA instances, autotuned launch shape
     python pt-icm_amd/microbench.py batched --mode 1 --waves 1 --steps 32   # split kernels, one wave per replica
+    python pt-icm_amd/microbench.py coop --launches 4   # G022 single instance, cooperative kernel, 500 steps/launch
 
 The copy kernel (k_copy16) also calibrates the FETCH_SIZE / WRITE_SIZE
 counters: a run of it under rocprofv3 --pmc reads back known byte counts.
@@ -28,7 +29,11 @@ def load_ptx():
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["lds", "copy", "batched"])
+    ap.add_argument("what", choices=["lds", "copy", "batched", "coop"])
+    ap.add_argument("--launches", type=int, default=4, help="coop: timed launches of 500 PT steps")
+    ap.add_argument("--icm", action="store_true", help="coop: ascending ladder, so the cold slots run cluster moves")
+    ap.add_argument("--short", type=int, default=0,
+                    help="coop: afterwards two launches of this many PT steps (launch-length dependence of counters)")
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--steps", type=int, default=32)
@@ -62,6 +67,42 @@ def main():
                           "spin_updates_per_s": info.spin_updates_per_step * a.steps / dt,
                           "lds_bytes_per_step": info.lds_bytes_per_step,
                           "steps_per_launch": info.steps_per_launch}))
+    elif a.what == "coop":
+        # G022 with a fixed 22-slot ladder over the range the 64-slot
+        # preprocess ladder spans (beta 1.667 .. 0.367): descending, as the
+        # preprocess returns it, so the slot rule gives no cluster-move slot;
+        # --icm hands it over ascending (the cold slots then run cluster moves)
+        import numpy as np
+        p = ptx.Problem.read(os.path.join(os.path.dirname(HERE), "tests", "golden", "graphs", "G022"))
+        lad = np.linspace(1.0 / 0.6, 0.367, 22)
+        if a.icm:
+            lad = lad[::-1].copy()
+        eng = ptx.Engine(p, lad, seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_COOP, waves_per_group=8,
+                         spins_per_lane=1, steps_per_launch=500)
+        eng.step(500)
+        eng.sync()
+        eng.timing(True)
+        eng.timing()
+        eng.step(500 * a.launches)
+        eng.sync()
+        t = eng.timing()
+        info = eng.info()
+        nc = eng.instance_info().nc
+        short = None
+        if a.short > 0:
+            eng.step(a.short)
+            eng.step(a.short)
+            ts = eng.timing()  # accumulated since timing was switched on
+            k = ts.fused_launches - t.fused_launches
+            short = {"steps_per_launch": a.short, "launches": k,
+                     "avg_launch_ms": (ts.fused_ms - t.fused_ms) / max(k, 1)}
+        eng.close()
+        avg = t.fused_ms / max(t.fused_launches, 1)
+        print(json.dumps({"workload": "G022, 22 slots x 2 chains, cooperative kernel, 8 waves", "icm_slots": nc,
+                          "launches": t.fused_launches, "steps_per_launch": 500, "avg_launch_ms": avg,
+                          "us_per_step": avg * 1e3 / 500, "lds_bytes_per_step": info.lds_bytes_per_step,
+                          "lds_gbs": info.lds_bytes_per_step * 500 / (avg * 1e-3) / 1e9, "short": short}),
+              flush=True)
     elif a.what == "lds":
         gbs = max(ptx.lds_bandwidth_gbs() for _ in range(3))
         print(json.dumps({"lds_read_gbs": gbs}))

@@ -7,6 +7,7 @@ there is no CPU fallback behind it.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -39,7 +40,8 @@ class EngineConfig(C.Structure):
     _fields_ = [("chains", C.c_int32), ("sweeps", C.c_int32), ("flags", C.c_uint32), ("nc", C.c_int32),
                 ("hist_start", C.c_int64), ("mode", C.c_int32), ("waves_per_group", C.c_int32),
                 ("steps_per_launch", C.c_int32), ("use_graph", C.c_int32), ("spins_per_lane", C.c_int32),
-                ("csr_placement", C.c_int32), ("icm_waves", C.c_int32), ("small_batch_coop", C.c_int32)]
+                ("csr_placement", C.c_int32), ("icm_waves", C.c_int32), ("small_batch_coop", C.c_int32),
+                ("coop_wait_us", C.c_int32)]
 
 
 class EngineInfo(C.Structure):
@@ -197,12 +199,20 @@ class Problem:
                                            _ptr(w, C.c_double), hp, int(dense), C.byref(hh)))
         return cls(hh)
 
+    def close(self):
+        """Releases the problem now (its engines keep their own copy)."""
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            lib().ptx_problem_destroy(h)
+
     def __del__(self):
-        try:
-            if self.h:
-                lib().ptx_problem_destroy(self.h)
-        except Exception:
-            pass
+        # no library calls while the interpreter shuts down: the HIP runtime
+        # (or a profiler attached to it) may already be gone
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def energy(self, s):
         s = np.ascontiguousarray(s, np.uint8)
@@ -293,7 +303,7 @@ class Engine:
 
     def __init__(self, problems, ladders, seed=0, chains=2, sweeps=1, flags=0, nc=-1, hist_start=0, mode=MODE_AUTO,
                  waves_per_group=0, steps_per_launch=0, use_graph=0, spins_per_lane=0, csr_placement=CSR_AUTO,
-                 icm_waves=0, small_batch_coop=0):
+                 icm_waves=0, small_batch_coop=0, coop_wait_us=0):
         if isinstance(problems, Problem):
             problems = [problems]
             ladders = [ladders]
@@ -304,6 +314,7 @@ class Engine:
         cfg.hist_start, cfg.mode, cfg.waves_per_group = hist_start, mode, waves_per_group
         cfg.steps_per_launch, cfg.use_graph, cfg.spins_per_lane = steps_per_launch, use_graph, spins_per_lane
         cfg.csr_placement, cfg.icm_waves, cfg.small_batch_coop = csr_placement, icm_waves, small_batch_coop
+        cfg.coop_wait_us = coop_wait_us
         self.cfg = cfg
         arr = (C.c_void_p * len(self.problems))(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h
                                                   for p in self.problems])
@@ -315,12 +326,24 @@ class Engine:
         self.h = h
         self.ninst = len(self.problems)
 
+    def close(self):
+        """Releases the engine's device memory, stream and graphs now."""
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            lib().ptx_engine_destroy(h)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
-        try:
-            if self.h:
-                lib().ptx_engine_destroy(self.h)
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def step(self, n=1):
         check(lib().ptx_engine_step(self.h, n))
@@ -495,12 +518,18 @@ class Mlp:
         check(lib().ptx_mlp_param_count(self.h, C.byref(n)))
         self.nparams = n.value
 
+    def close(self):
+        """Releases the network's device memory and stream now."""
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            lib().ptx_mlp_destroy(h)
+
     def __del__(self):
-        try:
-            if self.h:
-                lib().ptx_mlp_destroy(self.h)
-        except Exception:
-            pass
+        if not sys.is_finalizing():
+            try:
+                self.close()
+            except Exception:
+                pass
 
     def params(self):
         out = np.zeros(self.nparams, np.float32)

@@ -150,7 +150,7 @@ def multihistogram_ladders(ptx, problems, nt, T0, T1, burn_in, samples, seed, in
     t0 = time.perf_counter()
     eng, hb, bs, be = pilot(ptx, problems, nt, T0, T1, burn_in, samples, seed, inverse)
     hists = [eng.histograms(i) for i in range(len(problems))]
-    del eng
+    eng.close()
     t1 = time.perf_counter()
 
     def one(h):

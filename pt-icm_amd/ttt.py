@@ -94,7 +94,7 @@ def calibrate(ptx, specs, seeds, steps, log=print):
         m = eng.min_energies()
         log(f"seed {sd}: {steps} steps in {time.perf_counter() - t0:.1f} s, improved {int(np.sum(m < best))}")
         best = np.minimum(best, m)
-        del eng
+        eng.close()
     return best, [len(l) for l in lads]
 
 
@@ -286,7 +286,7 @@ def run_plain(ptx, specs, targets, max_steps, poll, seed, log=print):
         rows.append({"pre_s": t_pre, "run_s": t_run, "steps": steps, "hit": bool(h >= 0),
                      "ttt_s": t_pre + (h + 1) * t_run / steps if h >= 0 else float("inf"),
                      "mode": eng.info().mode})
-        del eng
+        eng.close()
     wall = sum(r["pre_s"] + r["run_s"] for r in rows)
     return {"instances": len(rows), "hit_rate": float(np.mean([r["hit"] for r in rows])),
             "p50_ttt_s": float(np.median([r["ttt_s"] for r in rows])), "wall_s": wall,
