@@ -294,6 +294,18 @@ ptx_status ptx_mlp_get_params(ptx_mlp* m, float* params, int64_t count);
 ptx_status ptx_mlp_get_grads(ptx_mlp* m, float* grads, int64_t count);
 /* y[batch][out] = softplus(network(x[batch][in])). */
 ptx_status ptx_mlp_forward(ptx_mlp* m, const float* x, int32_t batch, float* y);
+/* ptx_mlp_forward without any allocation, for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  The network is given by its widths and flat
+ * parameters (the layout of ptx_mlp_set_params) instead of a handle; x, y as
+ * in ptx_mlp_forward, and the result equals ptx_mlp_forward's bit for bit.
+ * Works in a fixed 4 MiB device array per GPU (held until the process ends):
+ * parameters (FP32 and BF16 copies) plus the batch's activations must fit,
+ * else PTX_ERR_UNSUPPORTED (the 516-256-256-256-64 surrogate fits batches up
+ * to 288 rows).  Calls on one GPU share the array and are serialised
+ * by a lock; each call writes everything it reads.  Synchronous. */
+ptx_status ptx_mlp_forward_noalloc(const int32_t* dims, int32_t n_dims, const float* params, int64_t param_count,
+                                   const float* x, int32_t batch, float* y);
 /* Loss = (1/batch) sum over outputs of: mode 1 (y - t)^2, mode 2
  * max(0, t - y)^2, mode 0 nothing; computes all gradients on the device. */
 ptx_status ptx_mlp_loss_grad(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int32_t batch,

@@ -501,6 +501,12 @@ ptx_status ptx_mlp_forward(ptx_mlp* m, const float* x, int32_t batch, float* y) 
     return guarded([&]() { return done(m->m.forward(x, batch, y)); });
 }
 
+ptx_status ptx_mlp_forward_noalloc(const int32_t* dims, int32_t n_dims, const float* params, int64_t param_count,
+                                   const float* x, int32_t batch, float* y) {
+    if (!dims || !params || !x || !y) return fail(PTX_ERR_INVALID, "null argument");
+    return guarded([&]() { return done(ptx::mlp_forward_noalloc(dims, n_dims, params, param_count, x, batch, y)); });
+}
+
 ptx_status ptx_mlp_loss_grad(ptx_mlp* m, const float* x, const float* target, const uint8_t* mode, int32_t batch,
                              double* loss) {
     if (!m || !x || !target || !mode) return fail(PTX_ERR_INVALID, "null argument");

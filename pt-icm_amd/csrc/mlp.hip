@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 
@@ -497,6 +498,140 @@ Status selftest_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, co
     g.C = c.p; g.ldc = N;
     PTX_TRY(launch_gemm(g, 0));
     return c.download(C, (size_t)M * N, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Forward pass with no allocation (ptx_mlp_forward_noalloc): the same kernels
+// and rounding points as Mlp::forward, on a file-scope device array instead
+// of buffers the library allocates.  The host addresses the array only
+// through the device pointer hipGetSymbolAddress returns (kept per device);
+// its name in host code is a host-side stand-in, never a device address.
+// Layout of one call, each part 256-byte aligned: the flat FP32 parameters
+// as given; per layer the padded BF16 weights and padded FP32 biases; the
+// FP32 input rows; per layer the BF16 input X_l and the FP32 pre-activation
+// Z_l of the padded batch; the softplus outputs.  Every byte a kernel reads
+// is written earlier in the same call (the pack kernels write the padding
+// zeros), so nothing depends on what the previous call left there.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu[kMaxScratchDevices];  // one call at a time per device: they share the array
+uint8_t* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// dst[i][j] (prow x pcol, BF16) = src[i][j] (rows x cols, FP32) inside, 0 in the padding
+__global__ void k_pack_bf16(const float* __restrict__ src, int32_t rows, int32_t cols, int32_t prow, int32_t pcol,
+                            uint16_t* __restrict__ dst) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)prow * pcol) return;
+    const int32_t i = (int32_t)(idx / pcol), j = (int32_t)(idx % pcol);
+    dst[idx] = (i < rows && j < cols) ? dev_bf16(src[(int64_t)i * cols + j]) : (uint16_t)0;
+}
+
+// dst[j] (pn) = src[j] (n) inside, 0 in the padding
+__global__ void k_pack_f32(const float* __restrict__ src, int32_t n, int32_t pn, float* __restrict__ dst) {
+    const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < pn) dst[j] = j < n ? src[j] : 0.0f;
+}
+
+size_t take(size_t& off, size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+}
+
+}  // namespace
+
+Status mlp_forward_noalloc(const int32_t* dims, int32_t ndims, const float* params, int64_t count, const float* x,
+                           int32_t batch, float* yout) {
+    if (ndims < 2 || ndims > 16) return Status::err(PTX_ERR_INVALID, "an MLP needs 2..16 layer widths");
+    if (batch <= 0) return Status::err(PTX_ERR_INVALID, "batch must be positive");
+    const int nl = ndims - 1;
+    int32_t pd[16];
+    int64_t np = 0;
+    for (int i = 0; i < ndims; ++i) {
+        if (dims[i] <= 0 || dims[i] > 8192) return Status::err(PTX_ERR_INVALID, "layer widths must be in 1..8192");
+        pd[i] = pad32(dims[i]);
+        if (i > 0) np += (int64_t)dims[i - 1] * dims[i] + dims[i];
+    }
+    if (count != np) return Status::err(PTX_ERR_INVALID, "parameter count mismatch");
+    if (batch > 65536) return Status::err(PTX_ERR_INVALID, "batch must be in 1..65536");
+    const int32_t bpad = pad32(batch);
+    // layout (offsets into the array)
+    size_t off = 0;
+    const size_t o_par = take(off, (size_t)np * 4);
+    size_t o_w[16], o_b[16], o_x[16], o_z[16];
+    for (int l = 0; l < nl; ++l) {
+        o_w[l] = take(off, (size_t)pd[l] * pd[l + 1] * 2);
+        o_b[l] = take(off, (size_t)pd[l + 1] * 4);
+    }
+    const size_t o_in = take(off, (size_t)batch * dims[0] * 4);
+    for (int l = 0; l < nl; ++l) {
+        o_x[l] = take(off, (size_t)bpad * pd[l] * 2);
+        o_z[l] = take(off, (size_t)bpad * pd[l + 1] * 4);
+    }
+    const size_t o_y = take(off, (size_t)bpad * pd[nl] * 4);
+    if (off > kNoAllocScratchBytes)
+        return Status::err(PTX_ERR_UNSUPPORTED, "network and batch need " + std::to_string(off) +
+                                                    " bytes of device scratch; the fixed array holds " +
+                                                    std::to_string(kNoAllocScratchBytes));
+    int dev = 0;
+    PTX_TRY(hip_check(hipGetDevice(&dev), "hipGetDevice"));
+    if (dev < 0 || dev >= kMaxScratchDevices) return Status::err(PTX_ERR_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
+    if (!g_scratch_ptr[dev]) {  // once per device, with that device current
+        void* p = nullptr;
+        PTX_TRY(hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"));
+        g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+    }
+    uint8_t* const base = g_scratch_ptr[dev];
+    const hipStream_t s = 0;
+    PTX_TRY(hip_check(hipMemcpy(base + o_par, params, (size_t)np * 4, hipMemcpyHostToDevice), "copy parameters"));
+    PTX_TRY(hip_check(hipMemcpy(base + o_in, x, (size_t)batch * dims[0] * 4, hipMemcpyHostToDevice), "copy input"));
+    const float* P = reinterpret_cast<const float*>(base + o_par);
+    int64_t po = 0;
+    for (int l = 0; l < nl; ++l) {
+        const int64_t tw = (int64_t)pd[l] * pd[l + 1];
+        hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((tw + 255) / 256)), dim3(256), 0, s, P + po, dims[l],
+                           dims[l + 1], pd[l], pd[l + 1], reinterpret_cast<uint16_t*>(base + o_w[l]));
+        po += (int64_t)dims[l] * dims[l + 1];
+        hipLaunchKernelGGL(k_pack_f32, dim3((unsigned)((pd[l + 1] + 255) / 256)), dim3(256), 0, s, P + po,
+                           dims[l + 1], pd[l + 1], reinterpret_cast<float*>(base + o_b[l]));
+        po += dims[l + 1];
+    }
+    const int64_t tx = (int64_t)bpad * pd[0];
+    hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((tx + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float*>(base + o_in), batch, dims[0], bpad, pd[0],
+                       reinterpret_cast<uint16_t*>(base + o_x[0]));
+    PTX_TRY(hip_check(hipGetLastError(), "pack kernels"));
+    for (int l = 0; l < nl; ++l) {  // the layers of Mlp::run_forward
+        Gemm g{};
+        g.A = reinterpret_cast<const uint16_t*>(base + o_x[l]); g.sai = pd[l]; g.sak = 1;
+        g.B = reinterpret_cast<const uint16_t*>(base + o_w[l]); g.sbk = pd[l + 1]; g.sbj = 1;
+        g.M = bpad; g.N = pd[l + 1]; g.K = pd[l];
+        g.bias = reinterpret_cast<const float*>(base + o_b[l]);
+        g.C = reinterpret_cast<float*>(base + o_z[l]); g.ldc = pd[l + 1];
+        if (l + 1 < nl) {
+            g.epi = kEpiBiasRelu;
+            g.Cb = reinterpret_cast<uint16_t*>(base + o_x[l + 1]); g.ldcb = pd[l + 1];
+        } else {
+            g.epi = kEpiBias;
+        }
+        PTX_TRY(launch_gemm(g, s));
+    }
+    const int32_t n = pd[nl];
+    const int64_t tot = (int64_t)bpad * n;
+    hipLaunchKernelGGL(k_head, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float*>(base + o_z[nl - 1]), nullptr, nullptr, batch, bpad, n, 1.0f,
+                       reinterpret_cast<float*>(base + o_y), nullptr, nullptr, nullptr);
+    PTX_TRY(hip_check(hipGetLastError(), "k_head"));
+    // the rows of the batch, without the padding columns (a synchronous copy: the call returns with y written)
+    return hip_check(hipMemcpy2D(yout, (size_t)dims[nl] * 4, base + o_y, (size_t)n * 4, (size_t)dims[nl] * 4,
+                                 (size_t)batch, hipMemcpyDeviceToHost),
+                     "copy output");
 }
 
 }  // namespace ptx

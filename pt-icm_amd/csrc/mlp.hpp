@@ -67,6 +67,11 @@ struct Mlp {
     Status run_backward(int32_t bpad, int32_t batch);
 };
 
+// Forward pass of the network given by dims and flat parameters, on a fixed
+// device array (no allocation); y[batch][dims[ndims-1]].
+Status mlp_forward_noalloc(const int32_t* dims, int32_t ndims, const float* params, int64_t count, const float* x,
+                           int32_t batch, float* y);
+
 // GEMM self-test: C[M][N] = A[M][K] . B[K][N], BF16 operands, FP32 result.
 Status selftest_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, const uint16_t* B, float* C);
 

@@ -131,6 +131,8 @@ _SIGS = {
     "ptx_mlp_get_params": (C.c_int32, [_vp, _P(C.c_float), C.c_int64]),
     "ptx_mlp_get_grads": (C.c_int32, [_vp, _P(C.c_float), C.c_int64]),
     "ptx_mlp_forward": (C.c_int32, [_vp, _P(C.c_float), C.c_int32, _P(C.c_float)]),
+    "ptx_mlp_forward_noalloc": (C.c_int32, [_P(C.c_int32), C.c_int32, _P(C.c_float), C.c_int64, _P(C.c_float),
+                                            C.c_int32, _P(C.c_float)]),
     "ptx_mlp_loss_grad": (C.c_int32, [_vp, _P(C.c_float), _P(C.c_float), _P(C.c_uint8), C.c_int32,
                                       _P(C.c_double)]),
     "ptx_mlp_adam": (C.c_int32, [_vp, C.c_float, C.c_float, C.c_float, C.c_float]),
@@ -571,6 +573,18 @@ class Mlp:
         check(lib().ptx_mlp_train(self.h, _ptr(x, C.c_float), _ptr(t, C.c_float), _ptr(m, C.c_uint8), x.shape[0],
                                   int(batch), int(epochs), float(lr), int(seed), _ptr(losses, C.c_double)))
         return losses
+
+
+def mlp_forward_noalloc(dims, params, x):
+    """The network's forward pass on the library's fixed device array (no
+    allocation); same result as Mlp.forward with the same parameters."""
+    d = np.ascontiguousarray(dims, np.int32)
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros((x.shape[0], int(d[-1])), np.float32)
+    check(lib().ptx_mlp_forward_noalloc(_ptr(d, C.c_int32), len(d), _ptr(p, C.c_float), len(p), _ptr(x, C.c_float),
+                                        x.shape[0], _ptr(y, C.c_float)))
+    return y
 
 
 def gemm_bf16(a_bits, b_bits):
