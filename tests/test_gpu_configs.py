@@ -5,8 +5,8 @@
     words) and Erdos-Renyi +-1 graphs G(5000, 0.004) (adjacency read through
     L2, BFS cluster moves), 64 slots x 2 chains, ICM on the cold slots.  Split
     kernels with four waves per cluster move (the autotuner's choice at
-    scale) and a split engine that switches to the cooperative kernel for a
-    batch this small: bit-exact with the oracle.
+    scale), and a split engine that switches to the cooperative kernel once
+    retirement leaves one instance active: bit-exact with the oracle.
   * The 4096-instance 3D EA batch (L in {6, 8, 10, 12}, 32 betas x 2 chains,
     ICM) in split mode with XCD-ordered work lists: tracked energies equal
     recomputed ones, permutations stay within their chains, every pair
@@ -71,8 +71,20 @@ def test_mixed_batch_at_scale_shapes_match_oracle(ptx, oracle, shape):
         for i, oe in enumerate(oes):
             compare(eng, oe, inst=i, what=f"{shape} inst {i} after +{step}")
         say("ok after", step, "more steps; mode", eng.info().mode)
-    if shape == "split_small_batch_coop" and eng.info().coop_capacity >= eng.info().total_replicas:
-        assert eng.info().mode == ptx.MODE_COOP
+    if shape == "split_small_batch_coop":
+        # the stragglers of a batch: one instance active at a time fits one
+        # cooperative launch, and the engine switches to that kernel
+        assert eng.info().coop_capacity >= 128
+        for keep in (0, 1):
+            active = np.zeros(len(pairs), bool)
+            active[keep] = True
+            eng.set_active(active)
+            eng.step(3)
+            oes[keep].step(3)
+            assert eng.info().mode == ptx.MODE_COOP
+            for i, oe in enumerate(oes):
+                compare(eng, oe, inst=i, what=f"only instance {keep} active, inst {i}")
+            say("cooperative steps ok for instance", keep)
     eng.close()
 
 
