@@ -11,7 +11,8 @@ the timed steps (inputs resident in HBM); roofline = the sweep kernel's
 algorithmic LDS bytes (4 d_i + 6 per attempted update) per launch over its
 average launch time, measured with HIP events on the engine stream;
 cpu_baseline = the oracle in reference order (index-order sweep, maintained
-dE, draws only for uphill moves) on one host core over a bounded sample.
+dE, draws only for uphill moves), one run per host core at once, over a
+bounded sample (about --cpu-seconds of work per core).
 """
 import argparse
 import importlib.util
@@ -69,22 +70,41 @@ def preprocess_ladder(ptx, problem, nt, T0, T1, seed, burn_in=5000, samples=2000
     return ptx.ladder_from_histograms(counts, e_base, hb, bs, be, mode=ptx.LADDER_FAST)
 
 
-def cpu_baseline(problem_path, betas, seconds, seed=0):
+def cpu_baseline(problem_path, betas, seconds, seed=0, cores=None):
+    """The oracle in reference order (index-order sweep, maintained dE, draws
+    only for uphill proposals) on the host's cores: one independent run (own
+    seed) per core, each sized to about `seconds` of work; value = the summed
+    attempted spin updates per second.  The oracle's C code runs outside the
+    GIL and keeps no global state, so one thread per core is one run per core."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as O
+    from concurrent.futures import ThreadPoolExecutor
+    if cores is None:
+        cores = min(len(os.sched_getaffinity(0)), 16)
     g = O.Graph.read(problem_path)
-    eng = O.Engine(g, betas, chains=2, seed=seed, flags=O.FLAG_ICM | O.FLAG_REFERENCE_ORDER)
+    probe = O.Engine(g, betas, chains=2, seed=seed, flags=O.FLAG_ICM | O.FLAG_REFERENCE_ORDER)
     t0 = time.perf_counter()
-    eng.step(2)
-    dt = time.perf_counter() - t0
-    steps = max(2, int(seconds / max(dt / 2, 1e-6)))
+    probe.step(2)
+    steps = max(2, int(seconds / max((time.perf_counter() - t0) / 2, 1e-6)))
+
+    def one(k):
+        eng = O.Engine(g, betas, chains=2, seed=seed + k, flags=O.FLAG_ICM | O.FLAG_REFERENCE_ORDER)
+        t = time.perf_counter()
+        eng.step(steps)
+        return time.perf_counter() - t
+
     t0 = time.perf_counter()
-    eng.step(steps)
-    dt = time.perf_counter() - t0
-    updates = 2 * len(betas) * g.n * steps
-    return {"value": updates / dt, "unit": "spin updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle reference-order PT-ICM on G022, same ladder ({len(betas)} slots x 2 chains), "
-                      f"{steps} PT steps in {dt:.1f} s on 1 host core"}
+    single = one(0)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        times = list(ex.map(one, range(1, cores + 1)))
+    wall = time.perf_counter() - t0
+    per_run = 2 * len(betas) * g.n * steps
+    return {"value": cores * per_run / wall, "unit": "spin updates/s", "cores": cores, "kind": "port",
+            "value_1core": per_run / single,
+            "sample": f"oracle reference-order PT-ICM on G022, same ladder ({len(betas)} slots x 2 chains): "
+                      f"{cores} independent runs of {steps} PT steps, one per host core, in {wall:.1f} s "
+                      f"(slowest run {max(times):.1f} s); one run alone {single:.1f} s"}
 
 
 def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
@@ -203,7 +223,8 @@ def main():
     ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: seconds of oracle work per host core (all cores run at once)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0,
                     help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
