@@ -125,6 +125,7 @@ ptx_status ptx_ladder_predict_rates(const uint32_t* counts, int32_t nt, int64_t 
 #define PTX_FLAG_PILOT (1u << 2)       /* pilot run: one chain, all pairs per step */
 #define PTX_FLAG_NO_EXCHANGE (1u << 3) /* sweeps only (testing) */
 #define PTX_FLAG_HISTOGRAM (1u << 5)   /* accumulate per-slot energy histograms */
+#define PTX_FLAG_OBSERVE (1u << 6)     /* accumulate per-slot observables (--log); split mode only */
 
 #define PTX_MODE_AUTO 0
 #define PTX_MODE_SPLIT 1 /* sweep / exchange / ICM kernels per step */
@@ -241,10 +242,41 @@ ptx_status ptx_engine_read_best(ptx_engine* e, int32_t inst, int64_t* best_e, in
                                 uint8_t* best_state);
 /* Histogram counts [nrep_slots][hist_bins], energy = e_base + 2*bin. */
 ptx_status ptx_engine_read_histograms(ptx_engine* e, int32_t inst, uint32_t* counts);
+/* Per-slot observables of --log (the writer of src/main.cc:490-530, whose
+ * samples were logged by Replica::sweep, src/replica.cc:10-12), accumulated
+ * by an engine created with PTX_FLAG_OBSERVE after every PT step from step
+ * hist_start on, on the state at the end of the step.  up = number of spins
+ * set (the reference's magnetisation m is up / n: 0.5 means no net moment);
+ * sums are exact (integers, the squares in FP64 below 2^53).  From them:
+ * u = mean_e / n, C = beta^2 / n (mean_e2 - mean_e^2), chi = beta n var(up / n),
+ * m = mean_up / n. */
+typedef struct {
+    int64_t samples;
+    int64_t sum_e;       /* sum of E */
+    double sum_e2;       /* sum of E^2 */
+    int64_t sum_up;      /* sum of the number of spins set */
+    double sum_up2;      /* sum of its square */
+    int64_t sum_abs_m;   /* sum of |2 up - n| (absolute magnetisation times n) */
+} ptx_slot_observables;
+ptx_status ptx_engine_read_observables(ptx_engine* e, int32_t inst, ptx_slot_observables* out); /* [nrep] */
 /* Lowest best energy of every instance over its replicas, out[n_instances]
  * (INT64_MAX where nothing is recorded yet), in one call: the poll of
  * time-to-target loops.  Best energies as in ptx_engine_read_best. */
 ptx_status ptx_engine_read_min_energies(ptx_engine* e, int64_t* out_per_instance);
+
+/* Checkpoint and resume (no counterpart in the reference, whose runs cannot
+ * be stopped and resumed).  The state is every array a PT step changes --
+ * spins, energies, best energies, sweeps and states, the temperature
+ * permutation, flip and swap counters, histograms, observables -- plus each
+ * instance's step counter and active flag, in a caller-owned buffer of
+ * ptx_engine_state_size bytes (host memory; write it to a file as is).  It
+ * loads into an engine created with the same problems, ladders, flags,
+ * sweeps, hist_start and seed (checked: PTX_ERR_INVALID otherwise), in any
+ * execution mode, and the loaded engine continues bit for bit as the saved
+ * one would have: the random streams are indexed by (seed, instance, step). */
+ptx_status ptx_engine_state_size(const ptx_engine* e, int64_t* bytes);
+ptx_status ptx_engine_save_state(ptx_engine* e, void* buf, int64_t bytes);
+ptx_status ptx_engine_load_state(ptx_engine* e, const void* buf, int64_t bytes);
 
 /* Device time of the kernels enqueued since the last reset, measured with
  * HIP events on the engine stream: total ms and launch count per kernel. */

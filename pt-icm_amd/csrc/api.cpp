@@ -287,7 +287,9 @@ ptx_status ptx_engine_autotune(const ptx_problem* const* problems, int32_t n_ins
                 for (int spl : {1, 4}) shapes.push_back({PTX_MODE_COOP, w, spl, PTX_CSR_AUTO, 0});
             shapes.push_back({PTX_MODE_COOP, 16, 1, PTX_CSR_AUTO, 0});  // taken as 8 with cluster moves
         }
-        for (int w : {4, 8, 16})
+        // fused: 4 or 8 waves (the 16-wave instantiation spills registers to
+        // scratch and writes many times its state per launch: not offered)
+        for (int w : {4, 8})
             for (int csr : {PTX_CSR_LDS, PTX_CSR_GLOBAL}) shapes.push_back({PTX_MODE_FUSED, w, 0, csr, 0});
         // split: one wave per cluster (the default, 0) or four
         const bool icm = (b0.flags & PTX_FLAG_ICM) != 0;
@@ -414,6 +416,23 @@ ptx_status ptx_engine_read_best(ptx_engine* e, int32_t inst, int64_t* be, int64_
 ptx_status ptx_engine_read_histograms(ptx_engine* e, int32_t inst, uint32_t* out) {
     if (!out) return fail(PTX_ERR_INVALID, "null buffer");
     PTX_ENGINE_CALL(e->e.read_histograms(inst, out));
+}
+ptx_status ptx_engine_read_observables(ptx_engine* e, int32_t inst, ptx_slot_observables* out) {
+    if (!out) return fail(PTX_ERR_INVALID, "null buffer");
+    PTX_ENGINE_CALL(e->e.read_observables(inst, out));
+}
+ptx_status ptx_engine_state_size(const ptx_engine* e, int64_t* bytes) {
+    if (!e || !bytes) return fail(PTX_ERR_INVALID, "null argument");
+    *bytes = e->e.state_bytes();
+    return PTX_OK;
+}
+ptx_status ptx_engine_save_state(ptx_engine* e, void* buf, int64_t bytes) {
+    if (!buf) return fail(PTX_ERR_INVALID, "null buffer");
+    PTX_ENGINE_CALL(e->e.save_state(buf, bytes));
+}
+ptx_status ptx_engine_load_state(ptx_engine* e, const void* buf, int64_t bytes) {
+    if (!buf) return fail(PTX_ERR_INVALID, "null buffer");
+    PTX_ENGINE_CALL(e->e.load_state(buf, bytes));
 }
 ptx_status ptx_engine_read_min_energies(ptx_engine* e, int64_t* out) {
     if (!out) return fail(PTX_ERR_INVALID, "null buffer");

@@ -326,7 +326,12 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // ---- execution mode ----
     spin_words = spin_off;
     mode = PTX_MODE_SPLIT;
-    if (cfg.mode == PTX_MODE_COOP) {
+    const bool observe = (cfg.flags & PTX_FLAG_OBSERVE) != 0;
+    if (observe && (cfg.mode == PTX_MODE_COOP || cfg.mode == PTX_MODE_FUSED))
+        return Status::err(PTX_ERR_UNSUPPORTED, "observables (PTX_FLAG_OBSERVE) are taken by the split kernels only");
+    if (observe) {
+        // split kernels: the state is in device memory after every step
+    } else if (cfg.mode == PTX_MODE_COOP) {
         spl = cfg.spins_per_lane == 4 ? 4 : 1;
         // 16 waves only for one position per lane and no cluster moves
         coop_W = cfg.waves_per_group == 16 && spl == 1 && !coop_icm ? 16
@@ -392,7 +397,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
-    if (mode == PTX_MODE_SPLIT && cfg.small_batch_coop && (cfg.mode == PTX_MODE_AUTO || cfg.mode == PTX_MODE_SPLIT)) {
+    if (mode == PTX_MODE_SPLIT && cfg.small_batch_coop && !observe &&
+        (cfg.mode == PTX_MODE_AUTO || cfg.mode == PTX_MODE_SPLIT)) {
         coop_W = 4;
         spl = 1;
         coop_K = 256;
@@ -428,6 +434,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_succ.alloc((size_t)total_rep));
     PTX_TRY(d_swaps.alloc((size_t)ninst));
     PTX_TRY(d_hist.alloc((size_t)std::max<int64_t>(hist_off, 1)));
+    PTX_TRY(d_obs_i.alloc(observe ? (size_t)total_rep * 4 : 1));
+    PTX_TRY(d_obs_d.alloc(observe ? (size_t)total_rep * 2 : 1));
     PTX_TRY(hip_check(hipStreamSynchronize(stream), "upload sync"));
     PTX_TRY(launch_init(0));
     return sync();
@@ -463,6 +471,8 @@ static DevArgs build_args(Engine& e) {
     a.jneg = e.d_jneg.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.fault = e.d_fault;
+    a.obs_i = reinterpret_cast<long long*>(e.d_obs_i.p);
+    a.obs_d = e.d_obs_d.p;
     a.inst_step = e.d_inst_step.p;
     a.seed = e.seed;
     a.tag_base = (e.cfg.flags & PTX_FLAG_PILOT) ? kTagPilot : 0u;
@@ -569,6 +579,10 @@ Status Engine::launch_step() {
         }));
     }
     const int na = (int)inst_list.size();
+    if (cfg.flags & PTX_FLAG_OBSERVE) {
+        hipLaunchKernelGGL(k_observe, dim3((unsigned)na), dim3(kWave), 0, stream, a);
+        PTX_TRY(hip_check(hipGetLastError(), "k_observe"));
+    }
     hipLaunchKernelGGL(k_step_advance, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, stream, d_order.p, na,
                        d_inst_step.p);
     return hip_check(hipGetLastError(), "k_step_advance");
@@ -1055,6 +1069,159 @@ Status Engine::read_histograms(int32_t inst, uint32_t* out) {
     const InstDesc& d = desc[inst];
     if (!(cfg.flags & PTX_FLAG_HISTOGRAM)) return Status::err(PTX_ERR_INVALID, "engine built without histograms");
     return d_hist.download(out, (size_t)d.nrep * d.hist_bins, (size_t)d.hist_off, stream);
+}
+
+Status Engine::read_observables(int32_t inst, ptx_slot_observables* out) {
+    PTX_TRY(check_inst(*this, inst));
+    PTX_TRY(sync());
+    if (!(cfg.flags & PTX_FLAG_OBSERVE)) return Status::err(PTX_ERR_INVALID, "engine built without observables");
+    const InstDesc& d = desc[inst];
+    std::vector<unsigned long long> vi((size_t)d.nrep * 4);
+    std::vector<double> vd((size_t)d.nrep * 2);
+    PTX_TRY(d_obs_i.download(vi.data(), vi.size(), (size_t)d.rep_off * 4, stream));
+    PTX_TRY(d_obs_d.download(vd.data(), vd.size(), (size_t)d.rep_off * 2, stream));
+    for (int32_t s = 0; s < d.nrep; ++s) {
+        out[s].samples = (int64_t)vi[4 * s];
+        out[s].sum_e = (int64_t)vi[4 * s + 1];
+        out[s].sum_up = (int64_t)vi[4 * s + 2];
+        out[s].sum_abs_m = (int64_t)vi[4 * s + 3];
+        out[s].sum_e2 = vd[2 * s];
+        out[s].sum_up2 = vd[2 * s + 1];
+    }
+    return Status::ok();
+}
+
+// ---- checkpoint ----
+// Blob: a header that identifies the engine's shape (what load_state checks
+// before it writes anything), then every array a PT step changes, in a fixed
+// order.  The random streams are indexed by (seed, instance, step), so a
+// restored engine continues exactly as the saved one would have.
+namespace {
+constexpr uint32_t kStateMagic = 0x53585450u;  // "PTXS"
+constexpr uint32_t kStateVersion = 1;
+struct StateHeader {
+    uint32_t magic, version;
+    uint64_t shape;       // FNV-1a of the instance descriptors, ladders, flags, sweeps and seed
+    int64_t ninst, total_rep, spin_words, hist_len, observe;
+    int64_t steps_done;
+};
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+};
+}  // namespace
+
+static uint64_t shape_hash(const Engine& e) {
+    Fnv f;
+    for (const InstDesc& d : e.desc) {
+        const int32_t v[9] = {d.n, d.nw, d.nt, d.nrep, d.ncls, d.fmax, d.nc, d.hist_bins, d.e_base};
+        f.add(v, sizeof v);
+    }
+    for (const auto& l : e.ladders) f.add(l.data(), l.size() * sizeof(double));
+    const uint32_t fl = e.cfg.flags;
+    const int32_t sw = e.cfg.sweeps, ch = e.cfg.chains;
+    f.add(&fl, sizeof fl);
+    f.add(&sw, sizeof sw);
+    f.add(&ch, sizeof ch);
+    f.add(&e.seed, sizeof e.seed);
+    f.add(&e.cfg.hist_start, sizeof e.cfg.hist_start);
+    return f.h;
+}
+
+int64_t Engine::state_bytes() const {
+    const int64_t R = total_rep, I = ninst;
+    const int64_t hist_len = (int64_t)d_hist.n, obs = (cfg.flags & PTX_FLAG_OBSERVE) ? 1 : 0;
+    return (int64_t)sizeof(StateHeader) + I * 4 /*step*/ + I /*active*/ + 2 * spin_words * 4 + 5 * R * 4 +
+           R * 8 /*flips*/ + I * 8 /*swaps*/ + 2 * R * 4 /*att, succ*/ + hist_len * 4 + obs * R * (4 * 8 + 2 * 8);
+}
+
+Status Engine::save_state(void* buf, int64_t bytes) {
+    if (bytes < state_bytes()) return Status::err(PTX_ERR_INVALID, "state buffer too small");
+    PTX_TRY(sync());
+    uint8_t* o = static_cast<uint8_t*>(buf);
+    StateHeader h{kStateMagic, kStateVersion, shape_hash(*this), ninst, total_rep, spin_words, (int64_t)d_hist.n,
+                  (cfg.flags & PTX_FLAG_OBSERVE) ? 1 : 0, steps_done};
+    std::memcpy(o, &h, sizeof h);
+    o += sizeof h;
+    auto take = [&](auto& dev, size_t count) -> Status {
+        using T = std::remove_pointer_t<decltype(dev.p)>;
+        PTX_TRY(dev.download(reinterpret_cast<T*>(o), count, 0, stream));
+        o += count * sizeof(T);
+        return Status::ok();
+    };
+    const size_t R = (size_t)total_rep, I = (size_t)ninst, S = (size_t)spin_words;
+    PTX_TRY(take(d_inst_step, I));
+    std::memcpy(o, active.data(), I);
+    o += I;
+    PTX_TRY(take(d_spins, S));
+    PTX_TRY(take(d_best_spins, S));
+    PTX_TRY(take(d_E, R));
+    PTX_TRY(take(d_bestE, R));
+    PTX_TRY(take(d_best_sweep, R));
+    PTX_TRY(take(d_slot2rep, R));
+    PTX_TRY(take(d_rep2slot, R));
+    PTX_TRY(take(d_flips, R));
+    PTX_TRY(take(d_swaps, I));
+    PTX_TRY(take(d_att, R));
+    PTX_TRY(take(d_succ, R));
+    PTX_TRY(take(d_hist, d_hist.n));
+    if (h.observe) {
+        PTX_TRY(take(d_obs_i, 4 * R));
+        PTX_TRY(take(d_obs_d, 2 * R));
+    }
+    return Status::ok();
+}
+
+Status Engine::load_state(const void* buf, int64_t bytes) {
+    if (bytes < (int64_t)sizeof(StateHeader)) return Status::err(PTX_ERR_INVALID, "state buffer too small");
+    const uint8_t* o = static_cast<const uint8_t*>(buf);
+    StateHeader h;
+    std::memcpy(&h, o, sizeof h);
+    o += sizeof h;
+    if (h.magic != kStateMagic || h.version != kStateVersion)
+        return Status::err(PTX_ERR_INVALID, "not an engine state (magic or version)");
+    if (h.shape != shape_hash(*this) || h.ninst != ninst || h.total_rep != total_rep || h.spin_words != spin_words ||
+        h.hist_len != (int64_t)d_hist.n || h.observe != ((cfg.flags & PTX_FLAG_OBSERVE) ? 1 : 0))
+        return Status::err(PTX_ERR_INVALID, "the state was saved by an engine of another shape (instances, "
+                                            "ladders, flags, sweeps or seed differ)");
+    if (bytes < state_bytes()) return Status::err(PTX_ERR_INVALID, "state buffer too small");
+    PTX_TRY(sync());
+    auto put = [&](auto& dev, size_t count) -> Status {
+        using T = std::remove_pointer_t<decltype(dev.p)>;
+        PTX_TRY(dev.upload(reinterpret_cast<const T*>(o), count, 0, stream));
+        o += count * sizeof(T);
+        return Status::ok();
+    };
+    const size_t R = (size_t)total_rep, I = (size_t)ninst, S = (size_t)spin_words;
+    PTX_TRY(put(d_inst_step, I));
+    std::vector<uint8_t> act(o, o + I);
+    o += I;
+    PTX_TRY(put(d_spins, S));
+    PTX_TRY(put(d_best_spins, S));
+    PTX_TRY(put(d_E, R));
+    PTX_TRY(put(d_bestE, R));
+    PTX_TRY(put(d_best_sweep, R));
+    PTX_TRY(put(d_slot2rep, R));
+    PTX_TRY(put(d_rep2slot, R));
+    PTX_TRY(put(d_flips, R));
+    PTX_TRY(put(d_swaps, I));
+    PTX_TRY(put(d_att, R));
+    PTX_TRY(put(d_succ, R));
+    PTX_TRY(put(d_hist, d_hist.n));
+    if (h.observe) {
+        PTX_TRY(put(d_obs_i, 4 * R));
+        PTX_TRY(put(d_obs_d, 2 * R));
+    }
+    // the cooperative step's tagged energy slots of earlier launches could
+    // match a step of the restored run: clear them
+    if (d_pub_E.p) PTX_TRY(hip_check(hipMemsetAsync(d_pub_E.p, 0, d_pub_E.n * sizeof(unsigned long long), stream),
+                                     "clear published energies"));
+    steps_done = h.steps_done;
+    PTX_TRY(hip_check(hipStreamSynchronize(stream), "load state"));
+    return set_active(act.data());  // rebuilds the work lists (and drops a captured graph)
 }
 
 Status Engine::read_min_energies(int64_t* out) {

@@ -90,6 +90,8 @@ struct Engine {
     DevBuf<int32_t> d_E, d_bestE, d_best_sweep, d_slot2rep, d_rep2slot;
     DevBuf<unsigned long long> d_flips, d_swaps;
     DevBuf<uint32_t> d_att, d_succ, d_hist;
+    DevBuf<unsigned long long> d_obs_i;  // per slot: samples, sum E, sum up, sum |2 up - n| (PTX_FLAG_OBSERVE)
+    DevBuf<double> d_obs_d;              // per slot: sum E^2, sum up^2
     DevBuf<int2> d_items, d_pairs;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
@@ -143,6 +145,11 @@ struct Engine {
     Status read_best(int32_t inst, int64_t* be, int64_t* bs, uint8_t* st);
     Status read_histograms(int32_t inst, uint32_t* out);
     Status read_min_energies(int64_t* out);
+    Status read_observables(int32_t inst, ptx_slot_observables* out);
+    // checkpoint of the whole mutable state (see ptx_engine_save_state)
+    int64_t state_bytes() const;
+    Status save_state(void* buf, int64_t bytes);
+    Status load_state(const void* buf, int64_t bytes);
     Status timing_get(ptx_engine_timing* out);
     Status phase_profile(int on, double* us);
 

@@ -431,6 +431,33 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
 
 // Ends a split-mode PT step: every active instance's step index advances
 // (a kernel rather than a host copy, so that graph replays advance it too).
+// Per-slot observables at the end of a PT step, from step hist_start on
+// (--log; the legacy writer of src/main.cc:490-530).  One wave per active
+// instance; lane r takes replica r and adds to the slot it holds (the
+// slots of an instance's replicas are distinct, so no two lanes share one).
+__global__ __launch_bounds__(kWave) void k_observe(DevArgs a) {
+    const int inst = a.order[blockIdx.x];
+    if ((int64_t)a.inst_step[inst] < a.hist_start) return;
+    const InstDesc d = a.desc[inst];
+    const uint32_t last = (d.n & 31) ? (1u << (d.n & 31)) - 1u : 0xFFFFFFFFu;
+    for (int r = threadIdx.x; r < d.nrep; r += kWave) {
+        const uint32_t* w = a.spins + d.spin_off + (int64_t)r * d.nw;
+        int up = 0;
+        for (int k = 0; k + 1 < d.nw; ++k) up += __popc(w[k]);
+        up += __popc(w[d.nw - 1] & last);
+        const int64_t gs = d.rep_off + a.rep2slot[d.rep_off + r];
+        const long long E = a.E[d.rep_off + r];
+        long long* oi = a.obs_i + 4 * gs;
+        double* od = a.obs_d + 2 * gs;
+        oi[0] += 1;
+        oi[1] += E;
+        oi[2] += up;
+        oi[3] += up * 2 > d.n ? up * 2 - d.n : d.n - up * 2;
+        od[0] += (double)E * (double)E;
+        od[1] += (double)up * (double)up;
+    }
+}
+
 __global__ void k_step_advance(const int32_t* order, int n, uint32_t* inst_step) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) inst_step[order[t]] += 1u;

@@ -58,6 +58,8 @@ struct DevArgs {
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
     uint32_t* fault;         // host-mapped fault words: [0] a cooperative wait gave up, [1] histogram
                              // samples outside their table
+    long long* obs_i;        // per slot: samples, sum E, sum up, sum |2 up - n| (k_observe)
+    double* obs_d;           // per slot: sum E^2, sum up^2
     uint64_t seed;
     uint32_t tag_base;       // 0 or kTagPilot
     int32_t sweeps;

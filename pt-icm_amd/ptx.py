@@ -18,6 +18,7 @@ FLAG_ICM = 1 << 0
 FLAG_PILOT = 1 << 2
 FLAG_NO_EXCHANGE = 1 << 3
 FLAG_HISTOGRAM = 1 << 5
+FLAG_OBSERVE = 1 << 6
 MODE_AUTO, MODE_SPLIT, MODE_FUSED, MODE_COOP = 0, 1, 2, 3
 LADDER_EXACT, LADDER_FAST, LADDER_NEWTON = 0, 1, 2
 CSR_AUTO, CSR_LDS, CSR_GLOBAL = 0, 1, 2
@@ -54,6 +55,11 @@ class EngineInfo(C.Structure):
 class InstanceInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("nt", C.c_int32), ("nrep", C.c_int32), ("nc", C.c_int32),
                 ("n_colours", C.c_int32), ("hist_bins", C.c_int32), ("e_base", C.c_int64)]
+
+
+class SlotObservables(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("sum_e", C.c_int64), ("sum_e2", C.c_double), ("sum_up", C.c_int64),
+                ("sum_up2", C.c_double), ("sum_abs_m", C.c_int64)]
 
 
 class TuneTrial(C.Structure):
@@ -116,6 +122,10 @@ _SIGS = {
     "ptx_engine_read_best": (C.c_int32, [_vp, C.c_int32, _P(C.c_int64), _P(C.c_int64), _P(C.c_uint8)]),
     "ptx_engine_read_histograms": (C.c_int32, [_vp, C.c_int32, _P(C.c_uint32)]),
     "ptx_engine_read_min_energies": (C.c_int32, [_vp, _P(C.c_int64)]),
+    "ptx_engine_read_observables": (C.c_int32, [_vp, C.c_int32, _P(SlotObservables)]),
+    "ptx_engine_state_size": (C.c_int32, [_vp, _P(C.c_int64)]),
+    "ptx_engine_save_state": (C.c_int32, [_vp, _vp, C.c_int64]),
+    "ptx_engine_load_state": (C.c_int32, [_vp, _vp, C.c_int64]),
     "ptx_engine_timing_enable": (C.c_int32, [_vp, C.c_int32]),
     "ptx_engine_timing_get": (C.c_int32, [_vp, _P(Timing)]),
     "ptx_engine_phase_profile": (C.c_int32, [_vp, C.c_int32, _P(C.c_double)]),
@@ -419,6 +429,26 @@ class Engine:
         out = np.zeros((ii.nrep, ii.hist_bins), np.uint32)
         check(lib().ptx_engine_read_histograms(self.h, inst, _ptr(out, C.c_uint32)))
         return out, ii.e_base
+
+    def observables(self, inst=0):
+        """Per-slot sums of --log: dict of arrays samples, sum_e, sum_e2, sum_up, sum_up2, sum_abs_m."""
+        ii = self.instance_info(inst)
+        arr = (SlotObservables * ii.nrep)()
+        check(lib().ptx_engine_read_observables(self.h, inst, arr))
+        return {f: np.array([getattr(x, f) for x in arr]) for f, _ in SlotObservables._fields_}
+
+    def save_state(self):
+        """The engine's whole mutable state as bytes (checkpoint)."""
+        n = C.c_int64()
+        check(lib().ptx_engine_state_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().ptx_engine_save_state(self.h, buf, n.value))
+        return buf.raw
+
+    def load_state(self, blob):
+        """Restores a checkpoint taken by an engine of the same shape."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        check(lib().ptx_engine_load_state(self.h, buf, len(blob)))
 
     def min_energies(self):
         out = np.zeros(self.ninst, np.int64)
