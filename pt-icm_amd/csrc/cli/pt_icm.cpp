@@ -27,14 +27,25 @@ const char* kUsage =
     "  -T1, --temp1 X               ending temperature                     [2.8]\n"
     "  -i, --inverse                read T0/T1 as inverse temperatures\n"
     "      --preprocess             multihistogram constant-acceptance ladder\n"
-    "      --log                    log data (accepted, no effect, as the reference)\n"
+    "      --log                    per-temperature observables T,C,X,u,m after the burn-in\n"
+    "                               (written to results.csv, or --log-file)\n"
     "      --icm                    isoenergetic cluster moves\n"
     "      --verbose N              verbosity 0..3                          [0]\n"
     "      --ladder-mode exact|fast|newton  ladder search: reference sums | O(nbins) sums |\n"
-    "                             O(nbins) sums + Newton multihistogram solve    [exact]\n";
+    "                             O(nbins) sums + Newton multihistogram solve    [exact]\n"
+    "      --log-file PATH          file of --log                          [results.csv]\n"
+    "      --ladder-in PATH         inverse temperatures, one per line (instead of -r/-T0/-T1/--preprocess)\n"
+    "      --ladder-out PATH        write the inverse temperatures used\n"
+    "      --histograms-out PATH    --preprocess: write the pilot's energy histograms (slot,beta,energy,count)\n"
+    "      --checkpoint PATH        write the engine state after the run\n"
+    "      --resume PATH            continue a run from --checkpoint up to --pts steps in all\n"
+    "      --folder DIR             solve every problem file of DIR (batched), --restarts times,\n"
+    "      --csv PATH               one CSV row per problem and restart   [results_folder.csv]\n"
+    "      --restarts N             restarts of --folder (seed + i)        [1]\n";
 
 struct Args {
-    std::string problem;
+    std::string problem, folder, csv = "results_folder.csv";
+    uint64_t restarts = 1;
     pticm::Parameters params;
     bool help = false;
 };
@@ -105,9 +116,18 @@ bool parse(int argc, char** argv, Args& a, std::string& err) {
             else if (s == "fast") p.ladder_mode = PTX_LADDER_FAST;
             else if (s == "newton") p.ladder_mode = PTX_LADDER_NEWTON;
             else { err = "--ladder-mode: exact, fast or newton"; return false; }
-        } else { err = "Unknown argument: " + k; return false; }
+        } else if (k == "--log-file") { if (!value(p.log_path)) return false; }
+        else if (k == "--ladder-in") { if (!value(p.ladder_in)) return false; }
+        else if (k == "--ladder-out") { if (!value(p.ladder_out)) return false; }
+        else if (k == "--histograms-out") { if (!value(p.histograms_out)) return false; }
+        else if (k == "--checkpoint") { if (!value(p.checkpoint)) return false; }
+        else if (k == "--resume") { if (!value(p.resume)) return false; }
+        else if (k == "--folder") { if (!value(a.folder)) return false; }
+        else if (k == "--csv") { if (!value(a.csv)) return false; }
+        else if (k == "--restarts") { if (!num_u(a.restarts)) return false; }
+        else { err = "Unknown argument: " + k; return false; }
     }
-    if (!a.help && a.problem.empty()) { err = "-p: required."; return false; }
+    if (!a.help && a.problem.empty() && a.folder.empty()) { err = "-p: required."; return false; }
     if (!a.help && p.Nt < 1) { err = "-r: at least one temperature"; return false; }
     if (!a.help && p.preprocess && p.Nt < 2) { err = "--preprocess needs at least 2 temperatures"; return false; }
     return true;
@@ -146,6 +166,10 @@ int main(int argc, char** argv) {
     try {
         pticm::PTWorker w;
         w.params = a.params;
+        if (!a.folder.empty()) {
+            pticm::solve_folder(w, a.folder, a.csv, a.restarts);
+            return 0;
+        }
         double best_ene = 0;
         size_t flips = 0;
         pticm::StateVector best_state;

@@ -42,3 +42,83 @@ def test_cli_prints_a_consistent_best_state(ptx, args):
     assert float(keys["best_energy"]) == p.energy(state) == float(keys["best_state_energy"])
     assert float(keys["best_maxcut"]) == p.maxcut(state)
     assert int(keys["num_of_flips"]) > 0
+
+
+def _results(keys):
+    return {k: keys[k] for k in ("best_energy", "best_state", "num_of_flips", "num_of_swaps", "swap_attempts",
+                                 "swap_successes", "best_state_sweep")}
+
+
+def test_ladder_file_round_trip(tmp_path):
+    """--ladder-out writes the preprocess ladder; --ladder-in with the same
+    seed reproduces the run (the main run's streams do not depend on the pilot)."""
+    lad, hist = str(tmp_path / "ladder.txt"), str(tmp_path / "hist.csv")
+    common = ["-p", graph_path("lattice_2d_10.gset"), "-s", "5", "--pts", "300", "--icm", "-T0", "0.5", "-T1", "3.0"]
+    _, a = run(*common, "-r", "12", "--preprocess", "--preprocess-samples", "300", "--ladder-out", lad,
+               "--histograms-out", hist)
+    _, b = run(*common, "--ladder-in", lad)
+    assert _results(a) == _results(b)
+    betas = [float(x) for x in open(lad).read().split()]
+    assert len(betas) >= 2 and all(x > y for x, y in zip(betas, betas[1:]))
+    rows = [r.split(",") for r in open(hist).read().splitlines()[1:]]
+    per_slot = {}
+    for slot, beta, e, c in rows:
+        per_slot[int(slot)] = per_slot.get(int(slot), 0) + int(c)
+    assert len(per_slot) == 12 and set(per_slot.values()) == {300}
+
+
+def test_checkpoint_and_resume(tmp_path):
+    """A run stopped with --checkpoint and continued with --resume prints the
+    results of the run that never stopped."""
+    ck = str(tmp_path / "state.bin")
+    common = ["-p", graph_path("lattice_2d_10.gset"), "-s", "9", "-r", "16", "--icm", "-T0", "0.5", "-T1", "3.0"]
+    _, full = run(*common, "--pts", "300")
+    run(*common, "--pts", "120", "--checkpoint", ck)
+    _, resumed = run(*common, "--pts", "300", "--resume", ck)
+    assert _results(full) == _results(resumed)
+
+
+def test_log_writes_observables(ptx, tmp_path):
+    """--log on the 2D ferromagnet L = 20: one row T,C,X,u,m per temperature,
+    u near the exact finite-lattice value at the hottest one."""
+    f = str(tmp_path / "obs.csv")
+    run("-p", graph_path("lattice_2d_20.gset"), "-s", "1", "-r", "8", "-T0", "1.5", "-T1", "3.5", "--pts", "30000",
+        "-b", "2000", "--log", "--log-file", f)
+    rows = [list(map(float, r.split(","))) for r in open(f).read().splitlines()[1:]]
+    assert len(rows) == 8
+    betas = ptx.beta_linspace(1.5, 3.5, 8)
+    for (T, C, X, u, m), b in zip(rows, betas):
+        assert abs(T - 1.0 / b) < 1e-5 and C > 0 and X > 0 and -2.0 <= u <= 0.0 and 0.0 <= m <= 1.0
+    hot = min(rows, key=lambda r: -r[0])
+    assert abs(hot[0] - 3.5) < 1e-5 and abs(hot[3] - (-0.66012)) < 0.01
+
+
+def test_folder_mode_batches_every_problem(ptx, tmp_path):
+    """--folder DIR --restarts 2: one batched engine per restart over the
+    folder's problems, one CSV row per problem and restart (src/main.cc:149-208);
+    every row's state reproduces its energy and cut."""
+    import csv
+    import shutil
+    d = tmp_path / "problems"
+    d.mkdir()
+    for name in ("lattice_2d_5.gset", "lattice_2d_10.gset", "G039"):
+        shutil.copy(graph_path(name), d / name)
+    out = str(tmp_path / "runs.csv")
+    run("--folder", str(d), "--csv", out, "--restarts", "2", "-s", "4", "-r", "8", "--pts", "200", "--icm",
+        "-T0", "0.5", "-T1", "3.0")
+    rows = list(csv.DictReader(open(out)))
+    assert len(rows) == 6
+    assert [int(r["iter"]) for r in rows] == [0, 0, 0, 1, 1, 1]
+    for r in rows:
+        p = ptx.Problem.read(r["problem"])
+        state = np.array([int(x) for x in r["state"].strip("[]").split(",")], np.uint8)
+        assert len(state) == p.n and float(r["ene"]) == p.energy(state) and float(r["cut"]) == p.maxcut(state)
+        assert len(r["tlist"].strip("[]").split(",")) == 16 and int(r["flips"]) > 0
+
+
+def test_reference_style_caller_sweeps_on_the_gpu():
+    caller = os.path.join(REPO, "pt-icm_amd", "build", "facade_caller")
+    for name in ("lattice_2d_10.gset", "G039"):
+        out = subprocess.run([caller, "gpu", graph_path(name)], capture_output=True, text=True, timeout=120)
+        print(out.stdout, out.stderr, flush=True)
+        assert out.returncode == 0 and out.stdout.strip() == "ok"
