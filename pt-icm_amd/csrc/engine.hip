@@ -74,6 +74,65 @@ Status Engine::check_fault() const {
 
 namespace {
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Order of the records within each adjacency row (unit format), chosen for
+// the cooperative sweep's neighbour gathers.  In a colour class of csz
+// positions, T = 4, 2 or 1 lanes share a position (the first T with
+// T * csz <= nthr) and lane sub of them reads the row's chunks sub, sub + T,
+// ...; gather h of chunk ch is one LDS byte read of the whole wave, at the
+// neighbour's position.  Two lanes of a wave that read different words of
+// one LDS bank in the same gather serialise, and random neighbours make that
+// common (about 3 words per bank and gather on G022).  Rows are walked in
+// position order; each record takes, among its row's free slots, the one
+// whose gather has the fewest other words in the record's bank so far in
+// this wave.  The records stay in the row's first deg slots (code that walks
+// a row's neighbours reads those; the padding after them reads the zero
+// byte), and the local field is a sum over the row, so the bits do not change.
+struct BankOrder {
+    static constexpr int kBanks = 32;
+    static constexpr int kKeys = 8 * (kMaxDegree / 8 + 1);  // gather (chunk / T, h) of a wave
+    int nthr;
+    std::vector<std::vector<uint32_t>> use;  // key * kBanks + bank -> words read so far by the wave
+    std::vector<int> touched;
+    int64_t wave_id = -1;
+    explicit BankOrder(int threads) : nthr(threads), use((size_t)kKeys * kBanks) {}
+    void order(int64_t cls_id, int j, int csz, uint16_t* rec, int deg, uint32_t pos_mask) {
+        const int T = 4 * csz <= nthr ? 4 : (2 * csz <= nthr ? 2 : 1);
+        const int per = nthr / T;
+        const int64_t wid = (cls_id << 32) | ((int64_t)(j / per) << 8) | (int64_t)((j % per) * T / 64);
+        if (wid != wave_id) {
+            for (int t : touched) use[(size_t)t].clear();
+            touched.clear();
+            wave_id = wid;
+        }
+        const int S = deg;
+        std::vector<uint16_t> in(rec, rec + S), out(S);
+        std::vector<char> taken(S, 0);
+        for (uint16_t r : in) {
+            const uint32_t word = (r & pos_mask) >> 2, bank = word % kBanks;
+            int best = -1, best_cost = 1 << 30;
+            for (int sl = 0; sl < S; ++sl) {
+                if (taken[sl]) continue;
+                const int key = (sl / 8 / T) * 8 + sl % 8;
+                const auto& u = use[(size_t)key * kBanks + bank];
+                const int c = (u.empty() || std::find(u.begin(), u.end(), word) != u.end()) ? 0 : (int)u.size();
+                if (c < best_cost) {
+                    best_cost = c;
+                    best = sl;
+                }
+            }
+            taken[best] = 1;
+            out[best] = r;
+            const int key = (best / 8 / T) * 8 + best % 8;
+            auto& u = use[(size_t)key * kBanks + bank];
+            if (std::find(u.begin(), u.end(), word) == u.end()) {
+                if (u.empty()) touched.push_back(key * kBanks + (int)bank);
+                u.push_back(word);
+            }
+        }
+        std::copy(out.begin(), out.end(), rec);
+    }
+};
 }  // namespace
 
 Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -182,6 +241,13 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         // (the wide format masks padding by weight 0, so there any in-range position will do)
         const uint32_t sentinel = std::min(32u * (uint32_t)d.nw, 65535u);
         int64_t cur16 = 0;
+        // engines that can run the cooperative kernel with one position per
+        // lane order each row's records for its gathers (any order is exact)
+        const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
+                               (cfg.mode == PTX_MODE_COOP || cfg.small_batch_coop ||
+                                (cfg.mode == PTX_MODE_AUTO && ninst == 1));
+        BankOrder bank_order(kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4));
+        int32_t cls_of = 0;
         for (int32_t p = 0; p < P.n; ++p) {
             const int32_t o = P.pos2orig[p];
             const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
@@ -193,6 +259,12 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 for (int32_t j = 0; j < dg; ++j) {
                     const int32_t q = P.rowptr[o] + j;
                     rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << d.kbit : 0u));
+                }
+                if (coop_rows) {
+                    while (p >= P.class_start[cls_of + 1]) ++cls_of;
+                    const int32_t c0 = P.class_start[cls_of];
+                    bank_order.order(((int64_t)i << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
+                                     dg, (1u << d.kbit) - 1u);
                 }
                 for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
                 cur16 += chunks;
