@@ -201,6 +201,9 @@ class Surrogate:
         self.net.set_params(z["params"])
         self.max_batch = max_batch
 
+    def close(self):
+        self.net.close()
+
     def predict(self, X):
         out = []
         for i in range(0, len(X), self.max_batch):

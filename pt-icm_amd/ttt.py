@@ -106,6 +106,7 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate
     if surrogate:
         sur = S.Surrogate(ptx)
         lads, ok, _, stages = S.choose_ladders(ptx, sur, probs, stats, T0, T1, seed=seed)
+        sur.close()
     else:
         ft = {}
         lads, _, _ = S.multihistogram_ladders(ptx, probs, S.N_OUT, T0, T1, 5000, 2000, seed=seed + 2, timing=ft)
@@ -142,6 +143,8 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate
     per_step = t_run / steps
     hits = first_hits(eng, targets)
     info = eng.info()
+    mean_icm = float(np.mean([eng.instance_info(i).nc for i in range(len(specs))]))
+    eng.close()
     # time to target: the poll that first saw E* (an upper bound within `poll` steps)
     ttt = np.where(hits >= 0, t_lad + t_hit, np.inf)
     wall = t_lad + t_run
@@ -151,8 +154,7 @@ def run_batched(ptx, specs, targets, max_steps, poll, seed, log=print, surrogate
             "ladder_stages_s": {k: round(v, 3) for k, v in stages.items()},
             "spin_updates_per_s": info.spin_updates_per_step * steps / t_run, "mode": info.mode,
             "waves_per_group": info.waves_per_group, "autotune_s": t_tune,
-            "mean_slots": float(np.mean([len(l) for l in lads])),
-            "mean_icm_slots": float(np.mean([eng.instance_info(i).nc for i in range(len(specs))]))}
+            "mean_slots": float(np.mean([len(l) for l in lads])), "mean_icm_slots": mean_icm}
 
 
 def _rates(eng, inst):
@@ -164,10 +166,14 @@ def _rates(eng, inst):
     return suc / np.maximum(att, 1)
 
 
-def run_batched_judged_in_run(ptx, specs, targets, max_steps, poll, seed, log=print, validate_steps=2000):
+def run_batched_judged_in_run(ptx, specs, targets, max_steps, poll, seed, log=print, validate_steps=2000,
+                              start="surrogate"):
     """Batched + surrogate with the judge inside the run: every instance starts
     the main run (ICM on) on its network ladder right after the 8-slot pilot,
     and the first validate_steps steps of that run are the validation run.
+    start="linear" runs the same protocol from ladders linear in beta with
+    the network's slot count (the network's spacing replaced by even steps),
+    which separates what the network adds from what the protocol adds.
     Then an instance whose measured adjacent swap rates are not all within
     TARGET +- TOL, and that has not reached E* yet, leaves the run and falls
     back to the multihistogram preprocess ladder (computed on a worker thread
@@ -185,7 +191,11 @@ def run_batched_judged_in_run(ptx, specs, targets, max_steps, poll, seed, log=pr
 
     sur = S.Surrogate(ptx)
     X, bs, be = S.short_features(ptx, probs, stats, T0, T1, 500, 1000, seed)
-    lads = [np.sort(l) for l in sur.ladders(X, bs, be)]
+    lads = sur.ladders(X, bs, be)
+    sur.close()
+    if start == "linear":
+        lads = [np.linspace(bs, be, len(l) + 1)[:-1] for l in lads]
+    lads = [np.sort(l) for l in lads]
     t_lad = now()
     t0 = time.perf_counter()
     kw, _ = ptx.autotune(probs, lads, seed=seed + 10, steps=32, flags=ptx.FLAG_ICM)
@@ -256,8 +266,11 @@ def run_batched_judged_in_run(ptx, specs, targets, max_steps, poll, seed, log=pr
             break
     wall = now()
     ttt = t_hit
-    return {"instances": n, "hit_rate": float(done.mean()), "p50_ttt_s": float(np.median(ttt)), "wall_s": wall,
-            "ladder_s": t_lad, "judge_at_s": t_judge, "validate_steps": validate_steps,
+    eng.close()
+    if eng2 is not None:
+        eng2.close()
+    return {"instances": n, "start_ladders": start, "hit_rate": float(done.mean()), "p50_ttt_s": float(np.median(ttt)),
+            "wall_s": wall, "ladder_s": t_lad, "judge_at_s": t_judge, "validate_steps": validate_steps,
             "surrogate_accepted": int(ok.sum()), "done_in_validation": int(n - len(fb) - int(in_run.sum())),
             "fallback": int(len(fb)), "fallback_ready_s": res.get("t_ready"),
             "fallback_timing_s": {k: round(v, 3) for k, v in res.get("timing", {}).items()},
@@ -299,22 +312,37 @@ def load_targets(path=TARGETS):
         return json.load(f)
 
 
-def run(ptx, n_plain=8, seed=2024, log=print, path=TARGETS):
+ICM_SEMANTICS = ("ladders handed to the engine in ascending beta order; the reference's slot rule "
+                 "(src/initialization.cc:158-164) then gives cluster moves to the cold slots with beta >= the "
+                 "characteristic scale.  On the descending ladder of --preprocess that rule gives all slots or "
+                 "none (none here), so 'ICM on' is realised by the ascending order, for every path alike.")
+
+
+def run(ptx, n_plain=32, seed=2024, log=print, path=TARGETS, linear=True):
     cfg = load_targets(path)
     specs, targets = cfg["specs"], np.array(cfg["targets"], np.int64)
     b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
     bj = run_batched_judged_in_run(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
     log(f"judged in the run: {bj}")
+    bl = None
+    if linear:
+        bl = run_batched_judged_in_run(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log, start="linear")
+        log(f"judged in the run from linear ladders: {bl}")
     bm = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log, surrogate=False)
     # an even sample of both families (specs alternate torus, ER)
     stride = max(2, 2 * (len(specs) // max(n_plain, 1) // 2))
     idx = sorted(list(range(0, len(specs), stride))[: (n_plain + 1) // 2] +
                  list(range(1, len(specs), stride))[: n_plain // 2])
     pl = run_plain(ptx, [specs[i] for i in idx], targets[idx], cfg["max_steps"], cfg["poll"], seed + 7, log)
-    return {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "batched_surrogate": b,
-            "batched_surrogate_judged_in_run": bj, "batched_multihistogram": bm, "plain": pl,
-            "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"],
-            "instances_per_hour_ratio_judged_in_run": bj["instances_per_hour"] / pl["instances_per_hour"]}
+    out = {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "icm_semantics": ICM_SEMANTICS,
+           "batched_surrogate": b, "batched_surrogate_judged_in_run": bj, "batched_multihistogram": bm, "plain": pl,
+           "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"],
+           "instances_per_hour_ratio_judged_in_run": bj["instances_per_hour"] / pl["instances_per_hour"],
+           "instances_per_hour_ratio_multihistogram": bm["instances_per_hour"] / pl["instances_per_hour"]}
+    if bl is not None:
+        out["batched_linear_judged_in_run"] = bl
+        out["instances_per_hour_ratio_linear_judged_in_run"] = bl["instances_per_hour"] / pl["instances_per_hour"]
+    return out
 
 
 def main():
@@ -330,7 +358,7 @@ def main():
     c.add_argument("--seeds", type=int, default=4)
     c.add_argument("--out", default=TARGETS)
     r = sub.add_parser("run")
-    r.add_argument("--plain", type=int, default=8)
+    r.add_argument("--plain", type=int, default=32)
     r.add_argument("--targets", default=TARGETS)
     a = ap.parse_args()
     ptx = S.load_ptx()
