@@ -217,6 +217,20 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
                          "bytes_per_launch": bpl, "steps_per_timed_launch": steps_per_launch}}
 
 
+def ttt_summary(r):
+    """The time-to-target result without per-stage detail."""
+    keep = ("instances", "start_ladders", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
+            "done_in_validation", "fallback", "ladder_s", "run_s", "judge_at_s", "fallback_ready_s", "wall_s",
+            "ms_per_step", "mean_slots", "mean_icm_slots", "mean_preprocess_s", "mode")
+    out = {k: r[k] for k in ("workload", "max_steps", "calibration", "icm_semantics") if k in r}
+    for k, v in r.items():
+        if isinstance(v, dict):
+            out[k] = {kk: vv for kk, vv in v.items() if kk in keep}
+        elif k.startswith("instances_per_hour_ratio"):
+            out[k] = v
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,7 +246,11 @@ def main():
                     help="timed PT steps of the 1024-instance mixed batch (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=64,
                     help="PT steps of the batched 3D EA workload reported beside the main line (0 = skip)")
-    ap.add_argument("--ttt-plain", type=int, default=16,
+    ap.add_argument("--ttt-c5-targets",
+                    default=os.path.join(REPO, "pt-icm_amd", "data", "ttt_targets_c5.json"),
+                    help="second time-to-target workload at the batched + surrogate configuration's instance "
+                         "sizes ('' = skip)")
+    ap.add_argument("--ttt-plain", type=int, default=32,
                     help="instances of the plain path timed for time-to-target (0 = skip the time-to-target part)")
     ap.add_argument("--ttt-targets", default=os.path.join(REPO, "pt-icm_amd", "data", "ttt_targets.json"))
     args = ap.parse_args()
@@ -323,29 +341,15 @@ def main():
         out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
     if args.scale_steps > 0:
         out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed)
-    if args.ttt_plain > 0 and os.path.exists(args.ttt_targets):
+    if args.ttt_plain > 0:
         sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
         import ttt
-        r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=args.ttt_targets)
-        b, pl = r["batched_surrogate"], r["plain"]
-        out["time_to_target"] = {
-            "workload": r["workload"], "max_steps": r["max_steps"],
-            "batched_surrogate": {k: b[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
-                                                    "surrogate_accepted", "ladder_s", "run_s", "ms_per_step",
-                                                    "spin_updates_per_s", "mean_slots", "mean_icm_slots", "mode",
-                                                    "waves_per_group", "autotune_s")},
-            "batched_surrogate_judged_in_run": {
-                k: r["batched_surrogate_judged_in_run"][k]
-                for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
-                          "done_in_validation", "fallback", "ladder_s", "judge_at_s", "fallback_ready_s",
-                          "validate_steps", "steps_accepted_run", "steps_fallback_run", "wall_s")},
-            "batched_multihistogram": {k: r["batched_multihistogram"][k]
-                                       for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
-                                                 "ladder_s", "run_s")},
-            "plain": {k: pl[k] for k in ("instances", "hit_rate", "p50_ttt_s", "instances_per_hour",
-                                         "mean_preprocess_s")},
-            "instances_per_hour_ratio": r["instances_per_hour_ratio"],
-            "instances_per_hour_ratio_judged_in_run": r["instances_per_hour_ratio_judged_in_run"]}
+        for key, path, full in (("time_to_target", args.ttt_targets, True),
+                                ("time_to_target_c5_sizes", args.ttt_c5_targets, False)):
+            if not path or not os.path.exists(path):
+                continue
+            r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=path, linear=full, judge_before=full)
+            out[key] = ttt_summary(r)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
     p.close()

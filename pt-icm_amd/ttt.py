@@ -88,9 +88,11 @@ def calibrate(ptx, specs, seeds, steps, log=print):
     best = np.full(len(probs), np.iinfo(np.int64).max)
     for sd in seeds:
         t0 = time.perf_counter()
-        eng = ptx.Engine(probs, lads, seed=sd, flags=ptx.FLAG_ICM)
-        eng.step(steps)
-        eng.sync()
+        eng = ptx.Engine(probs, lads, seed=sd, flags=ptx.FLAG_ICM, use_graph=50)
+        for done in range(0, steps, 5000):
+            eng.step(min(5000, steps - done))
+            eng.sync()
+            log(f"seed {sd}: {done + min(5000, steps - done)} steps, {time.perf_counter() - t0:.1f} s")
         m = eng.min_energies()
         log(f"seed {sd}: {steps} steps in {time.perf_counter() - t0:.1f} s, improved {int(np.sum(m < best))}")
         best = np.minimum(best, m)
@@ -318,10 +320,14 @@ ICM_SEMANTICS = ("ladders handed to the engine in ascending beta order; the refe
                  "none (none here), so 'ICM on' is realised by the ascending order, for every path alike.")
 
 
-def run(ptx, n_plain=32, seed=2024, log=print, path=TARGETS, linear=True):
+def run(ptx, n_plain=32, seed=2024, log=print, path=TARGETS, linear=True, judge_before=True):
+    """Every path on the workload of the targets file.  linear=False skips the
+    linear-start variant, judge_before=False the surrogate path that judges its
+    ladders before the run (both informative only where some network ladders
+    pass the judge)."""
     cfg = load_targets(path)
     specs, targets = cfg["specs"], np.array(cfg["targets"], np.int64)
-    b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
+    b = run_batched(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log) if judge_before else None
     bj = run_batched_judged_in_run(ptx, specs, targets, cfg["max_steps"], cfg["poll"], seed, log)
     log(f"judged in the run: {bj}")
     bl = None
@@ -334,11 +340,14 @@ def run(ptx, n_plain=32, seed=2024, log=print, path=TARGETS, linear=True):
     idx = sorted(list(range(0, len(specs), stride))[: (n_plain + 1) // 2] +
                  list(range(1, len(specs), stride))[: n_plain // 2])
     pl = run_plain(ptx, [specs[i] for i in idx], targets[idx], cfg["max_steps"], cfg["poll"], seed + 7, log)
-    out = {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "icm_semantics": ICM_SEMANTICS,
-           "batched_surrogate": b, "batched_surrogate_judged_in_run": bj, "batched_multihistogram": bm, "plain": pl,
-           "instances_per_hour_ratio": b["instances_per_hour"] / pl["instances_per_hour"],
+    out = {"workload": cfg["workload"], "max_steps": cfg["max_steps"], "calibration": cfg.get("calibration"),
+           "icm_semantics": ICM_SEMANTICS, "batched_surrogate_judged_in_run": bj, "batched_multihistogram": bm,
+           "plain": pl,
            "instances_per_hour_ratio_judged_in_run": bj["instances_per_hour"] / pl["instances_per_hour"],
            "instances_per_hour_ratio_multihistogram": bm["instances_per_hour"] / pl["instances_per_hour"]}
+    if b is not None:
+        out["batched_surrogate"] = b
+        out["instances_per_hour_ratio"] = b["instances_per_hour"] / pl["instances_per_hour"]
     if bl is not None:
         out["batched_linear_judged_in_run"] = bl
         out["instances_per_hour_ratio_linear_judged_in_run"] = bl["instances_per_hour"] / pl["instances_per_hour"]
