@@ -344,11 +344,11 @@ def main():
     if args.ttt_plain > 0:
         sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
         import ttt
-        for key, path, full in (("time_to_target", args.ttt_targets, True),
-                                ("time_to_target_c5_sizes", args.ttt_c5_targets, False)):
-            if not path or not os.path.exists(path):
+        for key, tpath, full in (("time_to_target", args.ttt_targets, True),
+                                 ("time_to_target_c5_sizes", args.ttt_c5_targets, False)):
+            if not tpath or not os.path.exists(tpath):
                 continue
-            r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=path, linear=full, judge_before=full)
+            r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=tpath, linear=full, judge_before=full)
             out[key] = ttt_summary(r)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)
