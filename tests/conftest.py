@@ -12,6 +12,19 @@ REF_GRAPHS = os.path.join(REPO, "tests", "golden", "graphs")
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built libptx.so")
     config.addinivalue_line("markers", "slow: long-running statistical check")
+    config.addinivalue_line("markers", "gpu_subprocess: runs the GPU in a child process (pt_icm, facade_caller)")
+
+
+def pytest_collection_modifyitems(config, items):
+    """GPU tests that run the engine in a child process go first.  Once a
+    process has made a cooperative launch, the HIP runtime keeps the device's
+    cooperative-launch resource for that process until it exits, and a
+    cooperative launch from another process then fails ("the operation cannot
+    be performed in the present state"); the pytest process itself makes
+    cooperative launches in many tests.  The order of the other tests is kept."""
+    first = [it for it in items if it.get_closest_marker("gpu_subprocess")]
+    rest = [it for it in items if not it.get_closest_marker("gpu_subprocess")]
+    items[:] = first + rest
 
 
 def load_ptx():

@@ -10,7 +10,7 @@ import pytest
 
 from conftest import REPO, graph_path
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.gpu_subprocess]
 
 CLI = os.path.join(REPO, "pt-icm_amd", "build", "pt_icm")
 
