@@ -361,8 +361,9 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[inst] * (uint32_t)a.sweeps, tid, nthr);
     __syncthreads();
-    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit}
-                           : Csr<Fmt>{info, smem + L.adj, d.neg, d.kbit};
+    const uint32_t sent = sentinel_pos(d.nw);
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, 0, sent}
+                           : Csr<Fmt>{info, smem + L.adj, d.neg, d.kbit, 0, sent};
     const int32_t* hf = a.hfield + d.pos_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);

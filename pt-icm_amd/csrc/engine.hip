@@ -239,7 +239,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.pos_off = (int64_t)rowinfo.size();
         d.adj_off16 = (int64_t)adjw.size() / 4;
         // (the wide format masks padding by weight 0, so there any in-range position will do)
-        const uint32_t sentinel = std::min(32u * (uint32_t)d.nw, 65535u);
+        const uint32_t sentinel = sentinel_pos(d.nw);
         int64_t cur16 = 0;
         // engines that can run the cooperative kernel with one position per
         // lane order each row's records for its gathers (any order is exact)
@@ -383,7 +383,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     Wi = cfg.icm_waves;
     if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
     spin_stride = (int32_t)align_up((int64_t)(tab ? sign_table_bytes(nw_max) : spin_bytes(nw_max)), 16);
-    thr_stride = (int32_t)align_up(fmax_max, 4);
+    thr_stride = (int32_t)align_up(std::max(fmax_max, 1), 4);  // at least one word: the sweep may read entry 0
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
     if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
     // work items: largest instances first so the tail of the grid holds the small ones

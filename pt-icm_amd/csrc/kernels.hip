@@ -49,7 +49,8 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
         expand_bits(words, d.nw, sp, lane);
         wave_sync();
     }
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, 0,
+                       sentinel_pos(d.nw)};
     const int32_t E = energy_bytes(csr, d.n, a.hfield + d.pos_off, d.has_field != 0, sp, lane);
     if (lane == 0) {
         a.E[gr] = E;
@@ -99,12 +100,19 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
         for (int q = lane; q < spin_bytes(d.nw) / 4; q += kWave) dst[q] = src[q] ^ 0x01010101u;
         wave_sync();
     }
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, a.tab};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, a.tab,
+                       sentinel_pos(d.nw)};
     const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
+    // rows of one chunk each (lattices, sparse graphs): fixed-stride rows, no chunk loop
+    const bool one = d.max_deg <= Csr<Fmt>::kPerChunk;
     for (int sw = 0; sw < a.sweeps; ++sw) {
         const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
-        sweep_replica(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw, c3, a.seed, r,
-                      gsweep, sp, thr_row, best_words, st, lane);
+        if (one)
+            sweep_replica<Fmt, true>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw, c3,
+                                     a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
+        else
+            sweep_replica<Fmt, false>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw,
+                                      c3, a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
     }
     pack_bits(sp, d.nw, words, lane);
     if (lane == 0) {
@@ -203,7 +211,8 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat
     const int r1 = a.slot2rep[ro + t], r2 = a.slot2rep[ro + t + d.nt];
     uint32_t* g1 = a.spins + d.spin_off + (int64_t)r1 * d.nw;
     uint32_t* g2 = a.spins + d.spin_off + (int64_t)r2 * d.nw;
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit};
+    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, 0,
+                       sentinel_pos(d.nw)};
     long long dE1 = 0, dE2 = 0;
     const uint32_t c3i = ctr3(kTagIcm | a.tag_base, d.inst_id);
     if (W > 1 && d.lat_dim > 0) {  // lattices grow as bit sets on the first wave
@@ -322,8 +331,9 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
         succ[s] = 0;
     }
     __syncthreads();
-    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit}
-                           : Csr<Fmt>{info, smem + L.adj, d.neg, d.kbit};
+    const uint32_t sent = sentinel_pos(d.nw);
+    const Csr<Fmt> csr = G ? Csr<Fmt>{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, 0, sent}
+                           : Csr<Fmt>{info, smem + L.adj, d.neg, d.kbit, 0, sent};
     const int32_t* hf = a.hfield + d.pos_off;
     const int32_t* cls = a.cls + d.cls_off;
     const uint32_t c3s = ctr3(kTagSweep | a.tag_base, d.inst_id);

@@ -129,6 +129,7 @@ struct Csr {
     int neg = 1;    // 0: no coupling is negative (unit format: every sign bit is 0, agree = up neighbours)
     int kbit = 15;  // unit format: a record's sign bit (below it the position)
     int tab = 0;    // 1: sp holds the sign table (negated bytes at 2^kbit): a record addresses b ^ sign
+    uint32_t sent = 0;  // unit format: the sentinel position 32 * nw (its byte is 0, its sign bit 0)
     __device__ __forceinline__ uint32_t pos_mask() const { return (1u << kbit) - 1u; }
     static constexpr int kPerChunk = Fmt == kFmtUnit ? 8 : 4;
     __device__ __forceinline__ int deg(int p) const { return (int)(info[p] & (uint32_t)kMaxDegree); }
@@ -187,6 +188,40 @@ struct Csr {
         }
         return agree;
     }
+    // agree_chunks without masks: chunk c0 + k * step of the row when below
+    // nch (k = 0 always is), else a chunk of sentinel records, whose bytes
+    // are 0 and add nothing.  Every gather is issued for every lane (the
+    // sentinel reads are one broadcast address) and nothing branches, so the
+    // byte loads of all R chunks are in flight together.
+    template <int R, bool NEG>
+    __device__ __forceinline__ int agree_chunks_s(const uint4* r, int c0, int step, int nch, const uint8_t* sp) const {
+        const uint32_t s2 = sent * 0x10001u;
+        uint4 v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k == 0) {
+                v[k] = r[c0];
+            } else {
+                const uint4 t = r[min(c0 + k * step, nch - 1)];  // a chunk of the row either way
+                const bool in = c0 + k * step < nch;
+                v[k] = make_uint4(in ? t.x : s2, in ? t.y : s2, in ? t.z : s2, in ? t.w : s2);
+            }
+        }
+        int agree = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            uint32_t b[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                b[h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & pos_mask()]
+                           : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                agree += NEG ? (int)(b[h] ^ ((wv[h >> 1] >> (16 * (h & 1) + kbit)) & 1u)) : (int)b[h];
+        }
+        return agree;
+    }
     // sum_j w_pj s_j with s = 2b - 1, from byte spins
     __device__ __forceinline__ int field(int p, const uint8_t* sp) const {
         const uint32_t ri = info[p];
@@ -233,10 +268,12 @@ struct Csr {
             // 4 / T chunks per lane per round: one round for rows of up to 32 records
             constexpr int R = T >= 4 ? 1 : 4 / T;
             const int nch = (d + 7) >> 3;
-            if (neg) {
-                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks<R, true>(r, c0, T, nch, sp);
+            // (the loop bound keeps chunk c0 in the row; the round's later
+            // chunks are sentinel-filled past it)
+            if (neg && !tab) {
+                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks_s<R, true>(r, c0, T, nch, sp);
             } else {
-                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks<R, false>(r, c0, T, nch, sp);
+                for (int c0 = sub; c0 < nch; c0 += T * R) acc += agree_chunks_s<R, false>(r, c0, T, nch, sp);
             }
             return acc;
         }
@@ -262,6 +299,71 @@ struct Csr {
             group_field_t<false>(p0, cs, ce, sp, f);
         else
             group_field_t<true>(p0, cs, ce, sp, f);
+    }
+    // group_field when every row of the instance is one 16-byte chunk (max
+    // degree <= kPerChunk): the rows then sit at a fixed stride, row p being
+    // chunk p of the instance's adjacency, so the chunk loads need not wait
+    // for the row words (which give only the degrees) and there is no chunk
+    // loop.  Positions outside [cs, ce) get 0.
+    __device__ __forceinline__ void group_field_one(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
+        if (Fmt == kFmtUnit && (!neg || tab))
+            group_field_one_t<false>(p0, cs, ce, sp, f);
+        else
+            group_field_one_t<true>(p0, cs, ce, sp, f);
+    }
+    template <bool NEG>
+    __device__ __forceinline__ void group_field_one_t(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
+        // every load is issued for every lane (a valid row either way) and
+        // positions outside the class are masked at the end, so nothing branches
+        uint4 v[4];
+        int d[4], in[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = p0 + k;
+            in[k] = -(int)(p >= cs && p < ce);
+            const int q = in[k] ? p : cs;
+            v[k] = reinterpret_cast<const uint4*>(base)[q];
+            d[k] = (int)(info[q] & (uint32_t)kMaxDegree);
+        }
+        if (Fmt == kFmtUnit) {
+            uint32_t b[4][8];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                for (int h = 0; h < 8; ++h)
+                    b[k][h] = NEG ? sp[(wv[h >> 1] >> (16 * (h & 1))) & pos_mask()]
+                                  : sp[h & 1 ? wv[h >> 1] >> 16 : wv[h >> 1] & 0xFFFFu];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                int a = 0;
+#pragma unroll
+                for (int h = 0; h < 8; ++h)
+                    a += NEG ? (int)(b[k][h] ^ ((wv[h >> 1] >> (16 * (h & 1) + kbit)) & 1u)) : (int)b[k][h];
+                f[k] = (2 * a - d[k]) & in[k];
+            }
+        } else {
+            uint32_t b[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) b[k][h] = sp[wv[h] & 0xFFFFu];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t wv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+                int a = 0;
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const int w = (int32_t)wv[h] >> 16;
+                    a += b[k][h] ? w : -w;
+                }
+                f[k] = a & in[k];
+            }
+        }
     }
     template <bool NEG>
     __device__ __forceinline__ void group_field_t(int p0, int cs, int ce, const uint8_t* sp, int f[4]) const {
@@ -338,6 +440,10 @@ struct Csr {
 // padding records read) and the padding bytes past n stay 0.  lane/stride
 // let several waves share the work.
 __host__ __device__ inline int spin_bytes(int nw) { return 32 * (nw + 1); }
+// The position padding records name: the first byte of the sentinel word.
+__host__ __device__ inline uint32_t sentinel_pos(int nw) {
+    return 32u * (uint32_t)nw < 65535u ? 32u * (uint32_t)nw : 65535u;
+}
 // Sign table of the split sweep: the spin bytes at [0, spin_bytes) and their
 // negations at [2^K, 2^K + spin_bytes), K the smallest bit with 2^K >=
 // spin_bytes(nw) (so every position, the sentinel included, is below 2^K).
@@ -394,7 +500,7 @@ struct SweepState {
 // [cs, ce): one Philox block serves the four (word k for position 4g+k);
 // the four spin bytes are read and written as one 32-bit word (no other lane
 // owns any byte of it during the class).
-template <int Fmt>
+template <int Fmt, bool ONE = false>
 __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs, int ce,
                                              const int32_t* __restrict__ hf, bool has_field, uint32_t c3, uint32_t k0,
                                              uint32_t k1, int r, uint32_t gsweep, uint8_t* sp,
@@ -404,7 +510,10 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
     // pre (optional): this sweep's random words, pre[p] = word p&3 of block p>>2
     const int p0 = 4 * g;
     int f[4];
-    csr.group_field(p0, cs, ce, sp, f);
+    if (ONE)
+        csr.group_field_one(p0, cs, ce, sp, f);
+    else
+        csr.group_field(p0, cs, ce, sp, f);
     const uint32_t own = *reinterpret_cast<const uint32_t*>(sp + p0);
     int dE[4];
     bool any_up = false;
@@ -412,8 +521,14 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
     for (int k = 0; k < 4; ++k) {
         const int p = p0 + k;
         const bool in = p >= cs && p < ce;
-        const int fk = f[k] + ((has_field && in) ? hf[p] : 0);
-        dE[k] = in ? (((own >> (8 * k)) & 1u) ? -2 * fk : 2 * fk) : 0;
+        int fk = f[k];
+        if (ONE) {  // has_field is uniform over the wave: one scalar branch, unconditional loads
+            if (has_field) fk += hf[in ? p : cs] & -(int)in;
+        } else {
+            fk += (has_field && in) ? hf[p] : 0;
+        }
+        const int m = -(int)((own >> (8 * k)) & 1u);  // -1 for a spin set: dE = -2 f, else 2 f
+        dE[k] = in ? 2 * ((fk ^ m) - m) : 0;
         any_up |= dE[k] > 0;
     }
     Philox4 rnd = {{0u, 0u, 0u, 0u}};
@@ -432,7 +547,17 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int p = p0 + k;
-        if (p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1])) {
+        if (ONE) {
+            // threshold read for every position (entry 0 when dE <= 0, where
+            // it is not used; the row holds at least one word), no branches
+            // dE is even: entry dE/2 - 1 sits at byte 2 dE - 4
+            const uint32_t t =
+                *reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(thr_row) + 2 * max(dE[k], 2) - 4);
+            const uint32_t acc = (uint32_t)(p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < t));
+            flip |= acc << (8 * k);
+            dsum += dE[k] & -(int)acc;
+            nacc += (int)acc;
+        } else if (p >= cs && p < ce && (dE[k] <= 0 || rnd.v[k] < thr_row[(dE[k] >> 1) - 1])) {
             flip |= 1u << (8 * k);
             dsum += dE[k];
             ++nacc;
@@ -444,7 +569,7 @@ __device__ __forceinline__ void group_update(const Csr<Fmt>& csr, int g, int cs,
     }
 }
 
-template <int Fmt>
+template <int Fmt, bool ONE = false>
 __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
                                               const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
                                               uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
@@ -456,7 +581,8 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
         int dsum = 0, nacc = 0;
         uint8_t* spn = csr.tab ? sp + (1 << csr.kbit) : nullptr;
         for (int g = (cs >> 2) + lane; 4 * g < ce; g += kWave) {
-            group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc, nullptr, spn);
+            group_update<Fmt, ONE>(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc,
+                                   nullptr, spn);
         }
         wave_sync();
         dsum = wave_sum_i32(dsum);

@@ -38,6 +38,11 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--warmup", type=int, default=8, help="batched: untimed PT steps first")
+    ap.add_argument("--sweeps", type=int, default=1, help="coop: Metropolis sweeps per PT step")
+    ap.add_argument("--coop-waves", type=int, default=8, help="coop: waves per workgroup (4, 8, 16)")
+    ap.add_argument("--phases", action="store_true",
+                    help="coop: afterwards 500 more PT steps with the phase profile on (workgroup 0's split of a step)")
     ap.add_argument("--waves", type=int, default=0, help="waves per workgroup (0 = engine default)")
     ap.add_argument("--mode", type=int, default=-1,
                     help="execution mode (1 split, 2 fused; -1 = the autotuner's choice)")
@@ -55,7 +60,7 @@ def main():
         else:
             kw = {"mode": a.mode, "waves_per_group": a.waves}
         eng = ptx.Engine(probs, [lad] * len(probs), seed=0, flags=ptx.FLAG_ICM, **kw)
-        eng.step(8)
+        eng.step(a.warmup)
         eng.sync()
         t0 = time.perf_counter()
         eng.step(a.steps)
@@ -77,8 +82,8 @@ def main():
         lad = np.linspace(1.0 / 0.6, 0.367, 22)
         if a.icm:
             lad = lad[::-1].copy()
-        eng = ptx.Engine(p, lad, seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_COOP, waves_per_group=8,
-                         spins_per_lane=1, steps_per_launch=500)
+        eng = ptx.Engine(p, lad, seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_COOP, waves_per_group=a.coop_waves,
+                         spins_per_lane=1, steps_per_launch=500, sweeps=a.sweeps)
         eng.step(500)
         eng.sync()
         eng.timing(True)
@@ -96,12 +101,22 @@ def main():
             k = ts.fused_launches - t.fused_launches
             short = {"steps_per_launch": a.short, "launches": k,
                      "avg_launch_ms": (ts.fused_ms - t.fused_ms) / max(k, 1)}
+        phases = None
+        if a.phases:
+            eng.phase_profile(True)
+            eng.step(500)
+            eng.sync()
+            us = eng.phase_profile()
+            eng.phase_profile(False)
+            phases = dict(zip(["sweep", "exchange", "icm", "publish_wait"], (us / 500).round(3).tolist()))
         eng.close()
         avg = t.fused_ms / max(t.fused_launches, 1)
-        print(json.dumps({"workload": "G022, 22 slots x 2 chains, cooperative kernel, 8 waves", "icm_slots": nc,
+        print(json.dumps({"workload": "G022, 22 slots x 2 chains, cooperative kernel, %d waves, %d sweeps per step"
+                                      % (a.coop_waves, a.sweeps), "icm_slots": nc,
                           "launches": t.fused_launches, "steps_per_launch": 500, "avg_launch_ms": avg,
                           "us_per_step": avg * 1e3 / 500, "lds_bytes_per_step": info.lds_bytes_per_step,
-                          "lds_gbs": info.lds_bytes_per_step * 500 / (avg * 1e-3) / 1e9, "short": short}),
+                          "lds_gbs": info.lds_bytes_per_step * 500 / (avg * 1e-3) / 1e9, "short": short,
+                          "phases_us_per_step": phases}),
               flush=True)
     elif a.what == "lds":
         gbs = max(ptx.lds_bandwidth_gbs() for _ in range(3))
