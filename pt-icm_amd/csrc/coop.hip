@@ -156,10 +156,10 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
 // wait.  The slots are double-buffered by step parity: a slot is
 // rewritten two steps later, after every replica of the instance has
 // published the step in between, which it does only after reading this one.
-// coop_wait returns false when the wait exceeded its bound of wait_ticks
-// wall-clock ticks (100 MHz), or another workgroup gave up: the error flag is
-// then set, the host-mapped fault word too, and every workgroup leaves the
-// step loop, so the launch always ends.
+// coop_wait (wave 0 alone) returns false when the wait exceeded its bound of
+// wait_ticks wall-clock ticks (100 MHz), or another workgroup gave up: the
+// error flag is then set, the host-mapped fault word too, and every
+// workgroup leaves the step loop, so the launch always ends.
 template <bool REL>
 __device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t step, int32_t E) {
     if (REL) __syncthreads();  // the spin words are published
@@ -168,17 +168,21 @@ __device__ __forceinline__ void coop_arrive(unsigned long long* slot, uint32_t s
                            REL ? __ATOMIC_RELEASE : __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Lane l polls slot slot0 in the first round (slot0 = l, or a permutation
+// of 0..nrep-1 over the lanes below nrep) and returns that slot's energy in
+// e0; E[s] receives every slot's energy.
 template <bool ACQ>
 __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int nrep, uint32_t step, const CoopArgs& c,
-                                          int32_t* E, int* lds_flag) {
+                                          int32_t* E, int slot0, int32_t& e0) {
     const int tid = threadIdx.x;
-    if (tid < kWave) {
+    int ok = 1;
+    e0 = 0;
+    {
         const unsigned long long want = (unsigned long long)(step + 1u);
-        int ok = 1;
         unsigned long long spins = 0;
         const uint64_t t0 = wall_clock64();
         for (int s0 = 0; s0 < nrep; s0 += kWave) {
-            const int s = s0 + tid;
+            const int s = s0 == 0 ? slot0 : s0 + tid;
             unsigned long long v = 0;
             while (true) {
                 bool ready = true;
@@ -202,17 +206,16 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
             }
             if (!ok) break;
             if (s < nrep) E[s] = (int32_t)(uint32_t)v;
+            if (s0 == 0) e0 = (int32_t)(uint32_t)v;
         }
         if (ACQ) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (tid == 0) *lds_flag = ok;
     }
-    __syncthreads();
-    return *lds_flag != 0;
+    return ok != 0;
 }
 
 struct CoopLayout {
     int info, adj, cls, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, red,
-        lat_tab, acc, flag, rnd, total;
+        lat_tab, acc, flag, lua, kbuf, rnd, total;
 };
 
 // pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
@@ -252,7 +255,9 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int ncls
     L.red = take(8 * kIcmRedSlots);
     L.lat_tab = take(lat_stage && lat_words > 0 ? 4 * (16 * lat_words + n) : 0);
     L.acc = take(4 * 2 * 2 * kMaxCoopWaves);
-    L.flag = take(4);
+    L.flag = take(8);  // [0] an energy shared by the waves, [1] the step's wait succeeded
+    L.lua = take(2 * kWave * 8);  // log-uniforms of the exchange attempts, by step parity
+    L.kbuf = take(2 * kWave * 4);
     L.rnd = take(pre_rnd ? 16 * ((n + 3) / 4) : 0);
     L.total = o;
     return L;
@@ -336,25 +341,36 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     if (tid < 4 * kMaxCoopWaves) acc[tid] = 0;
     expand_bits(a.spins + d.spin_off + (int64_t)r * nw, nw, sp, tid, nthr);
     // register-resident exchange (up to 64 replicas): the first step's
-    // preparation here, later ones overlapped with the previous exchange
-    // (waves 1 and 2 prepare one half each and store their fields of ExchPre)
+    // preparation here, later ones overlapped with the previous step's wait
+    // and exchange (waves 1 and 2 prepare one half each and store their
+    // fields of ExchPre; with eight or more waves wave 3 draws the attempts'
+    // log-uniforms meanwhile)
     const bool prepared = nrep <= kWave && !a.no_exchange && nt > 1;
+    const bool lu_apart = prepared && W >= 8;
     ExchPre* pre = reinterpret_cast<ExchPre*>(smem + L.pre);
+    double* lua = reinterpret_cast<double*>(smem + L.lua);
+    uint32_t* kbuf = reinterpret_cast<uint32_t*>(smem + L.kbuf);  // sort keys, one block of 64 per preparing wave
     auto prepare_half = [&](uint32_t s, int h) {
-        const ExchPre x = exch_prepare(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, s, lane, h);
+        const ExchPre x = exch_prepare(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, s, lane,
+                                       kbuf + h * kWave, h, !lu_apart);
         ExchPre& slot = pre[(s & 1u) * kWave + lane];
         // constant indices keep x in registers
         if (h == 0) {
             slot.tpos[0] = x.tpos[0];
             slot.level[0] = x.level[0];
             slot.lu[0] = x.lu[0];
+            slot.maxlv[0] = x.maxlv[0];
         } else {
             slot.tpos[1] = x.tpos[1];
             slot.level[1] = x.level[1];
             slot.lu[1] = x.lu[1];
+            slot.maxlv[1] = x.maxlv[1];
         }
     };
     if (prepared && (wave == 1 || wave == 2)) prepare_half(a.inst_step[inst], wave - 1);
+    if (lu_apart && wave == 3)
+        exch_attempt_logu(nt, a.pilot != 0, a.seed, a.tag_base, d.inst_id, a.inst_step[inst], lane,
+                          lua + (a.inst_step[inst] & 1u) * kWave);
     // random words of the first sweep of this launch (later ones are drawn
     // while the step's wait is pending)
     uint32_t* pre_rnd = c.pre_rnd ? reinterpret_cast<uint32_t*>(smem + L.rnd) : nullptr;
@@ -407,33 +423,65 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         // agent-scope store: no release, no acquire after the wait)
         if (IC) pack_bits(sp, nw, pub_sp + par * c.pub_words + (int64_t)r * nw, tid, nthr);
         coop_arrive<IC != 0>(pub_e + (int64_t)par * c.pub_reps + r, step, st.E);
-        // while thread 0 waits: waves 1 and 2 prepare the next step's
-        // exchange, and waves 1.. draw the next step's first-sweep random
-        // words (this step's sweep has finished reading them)
+        // Wave 0 waits for the step's energies, then records the histogram
+        // (workgroup 0) and computes the exchange (every workgroup).  The
+        // other waves meanwhile prepare the next step: waves 1 and 2 its
+        // exchange, wave 3 (eight or more waves) its attempts' log-uniforms,
+        // and the waves after them (waves 1.. without a preparation) its
+        // first-sweep random words (this step's sweep has finished reading
+        // them).  One barrier closes the step; flag[1] tells every wave
+        // whether the wait succeeded.
         if (prepared && (wave == 1 || wave == 2) && it + 1 < nsteps) prepare_half(step + 1u, wave - 1);
-        if (pre_rnd && wave > 0 && it + 1 < nsteps)
-            fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - kWave,
-                               nthr - kWave);
-        if (!coop_wait<IC != 0>(pub_e + (int64_t)par * c.pub_reps, nrep, step, c, E, flag)) break;
-        phase_done(3);
-        // ---- histogram (workgroup 0) and exchange (every workgroup, wave 0);
-        //      wave 1 meanwhile prepares the next step's exchange ----
+        if (lu_apart && wave == 3 && it + 1 < nsteps)
+            exch_attempt_logu(nt, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step + 1u, lane,
+                              lua + ((step + 1u) & 1u) * kWave);
+        const int w0 = lu_apart ? 4 : 1;
+        if (pre_rnd && wave >= w0 && it + 1 < nsteps)
+            fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - w0 * kWave,
+                               nthr - w0 * kWave);
         if (wave == 0) {
-            if (r == 0) record_histogram(a, d, step, E, s2r, lane);
-            if (!a.no_exchange) {
-                if (prepared) {
-                    // one lane per slot: r2s written from the lane's own registers
-                    const ExchPre x = pre[(step & 1u) * kWave + lane];
-                    nsw_total += exch_apply(nt, nrep, a.pilot != 0, x, beta, E, s2r, att, succ, lane, r2s);
-                } else {
-                    nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step, beta,
-                                               E, s2r, att, succ, xs, lane);
-                    wave_sync();
-                    for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
+            // the prepared exchange's inputs that do not depend on this
+            // step's energies are read before the wait; each lane polls the
+            // slot of the replica its temperature slot holds, so that its
+            // energy arrives in the register the exchange starts from
+            ExchPre x;
+            int rep0 = lane;
+            double bk = 0.0;
+            if (prepared) {
+                x = pre[(step & 1u) * kWave + lane];
+                if (lu_apart) {
+                    const double* lu = lua + (step & 1u) * kWave;
+                    x.lu[0] = x.tpos[0] >= 0 ? lu[x.tpos[0]] : 0.0;
+                    x.lu[1] = x.tpos[1] >= 0 ? lu[x.tpos[1]] : 0.0;
+                }
+                if (lane < nrep) {
+                    rep0 = s2r[lane];
+                    bk = beta[lane];
+                }
+            }
+            int32_t e0 = 0;
+            const bool ok = coop_wait<IC != 0>(pub_e + (int64_t)par * c.pub_reps, nrep, step, c, E, rep0, e0);
+            wave_sync();  // E as the lanes stored it, for the lanes that read it
+            if (lane == 0) flag[1] = ok ? 1 : 0;
+            phase_done(3);
+            if (ok) {
+                if (r == 0) record_histogram(a, d, step, E, s2r, lane);
+                if (!a.no_exchange) {
+                    if (prepared) {
+                        // one lane per slot: r2s written from the lane's own registers
+                        nsw_total += exch_apply(nt, nrep, a.pilot != 0, x, lane < nrep ? rep0 : 0,
+                                                lane < nrep ? e0 : 0, bk, s2r, att, succ, lane, r2s);
+                    } else {
+                        nsw_total += exchange_wave(nt, nrep, a.pilot != 0, a.seed, a.tag_base, d.inst_id, step,
+                                                   beta, E, s2r, att, succ, xs, lane);
+                        wave_sync();
+                        for (int s = lane; s < nrep; s += kWave) r2s[s2r[s]] = s;
+                    }
                 }
             }
         }
         __syncthreads();
+        if (!flag[1]) break;
         phase_done(1);
         // ---- cluster move of this replica's slot pair (both partners compute it) ----
         if (IC && d.nc > 0) {

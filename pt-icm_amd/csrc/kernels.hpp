@@ -647,6 +647,7 @@ struct ExchScratch {
 struct ExchPre {
     int32_t tpos[2], level[2];
     double lu[2];
+    int32_t maxlv[2];  // the highest level of each half (the same in every lane)
 };
 
 // The value of lane + 1 (lane_next) or lane - 1 (lane_prev) by DPP wave
@@ -660,10 +661,14 @@ __device__ __forceinline__ int lane_prev(int v) {
     return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
 }
 
-// only_half >= 0 prepares that half alone (the other keeps tpos -1): two
-// waves can prepare the two halves at once.
+// kbuf: at least nt - 1 words of LDS owned by the calling wave, where the
+// sort keys of a half are compared.  only_half >= 0 prepares that half
+// alone (the other keeps tpos -1): two waves can prepare the two halves at
+// once.  with_lu = false leaves lu at 0 (exch_attempt_logu gives the
+// values, by attempt, from another wave).
 __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, uint64_t seed, uint32_t tag_base,
-                                                uint32_t inst_id, uint32_t step, int lane, int only_half = -1) {
+                                                uint32_t inst_id, uint32_t step, int lane, uint32_t* kbuf,
+                                                int only_half = -1, bool with_lu = true) {
     const int np = nt - 1;
     const int nhalf = pilot ? 1 : 2;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -678,6 +683,7 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
         x.tpos[h] = -1;
         x.level[h] = 0;
         x.lu[h] = 0.0;
+        x.maxlv[h] = 0;
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {  // unrolled: x is indexed by constants and stays in registers
@@ -685,17 +691,29 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
         // rank of pair kc in this half's order: pairs sorted by (key, index)
         const uint32_t key =
             is_pair ? pick(philox4x32_10((uint32_t)kc >> 2, (uint32_t)h, step, c3s, k0, k1), (uint32_t)kc) : 0u;
+        // the keys of pairs 0..np-1 (lanes 0..np-1: chain 0's pairs) in LDS,
+        // read back at one address for every lane, four reads in flight
+        wave_sync();  // an earlier half's reads of kbuf are done
+        if (lane < np) kbuf[lane] = key;
+        wave_sync();
         int rank = 0;
-        for (int i = 0; i < np; ++i) {
-            const uint32_t ki = (uint32_t)__builtin_amdgcn_readlane((int)key, i);  // lane i < np: pair i of chain 0
+        int i = 0;
+        for (; i + 4 <= np; i += 4) {
+            const uint32_t kk[4] = {kbuf[i], kbuf[i + 1], kbuf[i + 2], kbuf[i + 3]};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rank += (kk[j] < key) || (kk[j] == key && i + j < kc);
+        }
+        for (; i < np; ++i) {
+            const uint32_t ki = kbuf[i];
             rank += (ki < key) || (ki == key && i < kc);
         }
         const int a = h * np + rank;
         const int chain = pilot ? 0 : (a & 1);
         const int tpos = (is_pair && chain == my_chain) ? a : -1;
         x.tpos[h] = tpos;
-        x.lu[h] =
-            tpos >= 0 ? uniform_log(pick(philox4x32_10((uint32_t)a >> 2, 0u, step, c3e, k0, k1), (uint32_t)a)) : 0.0;
+        if (with_lu)
+            x.lu[h] = tpos >= 0 ? uniform_log(pick(philox4x32_10((uint32_t)a >> 2, 0u, step, c3e, k0, k1), (uint32_t)a))
+                                : 0.0;
         // dependency levels: 1 + the highest level of an earlier attempt on k-1 or k+1
         int level = 0;
         while (true) {
@@ -712,39 +730,54 @@ __device__ __forceinline__ ExchPre exch_prepare(int nt, int nrep, bool pilot, ui
             if (!changed) break;
         }
         x.level[h] = level;
+        x.maxlv[h] = wave_max_i32(level);
     }
     return x;
 }
 
+// The log-uniform of every attempt of the step, by attempt index (lane a
+// takes attempt a; at most 64 attempts: 2 (nt - 1) with two chains of at
+// most 32 slots, nt - 1 in pilot mode with at most 64): the lu values
+// exch_prepare would give the lanes whose tpos is a.
+__device__ __forceinline__ void exch_attempt_logu(int nt, bool pilot, uint64_t seed, uint32_t tag_base,
+                                                  uint32_t inst_id, uint32_t step, int lane, double* out) {
+    const int na = (pilot ? 1 : 2) * (nt - 1);
+    const uint32_t c3e = ctr3(kTagExch | tag_base, inst_id);
+    if (lane < na)
+        out[lane] = uniform_log(
+            pick(philox4x32_10((uint32_t)lane >> 2, 0u, step, c3e, (uint32_t)seed, (uint32_t)(seed >> 32)),
+                 (uint32_t)lane));
+}
+
 // r2s (optional): also written, r2s[replica] = slot, from the same registers.
-__device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool pilot, const ExchPre& x,
-                                                         const double* beta, const int32_t* E, int32_t* s2r,
-                                                         uint32_t* att, uint32_t* succ, int lane,
-                                                         int32_t* r2s = nullptr) {
+// rep, es, bk: the lane's slot's replica, its energy and the slot's beta
+// (0 for lanes at or past nrep), read by the caller -- a caller can read
+// rep and bk, and poll es, before the step's energies are all in.
+__device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool pilot, const ExchPre& x, int rep,
+                                                         int32_t es, double bk, int32_t* s2r, uint32_t* att,
+                                                         uint32_t* succ, int lane, int32_t* r2s = nullptr) {
     const int nhalf = pilot ? 1 : 2;
     const bool valid = lane < nrep;
     const int kc = valid ? lane % nt : 0;
-    int rep = valid ? s2r[lane] : 0;
-    int32_t es = valid ? E[rep] : 0;
-    const double bk = valid ? beta[lane] : 0.0;
     // beta of slot k + 1 (used only at a decision), by DPP on both halves
     const unsigned long long bkb = (unsigned long long)__double_as_longlong(bk);
     const double bk1 = __longlong_as_double(
         (long long)(((unsigned long long)(uint32_t)lane_next((int)(uint32_t)(bkb >> 32)) << 32) |
                     (uint32_t)lane_next((int)(uint32_t)bkb)));
+    const double db = bk - bk1;
     uint32_t natt = 0, nsucc = 0;
     unsigned long long nsw = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {  // unrolled: x is indexed by constants and stays in registers
         if (h >= nhalf) break;
         const int level = x.level[h];
-        const int maxlv = wave_max_i32(level);
+        const int maxlv = __builtin_amdgcn_readfirstlane(x.maxlv[h]);
         for (int L = 1; L <= maxlv; ++L) {
             const int rep_r = lane_next(rep);
             const int32_t es_r = lane_next(es);
             int sw = 0;
             if (level == L) {
-                const double lb = (bk - bk1) * (double)(es - es_r);
+                const double lb = db * (double)(es - es_r);
                 ++natt;
                 if (x.lu[h] < lb) {
                     sw = 1;
@@ -776,9 +809,12 @@ __device__ __forceinline__ unsigned long long exch_apply(int nt, int nrep, bool 
 __device__ __forceinline__ unsigned long long exchange_wave_reg(int nt, int nrep, bool pilot, uint64_t seed,
                                                                 uint32_t tag_base, uint32_t inst_id, uint32_t step,
                                                                 const double* beta, const int32_t* E, int32_t* s2r,
-                                                                uint32_t* att, uint32_t* succ, int lane) {
-    const ExchPre x = exch_prepare(nt, nrep, pilot, seed, tag_base, inst_id, step, lane);
-    return exch_apply(nt, nrep, pilot, x, beta, E, s2r, att, succ, lane);
+                                                                uint32_t* att, uint32_t* succ, uint32_t* kbuf,
+                                                                int lane) {
+    const ExchPre x = exch_prepare(nt, nrep, pilot, seed, tag_base, inst_id, step, lane, kbuf);
+    const bool valid = lane < nrep;
+    const int rep = valid ? s2r[lane] : 0;
+    return exch_apply(nt, nrep, pilot, x, rep, valid ? E[rep] : 0, valid ? beta[lane] : 0.0, s2r, att, succ, lane);
 }
 
 __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bool pilot, uint64_t seed,
@@ -789,7 +825,7 @@ __device__ __forceinline__ unsigned long long exchange_wave(int nt, int nrep, bo
     if (np <= 0) return 0ull;
     if (nrep <= kWave) {
         const unsigned long long r = exchange_wave_reg(nt, nrep, pilot, seed, tag_base, inst_id, step, beta, E, s2r,
-                                                       att, succ, lane);
+                                                       att, succ, x.key, lane);
         wave_sync();
         return r;
     }

@@ -137,8 +137,36 @@ PTX_HD double pexp(double x) {
      0x1.306fe0a31b715p-1, 0x1.29e9df51fdee1p-1, 0x1.2387a6e756238p-1, 0x1.1d4873168b9aap-1,               \
      0x1.172b83c7d517bp-1, 0x1.11301d0125b51p-1, 0x1.0b5586cf9890fp-1, 0x1.059b0d3158574p-1}
 
+// j for m in [1,2): the largest j in 0..31 with m * tab[j] >= 1, tab[j] =
+// 2^(-j/32).  The test is monotone in j (tab decreases, rounding is
+// monotone), so from any guess h within one of j three independent tests
+// settle it: j = h + 1 if that passes, else h if that passes, else h - 1.
+// The guess is floor(32 log2 m): on the device from the single-precision
+// hardware logarithm, on the host in double precision.  Either is within
+// 1e-4 of 32 log2 m, and j differs from floor(32 log2 m) only where 32
+// log2 m is within 1e-13 of an integer, so the guess is within one of j.
+// (Three table reads in parallel instead of a bisection's five dependent
+// ones, which on the device are five memory round trips.)
+PTX_HD int plog_settle(double m, int h, const double* tab) {
+#pragma clang fp contract(off)
+    const int a = h - 1 < 0 ? 0 : (h - 1 > 31 ? 31 : h - 1);
+    const int b = h < 0 ? 0 : (h > 31 ? 31 : h);
+    const int c = h + 1 < 0 ? 0 : (h + 1 > 31 ? 31 : h + 1);
+    const double tc = tab[c], tb = tab[b];
+    if (m * tc >= 1.0) return c;
+    if (m * tb >= 1.0) return b;
+    return a;
+}
+PTX_HD int plog_index(double m, const double* tab) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return plog_settle(m, (int)(32.0f * __builtin_amdgcn_logf((float)m)), tab);
+#else
+    return plog_settle(m, (int)(32.0 * __builtin_log2(m)), tab);
+#endif
+}
+
 // log x: x = 2^k m, m in [1,2); j = largest index with m 2^(-j/32) >= 1
-// (binary search); t = m 2^(-j/32) - 1 in [0, 0.0222); log x = (k + j/32) ln2
+// (plog_index); t = m 2^(-j/32) - 1 in [0, 0.0222); log x = (k + j/32) ln2
 // + log1p(t) by its degree-13 Taylor series.  0 -> -inf, negative -> NaN.
 PTX_HD double plog(double x) {
 #pragma clang fp contract(off)
@@ -155,12 +183,7 @@ PTX_HD double plog(double x) {
     }
     k += (int)((b >> 52) & 0x7FF) - 1023;
     const double m = bits_as_double((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
-    int lo = 0, hi = 31;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (m * tab[mid] >= 1.0) lo = mid;
-        else hi = mid - 1;
-    }
+    const int lo = plog_index(m, tab);
     const double t = m * tab[lo] - 1.0;
     double p = 1.0 / 13.0;
     p = p * t + -1.0 / 12.0;
