@@ -224,6 +224,8 @@ typedef struct {
     int32_t n, nt, nrep, nc, n_colours;
     int32_t hist_bins;
     int64_t e_base;
+    int32_t lattice_sweep;   /* 2 or 3: split-mode sweeps of this instance use the multispin lattice
+                                sweep (periodic lattice, even L, couplings +-1, no field); 0: gathers */
 } ptx_instance_info;
 ptx_status ptx_engine_instance_info(const ptx_engine* e, int32_t inst, ptx_instance_info* out);
 

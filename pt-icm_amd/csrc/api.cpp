@@ -376,6 +376,7 @@ ptx_status ptx_engine_instance_info(const ptx_engine* e, int32_t inst, ptx_insta
     out->n_colours = d.ncls;
     out->hist_bins = d.hist_bins;
     out->e_base = d.e_base;
+    out->lattice_sweep = d.lsw_dim;
     return PTX_OK;
 }
 

@@ -314,12 +314,11 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
 
     // ---- stage the instance and this replica ----
     if (!G) {  // G = 1: the adjacency is read through L2
-        for (int p = tid; p < n; p += nthr) info[p] = a.rowinfo[d.pos_off + p];
-        const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
-        uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
-        for (int q = tid; q < (int)d.adj_len16; q += nthr) dst[q] = src[q];
+        stage_to_lds(info, a.rowinfo + d.pos_off, n, tid, nthr);
+        stage_to_lds(reinterpret_cast<uint4*>(smem + L.adj), reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16),
+                     (int)d.adj_len16, tid, nthr);
     }
-    for (int m = tid; m < nt * d.fmax; m += nthr) thr[m] = a.thr[d.thr_off + m];
+    stage_to_lds(thr, a.thr + d.thr_off, nt * d.fmax, tid, nthr);
     int32_t* cls = reinterpret_cast<int32_t*>(smem + L.cls);  // class bounds, read at every class
     for (int m = tid; m <= d.ncls; m += nthr) cls[m] = a.cls[d.cls_off + m];
     LatView lv = lat_view(a, d);

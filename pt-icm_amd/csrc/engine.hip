@@ -133,6 +133,54 @@ struct BankOrder {
         std::copy(out.begin(), out.end(), rec);
     }
 };
+
+// Table of the multispin lattice sweep (InstDesc::lsw_dim), appended to
+// out; false when the instance does not qualify: a periodic L^D lattice
+// (D = 2, 3) with even L, couplings +-1, no field, and positions that are the
+// two site parities in index order (DSatur colours an even periodic lattice
+// by parity, vertex 0 first; the check below confirms it for every
+// position).  For each class c and window w of 32 class positions q (site
+// x = 2k + par in row r = y + L z, q = r L/2 + k, par = (c + y + z) & 1):
+// the edge signs per direction (+x, -x, +y, -y, +z, -z; bit = the coupling
+// is negative) and the shape masks (par, k = 0, k = L/2 - 1, y = 0,
+// y = L - 1, z = 0, z = L - 1).
+bool lattice_sweep_table(const Problem& P, std::vector<uint32_t>& out) {
+    const int32_t D = P.lat_dim, L = P.lat_L;
+    if (D < 2 || D > 3 || L < 4 || (L & 1)) return false;
+    const int32_t H = L / 2;
+    if (P.has_field || P.ncls != 2 || P.class_start[1] != P.n / 2) return false;
+    for (int32_t wv : P.w)
+        if (wv != 1 && wv != -1) return false;
+    for (int32_t p = 0; p < P.n; ++p) {
+        const int32_t o = P.pos2orig[p], x = o % L, r = o / L, y = r % L, z = r / L;
+        if (p != ((x + y + z) & 1) * (P.n / 2) + r * H + x / 2) return false;
+    }
+    auto weight = [&](int32_t i, int32_t j) {
+        for (int32_t q = P.rowptr[i]; q < P.rowptr[i + 1]; ++q)
+            if (P.col[q] == j) return P.w[q];
+        return 0;
+    };
+    const int32_t hc = P.n / 2, nwin = (hc + 31) / 32;
+    const size_t base = out.size();
+    out.resize(base + (size_t)2 * nwin * kLswGroup, 0u);
+    for (int32_t c = 0; c < 2; ++c) {
+        for (int32_t q = 0; q < hc; ++q) {
+            const int32_t r = q / H, k = q % H, y = r % L, z = r / L, par = (c + y + z) & 1;
+            const int32_t x = 2 * k + par, o = x + L * r;
+            uint32_t* g = out.data() + base + ((size_t)c * nwin + q / 32) * kLswGroup;
+            const uint32_t bit = 1u << (q % 32);
+            const int32_t nb[6] = {(x + 1) % L + L * r, (x + L - 1) % L + L * r,
+                                   x + L * ((y + 1) % L + L * z), x + L * ((y + L - 1) % L + L * z),
+                                   x + L * (y + L * ((z + 1) % L)), x + L * (y + L * ((z + L - 1) % L))};
+            for (int32_t j = 0; j < 2 * D; ++j)
+                if (weight(o, nb[j]) < 0) g[j] |= bit;
+            const bool m[7] = {par == 1, k == 0, k == H - 1, y == 0, y == L - 1, z == 0, z == L - 1};
+            for (int32_t j = 0; j < 7; ++j)
+                if (m[j]) g[8 + j] |= bit;
+        }
+    }
+    return true;
+}
 }  // namespace
 
 Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problems, const double* betas,
@@ -176,6 +224,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     std::vector<double> beta;
     std::vector<uint32_t> lat;
     std::vector<uint32_t> jneg;  // lattice edge signs, per instance
+    std::vector<uint32_t> lsign; // edge signs of the multispin lattice sweep, per instance
+    int32_t lsw_bytes = 0;       // LDS bytes per wave of the lattice sweep's largest instance
     struct LatShape {
         int32_t dim, L;
         const Problem* prob;
@@ -362,6 +412,13 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 }
             }
         }
+        d.lsw_dim = d.lsw_L = 0;
+        d.lsw_off = (int64_t)lsign.size();
+        if (lattice_sweep_table(P, lsign)) {
+            d.lsw_dim = P.lat_dim;
+            d.lsw_L = P.lat_L;
+            lsw_bytes = std::max(lsw_bytes, lsw_lds_bytes(d.nw, d.n, d.lsw_dim));
+        }
         d.e_base = (int32_t)(-P.abs_sum);
         d.hist_bins = hist ? (int32_t)(P.abs_sum + 1) : 0;
         d.hist_off = hist_off;
@@ -382,7 +439,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
     Wi = cfg.icm_waves;
     if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
-    spin_stride = (int32_t)align_up((int64_t)(tab ? sign_table_bytes(nw_max) : spin_bytes(nw_max)), 16);
+    spin_stride = (int32_t)align_up(
+        std::max<int64_t>(tab ? sign_table_bytes(nw_max) : spin_bytes(nw_max), lsw_bytes), 16);
     thr_stride = (int32_t)align_up(std::max(fmax_max, 1), 4);  // at least one word: the sweep may read entry 0
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
     if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
@@ -491,6 +549,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     PTX_TRY(d_lat.upload(lat, stream));
     if (jneg.empty()) jneg.assign(4, 0u);
     PTX_TRY(d_jneg.upload(jneg, stream));
+    if (lsign.empty()) lsign.assign(kLswGroup, 0u);
+    PTX_TRY(d_lsign.upload(lsign, stream));
     PTX_TRY(d_thr.upload(thr, stream));
     PTX_TRY(d_beta.upload(beta, stream));
     PTX_TRY(build_lists());
@@ -541,6 +601,7 @@ static DevArgs build_args(Engine& e) {
     a.order = e.d_order.p;
     a.lat = e.d_lat.p;
     a.jneg = e.d_jneg.p;
+    a.lsign = e.d_lsign.p;
     a.phase = e.phase_on ? e.d_phase.p : nullptr;
     a.fault = e.d_fault;
     a.obs_i = reinterpret_cast<long long*>(e.d_obs_i.p);

@@ -84,6 +84,7 @@ struct Engine {
     DevBuf<int32_t> d_cls, d_h, d_p2o;
     DevBuf<uint32_t> d_lat;      // lattice tables (wpos, obit) per lattice shape
     DevBuf<uint32_t> d_jneg;     // lattice edge signs per instance (cut energies on bit sets)
+    DevBuf<uint32_t> d_lsign;    // edge signs of the multispin lattice sweep per instance (InstDesc::lsw_off)
     DevBuf<uint32_t> d_rowinfo, d_adj, d_thr;
     DevBuf<double> d_beta;
     DevBuf<uint32_t> d_spins, d_best_spins;

@@ -68,7 +68,9 @@ __global__ __launch_bounds__(W* kWave) void k_init(DevArgs a, int mode) {
 
 // ---------------------------------------------------------------------------
 // k_sweep: W wavefronts per workgroup, each sweeping one replica of the
-// workgroup's instance.  LDS per wave: spin bytes + the replica's threshold row.
+// workgroup's instance.  LDS per wave: spin bytes (lattice instances with the
+// multispin sweep: state bits and uphill level masks) + the replica's
+// threshold row.
 // ---------------------------------------------------------------------------
 template <int Fmt, int W>
 __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
@@ -87,34 +89,55 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
     uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r * d.nw;
     const int t = a.rep2slot[gr] % d.nt;
     for (int m = lane; m < d.fmax; m += kWave) thr_row[m] = a.thr[d.thr_off + (int64_t)t * d.fmax + m];
-    expand_bits(words, d.nw, sp, lane);
     SweepState st;
     st.E = a.E[gr];
     st.bestE = a.bestE[gr];
     st.best_sweep = a.best_sweep[gr];
     st.flips = 0;
-    wave_sync();
-    if (a.tab) {  // the sign table's negated bytes at 2^kbit
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(sp + (1 << d.kbit));
-        for (int q = lane; q < spin_bytes(d.nw) / 4; q += kWave) dst[q] = src[q] ^ 0x01010101u;
-        wave_sync();
-    }
-    const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, a.tab,
-                       sentinel_pos(d.nw)};
     const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
-    // rows of one chunk each (lattices, sparse graphs): fixed-stride rows, no chunk loop
-    const bool one = d.max_deg <= Csr<Fmt>::kPerChunk;
-    for (int sw = 0; sw < a.sweeps; ++sw) {
-        const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
-        if (one)
-            sweep_replica<Fmt, true>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw, c3,
-                                     a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
-        else
-            sweep_replica<Fmt, false>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw,
-                                      c3, a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
+    if (d.lsw_dim > 0) {  // periodic lattice: multispin sweep on the state bits
+        uint32_t* S0 = reinterpret_cast<uint32_t*>(sp);
+        uint32_t* S = S0 + 1;                      // state words; S[-1], S[nw], S[nw + 1] are zero
+        uint32_t* Z = S0 + lsw_state_words(d.nw);  // level masks, zero pads included
+        for (int w = lane; w < lsw_state_words(d.nw); w += kWave) S0[w] = w >= 1 && w <= d.nw ? words[w - 1] : 0u;
+        for (int w = lane; w < d.lsw_dim * lsw_zwords(d.n); w += kWave) Z[w] = 0u;
+        wave_sync();
+        const uint32_t* tab = a.lsign + d.lsw_off;
+        const int32_t* cls = a.cls + d.cls_off;
+        for (int sw = 0; sw < a.sweeps; ++sw) {
+            const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
+            if (d.lsw_dim == 3)
+                sweep_lattice<3>(tab, d.lsw_L, d.n, cls, d.nw, c3, a.seed, r, gsweep, S, Z, thr_row, best_words, st,
+                                 lane);
+            else
+                sweep_lattice<2>(tab, d.lsw_L, d.n, cls, d.nw, c3, a.seed, r, gsweep, S, Z, thr_row, best_words, st,
+                                 lane);
+        }
+        for (int w = lane; w < d.nw; w += kWave) words[w] = S[w];
+    } else {
+        expand_bits(words, d.nw, sp, lane);
+        wave_sync();
+        if (a.tab) {  // the sign table's negated bytes at 2^kbit
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(sp + (1 << d.kbit));
+            for (int q = lane; q < spin_bytes(d.nw) / 4; q += kWave) dst[q] = src[q] ^ 0x01010101u;
+            wave_sync();
+        }
+        const Csr<Fmt> csr{a.rowinfo + d.pos_off, a.adj + d.adj_off16 * 16, d.neg, d.kbit, a.tab,
+                           sentinel_pos(d.nw)};
+        // rows of one chunk each (lattices, sparse graphs): fixed-stride rows, no chunk loop
+        const bool one = d.max_deg <= Csr<Fmt>::kPerChunk;
+        for (int sw = 0; sw < a.sweeps; ++sw) {
+            const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
+            if (one)
+                sweep_replica<Fmt, true>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw, c3,
+                                         a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
+            else
+                sweep_replica<Fmt, false>(csr, a.cls + d.cls_off, d.ncls, a.hfield + d.pos_off, d.has_field != 0, d.nw,
+                                          c3, a.seed, r, gsweep, sp, thr_row, best_words, st, lane);
+        }
+        pack_bits(sp, d.nw, words, lane);
     }
-    pack_bits(sp, d.nw, words, lane);
     if (lane == 0) {
         a.E[gr] = st.E;
         a.bestE[gr] = st.bestE;
@@ -315,12 +338,11 @@ __global__ __launch_bounds__(W* kWave) void k_fused(DevArgs a, int nsteps) {
 
     // ---- stage the instance ----
     if (!G) {
-        for (int p = tid; p < n; p += W * kWave) info[p] = a.rowinfo[d.pos_off + p];
-        const uint4* src = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
-        uint4* dst = reinterpret_cast<uint4*>(smem + L.adj);
-        for (int q = tid; q < adj16; q += W * kWave) dst[q] = src[q];
+        stage_to_lds(info, a.rowinfo + d.pos_off, n, tid, W * kWave);
+        stage_to_lds(reinterpret_cast<uint4*>(smem + L.adj), reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16),
+                     adj16, tid, W * kWave);
     }
-    for (int m = tid; m < nt * d.fmax; m += W * kWave) thr[m] = a.thr[d.thr_off + m];
+    stage_to_lds(thr, a.thr + d.thr_off, nt * d.fmax, tid, W * kWave);
     const int64_t ro = d.rep_off;
     for (int s = tid; s < nrep; s += W * kWave) {
         beta[s] = a.beta[d.beta_off + s];

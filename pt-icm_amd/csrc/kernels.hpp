@@ -53,6 +53,7 @@ struct DevArgs {
     const int32_t* order;    // fused mode: instance per workgroup
     const uint32_t* lat;     // lattice tables (LatView), one per lattice shape
     const uint32_t* jneg;    // lattice edge signs per instance (InstDesc::jn_off)
+    const uint32_t* lsign;   // edge signs of the multispin lattice sweep per instance (InstDesc::lsw_off)
     int32_t fmt;             // adjacency record format
     int32_t tab;             // 1: unit records carry the sign at bit kbit and k_sweep keeps the sign table
     unsigned long long* phase;  // optional: wall-clock ticks of sweep/exchange/ICM (workgroup 0)
@@ -440,6 +441,25 @@ struct Csr {
 // padding records read) and the padding bytes past n stay 0.  lane/stride
 // let several waves share the work.
 __host__ __device__ inline int spin_bytes(int nw) { return 32 * (nw + 1); }
+// Copy of count elements from global memory into LDS by threads tid, tid +
+// nthr, ...: each thread issues up to 8 loads before its first store, so a
+// workgroup's staging costs about one memory round trip per 8 elements a
+// thread copies instead of one per element (a load feeding the LDS store
+// of the same iteration serialises the loop on memory latency).
+template <typename T>
+__device__ __forceinline__ void stage_to_lds(T* __restrict__ dst, const T* __restrict__ src, int count, int tid,
+                                             int nthr) {
+    constexpr int U = 8;
+    for (int q0 = tid; q0 < count; q0 += U * nthr) {
+        T v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (q0 + k * nthr < count) v[k] = src[q0 + k * nthr];
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (q0 + k * nthr < count) dst[q0 + k * nthr] = v[k];
+    }
+}
 // The position padding records name: the first byte of the sentinel word.
 __host__ __device__ inline uint32_t sentinel_pos(int nw) {
     return 32u * (uint32_t)nw < 65535u ? 32u * (uint32_t)nw : 65535u;
@@ -593,6 +613,182 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
             st.bestE = st.E;
             st.best_sweep = (int32_t)gsweep;
             pack_bits(sp, nw, best_words, lane);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multispin sweep of a periodic lattice (InstDesc::lsw_dim = D > 0): one wave
+// per replica, the state as position-order bits in LDS, the same decisions
+// as sweep_replica (Replica::sweep, src/replica.cc:5-38, in colour order).
+//
+// The two colour classes are the site parities, each listing its sites in
+// index order: the class-c sites of lattice row r = y + L z are the H = L/2
+// consecutive positions class_start[c] + r H + k (site x = 2k + par, par =
+// (c + y + z) & 1).  So the neighbours of class position q in the other class
+// sit at q + delta for a few fixed offsets: 0 and +1 or -(H-1) (+x), 0 and -1
+// or +(H-1) (-x), by par and by k = H-1 or 0; +-H or -+(L-1) H along y and
+// +-L H or -+(L-1) L H along z, the second value where the axis wraps.  A
+// colour class is swept in two phases:
+//  A  one lane per window of 32 class positions: the neighbours' bits are 32-
+//     bit fields of the other class's bits at those offsets, chosen per bit by
+//     the window's shape masks; the bond bits B = ~(s ^ s' ^ neg), 1 where
+//     w s s' = +1 (dE = 2 deg - 4 count(B)), are counted in three bit planes;
+//     sites with count >= D (dE <= 0) flip at once, and the others' uphill
+//     level j = count (dE = 4 (D - j)) goes to LDS as D bit masks;
+//  B  one lane per Philox block of 4 positions (block p >> 2, word p & 3: the
+//     word every sweep draws for position p): an uphill site flips when its
+//     word is below the threshold of its level.
+// Flips are XORs into the state words (windows and blocks share words).
+// ---------------------------------------------------------------------------
+// 32 bits of S from bit p.  A field whose bits are used starts within 31
+// positions of the state (its used bits lie inside it), so S has a zero word
+// before it and two after; a field none of whose bits is used may start
+// anywhere, and p is clamped into [-32, n] for it.
+__device__ __forceinline__ uint32_t lsw_bits(const uint32_t* S, int p, int n) {
+    p = min(max(p, -32), n);
+    const int w0 = p >> 5;  // -1 for p in [-32, -1]
+    return (uint32_t)((((uint64_t)S[w0 + 1] << 32) | S[w0]) >> (p & 31));
+}
+
+__device__ __forceinline__ void lsw_xor(uint32_t* S, int p, uint32_t v) {
+    if (!v) return;
+    const uint64_t x = (uint64_t)v << (p & 31);
+    const int w0 = p >> 5;
+    if ((uint32_t)x) atomicXor(S + w0, (uint32_t)x);
+    if ((uint32_t)(x >> 32)) atomicXor(S + w0 + 1, (uint32_t)(x >> 32));
+}
+
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c) { return (a & b) | (c & (a | b)); }
+
+// LDS words per wave: the state (a zero word, nw words, two zero words) and
+// D level masks of one class, each with a zero word before and after.
+__host__ __device__ inline int lsw_state_words(int nw) { return (nw + 3 + 3) & ~3; }
+__host__ __device__ inline int lsw_zwords(int n) { return ((n / 2 + 31) >> 5) + 2; }
+__host__ __device__ inline int lsw_lds_bytes(int nw, int n, int D) {
+    return 4 * (lsw_state_words(nw) + D * lsw_zwords(n));
+}
+
+// Phase A of class c (class starts cs, other class cn): tab holds per
+// window 16 words, the edge signs (+x, -x, +y, -y, +z, -z, 0, 0) and the
+// shape masks (par, k = 0, k = H-1, y = 0, y = L-1, z = 0, z = L-1, 0).
+template <int D>
+__device__ __forceinline__ void lsw_phase_a(const uint32_t* __restrict__ tab, int L, int n, int c, int cs, int cn,
+                                            uint32_t* S, uint32_t* Z, int zw, int lane, int& dsum, int& nacc) {
+    const int H = L >> 1, hc = n >> 1, nwin = (hc + 31) >> 5;
+    const int dYw = (L - 1) * H, dZ = L * H, dZw = (L - 1) * L * H;
+    for (int w = lane; w < nwin; w += kWave) {
+        const int q0 = 32 * w;
+        const uint32_t vmask = hc - q0 >= 32 ? ~0u : (1u << (hc - q0)) - 1u;
+        const uint4* g = reinterpret_cast<const uint4*>(tab + ((size_t)c * nwin + w) * kLswGroup);
+        const uint4 s0 = g[0], s1 = g[1], m0 = g[2], m1 = g[3];
+        const int pn = cn + q0;
+        const uint32_t own = lsw_bits(S, cs + q0, n);
+        const uint32_t f0 = lsw_bits(S, pn, n);
+        const uint32_t par = m0.x, k0 = m0.y, kh = m0.z;
+        uint32_t N[2 * D];
+        N[0] = (par & ((kh & lsw_bits(S, pn - (H - 1), n)) | (~kh & lsw_bits(S, pn + 1, n)))) | (~par & f0);
+        N[1] = (par & f0) | (~par & ((k0 & lsw_bits(S, pn + (H - 1), n)) | (~k0 & lsw_bits(S, pn - 1, n))));
+        N[2] = (m1.x & lsw_bits(S, pn - dYw, n)) | (~m1.x & lsw_bits(S, pn + H, n));
+        N[3] = (m0.w & lsw_bits(S, pn + dYw, n)) | (~m0.w & lsw_bits(S, pn - H, n));
+        if (D == 3) {
+            N[4] = (m1.z & lsw_bits(S, pn - dZw, n)) | (~m1.z & lsw_bits(S, pn + dZ, n));
+            N[5] = (m1.y & lsw_bits(S, pn + dZw, n)) | (~m1.y & lsw_bits(S, pn - dZ, n));
+        }
+        const uint32_t sg[6] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y};
+        uint32_t B[2 * D];
+#pragma unroll
+        for (int j = 0; j < 2 * D; ++j) B[j] = ~(own ^ N[j] ^ sg[j]);
+        // count(B) = c0 + 2 c1 + 4 c2
+        uint32_t c0, c1, c2;
+        {
+            const uint32_t s1a = B[0] ^ B[1] ^ B[2], k1 = maj3(B[0], B[1], B[2]);
+            if (D == 3) {
+                const uint32_t s2a = B[3] ^ B[4] ^ B[5], k2 = maj3(B[3], B[4], B[5]);
+                c0 = s1a ^ s2a;
+                const uint32_t k3 = s1a & s2a;
+                c1 = k1 ^ k2 ^ k3;
+                c2 = maj3(k1, k2, k3);
+            } else {
+                c0 = s1a ^ B[3];
+                const uint32_t k2 = s1a & B[3];
+                c1 = k1 ^ k2;
+                c2 = k1 & k2;
+            }
+        }
+        const uint32_t dn = (D == 3 ? (c2 | (c1 & c0)) : (c1 | c2)) & vmask;  // count >= D: dE <= 0
+        const int na = __popc(dn);
+        nacc += na;
+        dsum += 4 * D * na - 4 * (__popc(dn & c0) + 2 * __popc(dn & c1) + 4 * __popc(dn & c2));
+        const uint32_t nz = ~(c1 | c2);
+        Z[1 + w] = nz & ~c0 & vmask;       // count 0
+        Z[zw + 1 + w] = nz & c0 & vmask;   // count 1
+        if (D == 3) Z[2 * zw + 1 + w] = c1 & ~c2 & ~c0 & vmask;  // count 2
+        lsw_xor(S, cs + q0, dn);
+    }
+}
+
+// Phase B of class c: lane per Philox block g of the class's positions.
+template <int D>
+__device__ __forceinline__ void lsw_phase_b(int n, int cs, const uint32_t* Z, int zw, const uint32_t* T, uint32_t c3,
+                                            uint32_t k0, uint32_t k1, int r, uint32_t gsweep, uint32_t* S, int lane,
+                                            int& dsum, int& nacc) {
+    const int hc = n >> 1;
+    const int gs = cs >> 2, ge = (cs + hc + 3) >> 2;
+    for (int g = gs + lane; g < ge; g += kWave) {
+        // level bits of the block's positions (class-relative bit 32 + q of the
+        // padded masks; the pads are zero, so positions outside the class have none)
+        const int b = 32 + 4 * g - cs, bw = b >> 5, bs = b & 31;
+        uint32_t z[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const uint32_t* zj = Z + j * zw;
+            z[j] = (uint32_t)((((uint64_t)zj[bw + 1] << 32) | zj[bw]) >> bs) & 0xFu;
+        }
+        const Philox4 x = philox4x32_10((uint32_t)g, (uint32_t)r, gsweep, c3, k0, k1);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            uint32_t cj = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) cj |= (x.v[i] < T[j] ? 1u : 0u) << i;
+            acc |= cj & z[j];
+        }
+        if (acc) {
+            const int na = __popc(acc);
+            nacc += na;
+            dsum += 4 * D * na - 4 * __popc(acc & z[1]) - (D == 3 ? 8 * __popc(acc & z[D - 1]) : 0);
+            atomicXor(S + (g >> 3), acc << (4 * (g & 7)));
+        }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void sweep_lattice(const uint32_t* __restrict__ tab, int L, int n,
+                                              const int32_t* __restrict__ cls, int nw, uint32_t c3, uint64_t seed,
+                                              int r, uint32_t gsweep, uint32_t* S, uint32_t* Z,
+                                              const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
+                                              int lane) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const int zw = lsw_zwords(n);
+    // thresholds of the uphill levels: count j gives dE = 4 (D - j), entry 2 (D - j) - 1
+    uint32_t T[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) T[j] = thr_row[2 * (D - j) - 1];
+    for (int c = 0; c < 2; ++c) {
+        int dsum = 0, nacc = 0;
+        lsw_phase_a<D>(tab, L, n, c, cls[c], cls[1 - c], S, Z, zw, lane, dsum, nacc);
+        wave_sync();
+        lsw_phase_b<D>(n, cls[c], Z, zw, T, c3, k0, k1, r, gsweep, S, lane, dsum, nacc);
+        wave_sync();
+        dsum = wave_sum_i32(dsum);
+        nacc = wave_sum_i32(nacc);
+        st.E += dsum;
+        st.flips += (unsigned long long)nacc;
+        if (nacc > 0 && st.E < st.bestE) {
+            st.bestE = st.E;
+            st.best_sweep = (int32_t)gsweep;
+            for (int w = lane; w < nw; w += kWave) best_words[w] = S[w];
         }
     }
 }

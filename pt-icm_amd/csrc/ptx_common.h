@@ -249,6 +249,17 @@ struct InstDesc {
     int64_t lat_off;     // -> lat_o2p[n] (index -> position) and lat_obit[n] (position -> word<<5|bit)
     int64_t jn_off;      // -> edge signs [lat_dim][rows] (bit x of row r: the coupling of the site to its +axis
                          //    neighbour is negative) for cut energies on bit sets, or -1
+    // Multispin sweep of the split kernels: 2 or 3 for a periodic L^D lattice
+    // with even L, couplings +-1 and no field, whose two colour classes are
+    // the site parities with positions in index order (class c, row r =
+    // y + L z, site x: position class_start[c] + r L/2 + x/2); else 0.
+    int32_t lsw_dim;
+    int32_t lsw_L;
+    int64_t lsw_off;     // -> per class and window of 32 class positions, kLswGroup words: the edge
+                         //    signs to the +x, -x, +y, -y, +z, -z neighbours (bit = coupling < 0), 0, 0,
+                         //    and the shape masks par, k = 0, k = L/2 - 1, y = 0, y = L-1, z = 0, z = L-1, 0
 };
+
+constexpr int kLswGroup = 16;  // words per (class, window) of the multispin sweep's table
 
 }  // namespace ptx

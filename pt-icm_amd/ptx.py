@@ -54,7 +54,8 @@ class EngineInfo(C.Structure):
 
 class InstanceInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("nt", C.c_int32), ("nrep", C.c_int32), ("nc", C.c_int32),
-                ("n_colours", C.c_int32), ("hist_bins", C.c_int32), ("e_base", C.c_int64)]
+                ("n_colours", C.c_int32), ("hist_bins", C.c_int32), ("e_base", C.c_int64),
+                ("lattice_sweep", C.c_int32)]
 
 
 class SlotObservables(C.Structure):

@@ -201,6 +201,67 @@ def test_lattice_cluster_growth_parity(ptx, oracle, dim, L, mode):
     compare(eng, oe, what=f"L={L} dim={dim} bulk")
 
 
+@pytest.mark.parametrize("dim,L,pm", [(2, 4, True), (2, 6, True), (2, 10, False), (2, 32, True), (2, 64, True),
+                                      (2, 66, True), (2, 100, False), (2, 128, True), (3, 4, True), (3, 6, True),
+                                      (3, 10, False), (3, 16, True)])
+@pytest.mark.parametrize("icm", [False, True])
+def test_multispin_lattice_sweep_parity(ptx, oracle, dim, L, pm, icm):
+    """Split-mode sweeps of even periodic lattices with couplings +-1 run the
+    multispin sweep on the state bits (rows of one or two words: L <= 64 and
+    L = 66 .. 128; L = 4; class boundaries inside a Philox block: 2D L = 6,
+    10), bit for bit with the oracle's site-by-site sweep; ferromagnets (pm
+    False, w = -1) and +-J couplings; two sweeps per step for the last part."""
+    if dim == 2:
+        n, u, v, w = instances.torus_2d(L, seed=L, pm=pm)
+    else:
+        n, u, v, w = instances.ea_3d(L, seed=L)
+        if not pm:
+            w = -np.ones_like(w)
+    p, g = ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)
+    nt = 8
+    betas = oracle.beta_linspace(0.3, 3.0, nt, 0)
+    say("multispin lattice", dim, L, "pm", pm, "icm", icm)
+    flags = ptx.FLAG_ICM if icm else 0
+    eng = ptx.Engine(p, betas, seed=3 * L, flags=flags, mode=1, waves_per_group=1)
+    assert eng.instance_info().lattice_sweep == dim
+    oe = oracle.Engine(g, betas, chains=2, seed=3 * L, flags=(oracle.FLAG_ICM if icm else 0))
+    for s_ in range(4):
+        eng.step(1)
+        oe.step(1)
+        compare(eng, oe, what=f"L={L} dim={dim} step {s_}")
+    eng.step(20)
+    oe.step(20)
+    compare(eng, oe, what=f"L={L} dim={dim} bulk")
+    eng2 = ptx.Engine(p, betas, seed=L, flags=flags, mode=1, waves_per_group=2, sweeps=2)
+    oe2 = oracle.Engine(g, betas, chains=2, seed=L, sweeps=2, flags=(oracle.FLAG_ICM if icm else 0))
+    eng2.step(6)
+    oe2.step(6)
+    compare(eng2, oe2, what=f"L={L} dim={dim} two sweeps")
+
+
+def test_multispin_lattice_sweep_only_where_it_applies(ptx, oracle):
+    """Odd L (three colour classes), a field, or couplings other than +-1 keep
+    the gather sweep; the engine reports which instances take which."""
+    cases = []
+    n, u, v, w = instances.torus_2d(9, seed=1)
+    cases.append(("odd L", (n, u, v, w, None), 0))
+    n, u, v, w = instances.torus_2d(8, seed=1)
+    cases.append(("field", (n, u, v, w, np.where(np.arange(n) % 7 == 0, 1.0, 0.0)), 0))
+    cases.append(("weights 2", (n, u, v, 2 * w, None), 0))
+    cases.append(("even L", (n, u, v, w, None), 2))
+    probs = [ptx.Problem.from_coo(*c[1]) for c in cases]
+    graphs = [oracle.Graph.from_coo(*c[1]) for c in cases]
+    betas = oracle.beta_linspace(0.4, 2.5, 6, 0)
+    eng = ptx.Engine(probs, [betas] * len(probs), seed=9, mode=1)
+    for i, (what, _, want) in enumerate(cases):
+        assert eng.instance_info(i).lattice_sweep == want, what
+    eng.step(8)
+    for i, g in enumerate(graphs):
+        oe = oracle.Engine(g, betas, chains=2, seed=9, instance=i)
+        oe.step(8)
+        compare(eng, oe, inst=i, what=cases[i][0])
+
+
 @pytest.mark.parametrize("kind,nc", [("G039", 8), ("pm_torus20", -1), ("field_random", 4)])
 @pytest.mark.parametrize("icm_waves", [1, 2, 4])
 def test_icm_waves_independence(ptx, oracle, kind, nc, icm_waves):
