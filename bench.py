@@ -29,9 +29,10 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # gfx950: 256 CUs x 128 B/clk x 2.4 GHz (GB/s)
 # HBM bytes per launch of the batched workload's dominant kernel, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of `microbench.py batched`,
 # scaled by the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# k_sweep of the split kernels (one wave per replica), and k_fused with 16
-# waves and 16 steps per launch.
-BATCHED_HBM_BYTES = {"k_sweep": (2 * 71276.8 + 38424.8) * 1024,
+# k_sweep of the split kernels (one wave per replica; the multispin lattice
+# sweep, after 2000 warm-up steps), and k_fused with 16 waves and 16 steps
+# per launch.
+BATCHED_HBM_BYTES = {"k_sweep": (2 * 40010.0 + 33392.0) * 1024,
                      "k_fused": (2 * 69305.5 + 475067.6) * 1024}
 # HBM bytes of one k_coop launch on G022 (22 slots x 2 chains, 8 waves) as a
 # function of the PT steps it runs: a per-launch part (staging the adjacency,
@@ -40,9 +41,9 @@ BATCHED_HBM_BYTES = {"k_sweep": (2 * 71276.8 + 38424.8) * 1024,
 # XCDs' L2s).  From rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # `microbench.py coop --launches 2 --short 20` (launches of 500 and of 20
 # steps), scaled by the k_copy16 calibration (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# 500 steps 7,800 KiB, 20 steps 1,607 KiB.
-COOP_HBM_KIB_PER_LAUNCH = 1349.0
-COOP_HBM_KIB_PER_STEP = 12.90
+# 500 steps 4,793 KiB, 20 steps 1,585 KiB.
+COOP_HBM_KIB_PER_LAUNCH = 1451.0
+COOP_HBM_KIB_PER_STEP = 6.68
 
 
 def load_ptx():

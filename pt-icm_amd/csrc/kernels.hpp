@@ -446,15 +446,17 @@ __host__ __device__ inline int spin_bytes(int nw) { return 32 * (nw + 1); }
 // workgroup's staging costs about one memory round trip per 8 elements a
 // thread copies instead of one per element (a load feeding the LDS store
 // of the same iteration serialises the loop on memory latency).
+// (Every load is unconditional, from an index clamped into the source, so the
+// staging registers stay registers; only the stores are conditional.)
 template <typename T>
 __device__ __forceinline__ void stage_to_lds(T* __restrict__ dst, const T* __restrict__ src, int count, int tid,
                                              int nthr) {
     constexpr int U = 8;
+    if (count <= 0) return;
     for (int q0 = tid; q0 < count; q0 += U * nthr) {
         T v[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k)
-            if (q0 + k * nthr < count) v[k] = src[q0 + k * nthr];
+        for (int k = 0; k < U; ++k) v[k] = src[min(q0 + k * nthr, count - 1)];
 #pragma unroll
         for (int k = 0; k < U; ++k)
             if (q0 + k * nthr < count) dst[q0 + k * nthr] = v[k];
