@@ -29,6 +29,8 @@ struct CoopArgs {
     int32_t pub_reps;           // replicas of all instances (the stride of pub_E's halves)
     int32_t pre_rnd;            // 1: the LDS layout holds the next sweep's random words
     int32_t lat_stage;          // 1: the lattice tables are copied into LDS
+    int32_t pre_f0;             // 1: the LDS layout holds the next sweep's class-0 local fields (drawn
+                                // during the step wait; kernels without cluster moves)
 };
 
 // Metropolis sweep of one replica by a whole workgroup (W waves).  Lane gl of
@@ -50,12 +52,21 @@ __device__ __forceinline__ void fill_sweep_randoms(uint32_t* pre, int n, uint32_
     }
 }
 
+// Local fields (with the field h) of the positions [cs, ce) of one class for
+// the next sweep, by threads t0, t0 + stride, ...
+template <int Fmt>
+__device__ __forceinline__ void class_fields(const Csr<Fmt>& csr, int cs, int ce, const int32_t* __restrict__ hf,
+                                             bool has_field, const uint8_t* sp, int32_t* f0, int t0, int stride) {
+    for (int p = cs + t0; p < ce; p += stride) f0[p - cs] = csr.field(p, sp) + (has_field ? hf[p] : 0);
+}
+
 template <int Fmt, int SPL>
 __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int32_t* __restrict__ cls, int ncls,
                                                  const int32_t* __restrict__ hf, bool has_field, int nw, uint32_t c3,
                                                  uint64_t seed, int r, uint32_t gsweep, uint8_t* sp,
                                                  const uint32_t* thr_row, uint32_t* best_words, SweepState& st,
-                                                 int gl, int nthr, int* acc, uint32_t& cc, const uint32_t* pre) {
+                                                 int gl, int nthr, int* acc, uint32_t& cc, const uint32_t* pre,
+                                                 const int32_t* f0) {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const int lane = gl % kWave;
     const int nwv = nthr / kWave, wv = gl / kWave;
@@ -65,7 +76,26 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
         // after its class's barrier and rewritten only after the next one.
         const int cs = cls[c], ce = cls[c + 1];
         int dsum = 0, nacc = 0;
-        if (SPL == 1) {
+        if (c == 0 && f0) {
+            // local fields drawn during the previous step's wait: decisions only
+            for (int p = cs + gl; p < ce; p += nthr) {
+                const uint32_t b = sp[p];
+                const int f = f0[p - cs];
+                const int dE = b ? -2 * f : 2 * f;
+                bool accept = dE <= 0;
+                if (!accept) {
+                    const uint32_t u =
+                        pre ? pre[p]
+                            : pick(philox4x32_10((uint32_t)p >> 2, (uint32_t)r, gsweep, c3, k0, k1), (uint32_t)p);
+                    accept = u < thr_row[(dE >> 1) - 1];
+                }
+                if (accept) {
+                    sp[p] = b ^ 1u;
+                    dsum += dE;
+                }
+                nacc += (int)__popcll(__ballot(accept));
+            }
+        } else if (SPL == 1) {
             // T lanes per position when the class has at most nthr / T of
             // them (T = 2, 4): each lane gathers every T-th chunk of the row
             // and a quad permutation sums the parts, so short classes use
@@ -119,7 +149,7 @@ __device__ __forceinline__ void sweep_replica_wg(const Csr<Fmt>& csr, const int3
                 group_update(csr, g, cs, ce, hf, has_field, c3, k0, k1, r, gsweep, sp, thr_row, dsum, nacc, pre);
         }
         dsum = wave_sum_i32(dsum);
-        if (SPL != 1) nacc = wave_sum_i32(nacc);
+        if (SPL != 1 && !(c == 0 && f0)) nacc = wave_sum_i32(nacc);
         int* a = acc + 2 * kMaxCoopWaves * (int)(cc & 1u);
         if (lane == 0) {
             a[2 * wv] = dsum;
@@ -215,16 +245,18 @@ __device__ __forceinline__ bool coop_wait(const unsigned long long* slots, int n
 
 struct CoopLayout {
     int info, adj, cls, thr, beta, E, s2r, r2s, att, succ, logu, key, perm, tpos, level, pre, sp, icm, lat, red,
-        lat_tab, acc, flag, lua, kbuf, rnd, total;
+        lat_tab, acc, flag, lua, kbuf, rnd, f0, total;
 };
 
 // pre_rnd = 1 adds the random words of one sweep (4 bytes per position);
 // lat_words > 0 two lattice-order bit sets for cluster growth, and with
 // lat_stage = 1 also a copy of the lattice tables (wpos, obit); csr_global
 // = 1 leaves the adjacency in global memory (read through L2), so that more
-// workgroups fit a CU.
+// workgroups fit a CU; f0_len > 0 adds the next sweep's local fields of
+// colour class 0 (f0_len positions, 4 bytes each).
 __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int ncls, int nrep, int nt, int fmax,
-                                                  int pre_rnd, int lat_words, int lat_stage, int csr_global) {
+                                                  int pre_rnd, int lat_words, int lat_stage, int csr_global,
+                                                  int f0_len = 0) {
     CoopLayout L;
     int o = 0;
     auto take = [&](int bytes) {
@@ -259,6 +291,7 @@ __host__ __device__ inline CoopLayout coop_layout(int n, int adj_len16, int ncls
     L.lua = take(2 * kWave * 8);  // log-uniforms of the exchange attempts, by step parity
     L.kbuf = take(2 * kWave * 4);
     L.rnd = take(pre_rnd ? 16 * ((n + 3) / 4) : 0);
+    L.f0 = take(4 * f0_len);
     L.total = o;
     return L;
 }
@@ -276,8 +309,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     const InstDesc d = a.desc[inst];
     const int n = d.n, nw = d.nw, nt = d.nt, nrep = d.nrep;
     const int r = item.y;  // this workgroup's replica of its instance
-    const CoopLayout L =
-        coop_layout(n, (int)d.adj_len16, d.ncls, nrep, nt, d.fmax, c.pre_rnd, d.lat_words, c.lat_stage, G);
+    const CoopLayout L = coop_layout(n, (int)d.adj_len16, d.ncls, nrep, nt, d.fmax, c.pre_rnd, d.lat_words,
+                                     c.lat_stage, G, c.pre_f0 ? d.cls0 : 0);
     uint32_t* info = reinterpret_cast<uint32_t*>(smem + L.info);
     uint32_t* thr = reinterpret_cast<uint32_t*>(smem + L.thr);
     double* beta = reinterpret_cast<double*>(smem + L.beta);
@@ -373,6 +406,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
     // random words of the first sweep of this launch (later ones are drawn
     // while the step's wait is pending)
     uint32_t* pre_rnd = c.pre_rnd ? reinterpret_cast<uint32_t*>(smem + L.rnd) : nullptr;
+    int32_t* f0buf = !IC && c.pre_f0 ? reinterpret_cast<int32_t*>(smem + L.f0) : nullptr;
     const uint32_t c3s0 = ctr3(kTagSweep | a.tag_base, d.inst_id);
     if (pre_rnd) fill_sweep_randoms(pre_rnd, n, c3s0, a.seed, r, a.inst_step[inst] * (uint32_t)a.sweeps, tid, nthr);
     __syncthreads();
@@ -413,7 +447,8 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         for (int sw = 0; sw < a.sweeps; ++sw) {
             const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
             sweep_replica_wg<Fmt, SPL>(csr, cls, d.ncls, hf, d.has_field != 0, nw, c3s, a.seed, r, gsweep, sp,
-                                       thr_row, best_words, st, tid, nthr, acc, cc, sw == 0 ? pre_rnd : nullptr);
+                                       thr_row, best_words, st, tid, nthr, acc, cc, sw == 0 ? pre_rnd : nullptr,
+                                       sw == 0 && it > 0 ? f0buf : nullptr);
         }
         phase_done(0);
         // ---- publish and wait ----
@@ -438,6 +473,10 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
         if (pre_rnd && wave >= w0 && it + 1 < nsteps)
             fill_sweep_randoms(pre_rnd, n, c3s, a.seed, r, (step + 1u) * (uint32_t)a.sweeps, tid - w0 * kWave,
                                nthr - w0 * kWave);
+        // without cluster moves the state stays as this sweep left it until
+        // the next one: its class-0 local fields are drawn now
+        if (f0buf && wave >= w0 && it + 1 < nsteps)
+            class_fields(csr, cls[0], cls[1], hf, d.has_field != 0, sp, f0buf, tid - w0 * kWave, nthr - w0 * kWave);
         if (wave == 0) {
             // the prepared exchange's inputs that do not depend on this
             // step's energies are read before the wait; each lane polls the

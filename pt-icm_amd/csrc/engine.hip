@@ -271,6 +271,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.nt = nt;
         d.nrep = nt * cfg.chains;
         d.ncls = P.ncls;
+        d.cls0 = P.class_start[1] - P.class_start[0];
         d.fmax = P.fmax;
         d.has_field = P.has_field ? 1 : 0;
         d.neg = std::any_of(P.w.begin(), P.w.end(), [](int32_t w) { return w < 0; }) ? 1 : 0;
@@ -813,19 +814,24 @@ Status Engine::setup_coop(bool required) {
         CoopForm& cf = coop_form[g];
         cf = CoopForm{};
         if (!coopok || (g == 1 && (coop_W != 4 || spl != 1))) continue;
-        auto max_total = [&](int pre, int stage) {
+        auto max_total = [&](int pre, int stage, int f0) {
             int mx = 0;
             for (const InstDesc& d : desc)
-                mx = std::max(
-                    mx, coop_layout(d.n, (int)d.adj_len16, d.ncls, d.nrep, d.nt, d.fmax, pre, d.lat_words, stage, g)
-                            .total);
+                mx = std::max(mx, coop_layout(d.n, (int)d.adj_len16, d.ncls, d.nrep, d.nt, d.fmax, pre, d.lat_words,
+                                              stage, g, f0 ? d.cls0 : 0)
+                                      .total);
             return mx;
         };
+        // class-0 fields drawn during the step wait: kernels without cluster
+        // moves (a replica's state then does not change between sweeps)
+        cf.pre_f0 = coop_icm ? 0 : 1;
         cf.pre_rnd = 1;
         cf.lat_stage = 1;
-        if (max_total(1, 1) > lds_limit) cf.lat_stage = 0;
-        if (max_total(1, 0) > lds_limit) cf.pre_rnd = 0;
-        cf.lds = max_total(cf.pre_rnd, cf.lat_stage);
+        if (max_total(1, 1, cf.pre_f0) > lds_limit) cf.pre_f0 = 0;
+        if (max_total(1, 1, 0) > lds_limit) cf.lat_stage = 0;
+        if (max_total(1, 0, 0) > lds_limit) cf.pre_rnd = 0;
+        if (!cf.pre_rnd) cf.pre_f0 = 0;
+        cf.lds = max_total(cf.pre_rnd, cf.lat_stage, cf.pre_f0);
         if (cf.lds > lds_limit) continue;
         const void* f = coop_kernel(g);
         if (cf.lds > 64 * 1024)
@@ -863,6 +869,7 @@ Status Engine::launch_coop(int nsteps) {
     c.pub_reps = (int32_t)total_rep;
     c.pre_rnd = cf.pre_rnd;
     c.lat_stage = cf.lat_stage;
+    c.pre_f0 = cf.pre_f0;
     const dim3 grid((unsigned)coop_items.size()), block((unsigned)(coop_W * kWave));
     const unsigned lds = (unsigned)cf.lds;
     const void* f = coop_kernel(g);

@@ -114,6 +114,7 @@ struct Engine {
         int32_t lds = 0;         // dynamic LDS bytes per workgroup
         int32_t pre_rnd = 0;     // next sweep's random words drawn during the step wait
         int32_t lat_stage = 0;   // lattice tables copied into LDS
+        int32_t pre_f0 = 0;      // next sweep's class-0 local fields drawn during the step wait
         int64_t capacity = 0;    // co-resident workgroups (0: form not available)
     };
     CoopForm coop_form[2];

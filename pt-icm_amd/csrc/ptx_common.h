@@ -223,6 +223,7 @@ struct InstDesc {
     int32_t nt;        // temperatures per chain
     int32_t nrep;      // chains * nt (= slots)
     int32_t ncls;      // colour classes
+    int32_t cls0;      // positions of colour class 0
     int32_t fmax;      // threshold-table width (max |dE|/2)
     int32_t nc;        // ICM slots
     int32_t has_field; // any h != 0
