@@ -1145,7 +1145,38 @@ struct LatView {
     const uint32_t* obit;
     int dim, L, R, words;
     const uint32_t* jn = nullptr;  // edge signs [dim][rows] (rows of one word), or none
+    int half = 0;  // n / 2 when the colour classes are the site parities in index order (InstDesc::lsw_dim):
+                   // a lattice word is then the bit interleave of two 16-bit class fields
 };
+
+// 16 bits to the even bits of a word, and back
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+    x &= 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+__device__ __forceinline__ uint32_t compress16(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    return (x | (x >> 8)) & 0x0000FFFFu;
+}
+// 32 bits of a position-order bit set from bit p (0 <= p < n; the set has a
+// word past its last)
+__device__ __forceinline__ uint32_t bits_at(const uint32_t* S, int p) {
+    const int w0 = p >> 5;
+    return (uint32_t)((((uint64_t)S[w0 + 1] << 32) | S[w0]) >> (p & 31));
+}
+__device__ __forceinline__ void bits_or(uint32_t* S, int p, uint32_t v) {
+    if (!v) return;
+    const uint64_t x = (uint64_t)v << (p & 31);
+    const int w0 = p >> 5;
+    if ((uint32_t)x) atomicOr(S + w0, (uint32_t)x);
+    if ((uint32_t)(x >> 32)) atomicOr(S + w0 + 1, (uint32_t)(x >> 32));
+}
 
 // lattice words per lane kept in registers during cluster growth
 constexpr int kLatSlots = 4;
@@ -1154,7 +1185,7 @@ __device__ __forceinline__ LatView lat_view(const DevArgs& a, const InstDesc& d)
     if (d.lat_dim == 0) return LatView{nullptr, nullptr, 0, 0, 0, 0};
     const uint32_t* t = a.lat + d.lat_off;  // wpos (16-bit) then obit
     return LatView{reinterpret_cast<const uint16_t*>(t), t + 16 * (int64_t)d.lat_words, d.lat_dim, d.lat_L, d.lat_R,
-                   d.lat_words, d.jn_off >= 0 ? a.jneg + d.jn_off : nullptr};
+                   d.lat_words, d.jn_off >= 0 ? a.jneg + d.jn_off : nullptr, d.lsw_dim > 0 ? d.n / 2 : 0};
 }
 
 // Grows the cluster of seed_pos (a position in D) as a bit set in lattice
@@ -1177,6 +1208,30 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
     // list is written.
     const bool cut = lv.jn != nullptr;
     uint32_t* S1o = reinterpret_cast<uint32_t*>(sc.Q);
+    // the row's class-c sites x = 2i + par_c, positions c * half + row * H + i
+    auto row_parity = [&](int row) {
+        const int y = row % L;
+        return lv.dim == 3 ? (y + row / L) & 1 : y & 1;
+    };
+    if (lv.half > 0) {
+        // D (and S1) in lattice order from the classes' 16-bit fields: word k
+        // of a row holds sites i in [16 k, 16 k + 16) of both classes,
+        // interleaved by the row's parity
+        const int H = L >> 1;
+        for (int w = lane; w < NW; w += kWave) {
+            const int row = w / R, k = w - row * R;
+            const int par0 = row_parity(row);
+            const int i0 = 16 * k, q = row * H + i0;
+            const uint32_t fm = (1u << min(16, H - i0)) - 1u;
+            const uint32_t e0 = spread16(bits_at(D, q) & fm), e1 = spread16(bits_at(D, lv.half + q) & fm);
+            Do[w] = par0 ? (e1 | (e0 << 1)) : (e0 | (e1 << 1));
+            Vo[w] = 0u;
+            if (cut) {
+                const uint32_t t0 = spread16(bits_at(sc.S1, q) & fm), t1 = spread16(bits_at(sc.S1, lv.half + q) & fm);
+                S1o[w] = par0 ? (t1 | (t0 << 1)) : (t0 | (t1 << 1));
+            }
+        }
+    } else
     // D in lattice order: the 32 positions of each word are read as four
     // 16-byte loads, then their D bits
     for (int w = lane; w < NW; w += kWave) {
@@ -1411,9 +1466,32 @@ __device__ __forceinline__ int lattice_grow(const LatView& lv, int n, int nw, in
             if (__ballot(grew) == 0ull) break;
         }
     }
+    int csize = 0;
+    if (lv.half > 0 && cut) {
+        // back to position order by the classes' fields (no position list
+        // follows the cut energy on bit sets)
+        for (int w = lane; w < nw; w += kWave) sc.V[w] = 0u;
+        wave_sync();
+        const int H = L >> 1;
+        int cs = 0;
+        for (int w = lane; w < NW; w += kWave) {
+            const uint32_t v = Vo[w];
+            cs += __popc(v);
+            if (v) {
+                const int row = w / R, k = w - row * R;
+                const int par0 = row_parity(row);
+                const uint32_t fe = compress16(v), fo = compress16(v >> 1);
+                const int q = row * H + 16 * k;
+                bits_or(sc.V, q, par0 ? fo : fe);
+                bits_or(sc.V, lv.half + q, par0 ? fe : fo);
+            }
+        }
+        csize = wave_sum_i32(cs);
+        wave_sync();
+        return csize;
+    }
     // back to position order: V, and the cluster's positions in Q (the
     // table reads of four 64-position blocks are issued together)
-    int csize = 0;
     for (int p00 = 0; p00 < 32 * nw; p00 += 4 * kWave) {
         uint32_t ob[4];
 #pragma unroll
