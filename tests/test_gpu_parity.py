@@ -293,6 +293,40 @@ def test_launch_shape_independence(ptx, oracle, mode, wpg, spl, lanes):
     compare(eng, oe, what=f"W={wpg}")
 
 
+@pytest.mark.parametrize("batch", ["unit", "wide"])
+@pytest.mark.parametrize("wpg", [2, 4, 8])
+def test_multiwave_gather_sweep_parity(ptx, oracle, batch, wpg):
+    """Split gather sweeps with several waves (replicas of one instance) per
+    workgroup: bit-exact with the oracle for rows of one chunk (an odd
+    torus), of several chunks (G022), with the sign table (+-J ER), in the
+    32-bit record format with a field, and with workgroups whose last waves
+    have no replica (2 nt not a multiple of the wave count)."""
+    if batch == "unit":
+        pairs = [make_pair(ptx, oracle, k) for k in ["G022", "er_small"]]
+        n, u, v, w = instances.torus_2d(9, seed=4)
+        pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    else:
+        pairs = [make_pair(ptx, oracle, k) for k in ["field_random", "G022"]]
+    nts = [5, 7, 3][: len(pairs)]
+    ladders = [oracle.beta_linspace(0.5, 3.0, nt, 0) for nt in nts]
+    say("multiwave gather", batch, "W", wpg)
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=21, flags=ptx.FLAG_ICM, mode=1, waves_per_group=wpg)
+    assert eng.info().waves_per_group == wpg
+    oes = [oracle.Engine(g, lad, chains=2, seed=21, instance=i, flags=oracle.FLAG_ICM)
+           for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
+    for s_ in range(3):
+        eng.step(1)
+        for oe in oes:
+            oe.step(1)
+        for i, oe in enumerate(oes):
+            compare(eng, oe, inst=i, what=f"{batch} W={wpg} inst {i} step {s_}")
+    eng.step(12)
+    for oe in oes:
+        oe.step(12)
+    for i, oe in enumerate(oes):
+        compare(eng, oe, inst=i, what=f"{batch} W={wpg} inst {i} bulk")
+
+
 @pytest.mark.parametrize("kind", ["G022", "field_random", "ea3d_6"])
 def test_cooperative_16_waves_parity(ptx, oracle, kind):
     """Cooperative kernel with 16 waves per replica (no ICM slot; 4 lanes
