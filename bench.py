@@ -220,10 +220,10 @@ def batched_ea(ptx, steps, seed, n_inst=4096):
 
 def ttt_summary(r):
     """The time-to-target result without per-stage detail."""
-    keep = ("instances", "start_ladders", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
+    keep = ("instances", "copies", "start_ladders", "hit_rate", "p50_ttt_s", "instances_per_hour", "surrogate_accepted",
             "done_in_validation", "fallback", "ladder_s", "run_s", "judge_at_s", "fallback_ready_s", "wall_s",
             "ms_per_step", "mean_slots", "mean_icm_slots", "mean_preprocess_s", "mode")
-    out = {k: r[k] for k in ("workload", "max_steps", "calibration", "icm_semantics") if k in r}
+    out = {k: r[k] for k in ("workload", "max_steps", "calibration", "icm_semantics", "batched_copies") if k in r}
     for k, v in r.items():
         if isinstance(v, dict):
             out[k] = {kk: vv for kk, vv in v.items() if kk in keep}
@@ -349,7 +349,8 @@ def main():
                                  ("time_to_target_c5_sizes", args.ttt_c5_targets, False)):
             if not tpath or not os.path.exists(tpath):
                 continue
-            r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=tpath, linear=full, judge_before=full)
+            r = ttt.run(ptx, n_plain=args.ttt_plain, log=log, path=tpath, linear=full, judge_before=full,
+                        copies=ttt.COPIES if full else 1)
             out[key] = ttt_summary(r)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(path, betas, args.cpu_seconds, args.seed)

@@ -292,11 +292,13 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         // (the wide format masks padding by weight 0, so there any in-range position will do)
         const uint32_t sentinel = sentinel_pos(d.nw);
         int64_t cur16 = 0;
-        // engines that can run the cooperative kernel with one position per
-        // lane order each row's records for its gathers (any order is exact)
+        // engines that run the cooperative kernel with one position per lane
+        // from the start order each row's records for its gathers (any order
+        // is exact; a batch that may switch to it after retirement does not:
+        // the ordering costs ~1.4 us per row of degree 20 on the host, 0.36 s
+        // for 256 ER graphs of 1,000 vertices)
         const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
-                               (cfg.mode == PTX_MODE_COOP || cfg.small_batch_coop ||
-                                (cfg.mode == PTX_MODE_AUTO && ninst == 1));
+                               (cfg.mode == PTX_MODE_COOP || (cfg.mode == PTX_MODE_AUTO && ninst == 1));
         BankOrder bank_order(kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4));
         int32_t cls_of = 0;
         for (int32_t p = 0; p < P.n; ++p) {

@@ -156,3 +156,33 @@ def test_small_batch_coop_switches_without_changing_results(ptx):
     for i in np.nonzero(mask)[0]:
         sp = eng.spins(i)
         assert np.array_equal(eng.energies(i), [probs[i].energy(x) for x in sp])
+
+
+def test_time_to_target_warm_start_maps_slots_by_nearest_beta(ptx):
+    """The time-to-target harness hands a fallback ladder the replicas'
+    current states: slot t of each chain of the new engine starts from the
+    state at the slot of the same chain with the nearest beta in the old one
+    (after the old engine has exchanged, so its permutation is not the
+    identity); energies are recomputed from the written states."""
+    import ttt
+    probs, _ = _batch(ptx, k=3, seed=5)
+    lad_a = np.sort(ptx.beta_linspace(0.3, 2.5, 10, True))
+    lad_b = np.sort(ptx.beta_linspace(0.35, 2.4, 7, True))
+    src = ptx.Engine(probs, [lad_a] * 3, seed=3, flags=ptx.FLAG_ICM, mode=1)
+    src.step(40)
+    dst = ptx.Engine(probs[::-1], [lad_b] * 3, seed=4, flags=ptx.FLAG_ICM, mode=1)  # entry 0 is problem 2
+    dst.step(3)
+    ttt.warm_start(src, 2, lad_a, dst, 0, lad_b)
+    sp_a, s2r_a, e_a = src.spins(2), src.slot2rep(2), src.energies(2)
+    sp_b, s2r_b, e_b = dst.spins(0), dst.slot2rep(0), dst.energies(0)
+    assert not np.array_equal(s2r_a, np.arange(len(s2r_a)))
+    for c in range(2):
+        for t in range(len(lad_b)):
+            s = int(np.argmin(np.abs(lad_a - lad_b[t])))
+            ra, rb = s2r_a[c * len(lad_a) + s], s2r_b[c * len(lad_b) + t]
+            assert np.array_equal(sp_b[rb], sp_a[ra]), (c, t)
+            assert e_b[rb] == e_a[ra], (c, t)
+    src.close()
+    dst.close()
+    for p in probs:
+        p.close()
