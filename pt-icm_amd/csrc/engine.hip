@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -235,6 +236,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     int64_t beta_off = 0, rep_off = 0, spin_off = 0, hist_off = 0;
     const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
     int64_t t_off = 0;
+    std::unordered_map<const Problem*, int32_t> first_copy;  // problem -> its first entry
     // record format: 16-bit when every instance has |w| = 1 and n <= 32768
     bool unit_ok = true;
     for (const auto& pp : problems) {
@@ -285,64 +287,81 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.nc = nc;
         if (nc > 0) coop_icm = true;
         d.inst_id = (uint32_t)i;
-        // adjacency rows by position, each padded to 16 bytes (at least one
-        // chunk); padding records name the sentinel position 32 * nw
-        d.pos_off = (int64_t)rowinfo.size();
-        d.adj_off16 = (int64_t)adjw.size() / 4;
-        // (the wide format masks padding by weight 0, so there any in-range position will do)
-        const uint32_t sentinel = sentinel_pos(d.nw);
-        int64_t cur16 = 0;
-        // engines that run the cooperative kernel with one position per lane
-        // from the start order each row's records for its gathers (any order
-        // is exact; a batch that may switch to it after retirement does not:
-        // the ordering costs ~1.4 us per row of degree 20 on the host, 0.36 s
-        // for 256 ER graphs of 1,000 vertices)
-        const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
-                               (cfg.mode == PTX_MODE_COOP || (cfg.mode == PTX_MODE_AUTO && ninst == 1));
-        BankOrder bank_order(kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4));
-        int32_t cls_of = 0;
-        for (int32_t p = 0; p < P.n; ++p) {
-            const int32_t o = P.pos2orig[p];
-            const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
-            if (cur16 >= (1ll << 23)) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
-            rowinfo.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
-            if (fmt == kFmtUnit) {
-                const int32_t chunks = std::max(1, (dg + 7) / 8);
-                std::vector<uint16_t> rec((size_t)chunks * 8, (uint16_t)sentinel);
-                for (int32_t j = 0; j < dg; ++j) {
-                    const int32_t q = P.rowptr[o] + j;
-                    rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << d.kbit : 0u));
-                }
-                if (coop_rows) {
-                    while (p >= P.class_start[cls_of + 1]) ++cls_of;
-                    const int32_t c0 = P.class_start[cls_of];
-                    bank_order.order(((int64_t)i << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
-                                     dg, (1u << d.kbit) - 1u);
-                }
-                for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
-                cur16 += chunks;
-            } else {
-                const int32_t chunks = std::max(1, (dg + 3) / 4);
-                for (int32_t j = 0; j < chunks * 4; ++j) {
-                    if (j < dg) {
+        // copies of one problem in the batch (entries holding the same
+        // Problem) share its read-only tables: rows, adjacency, class bounds,
+        // field and position order, lattice edge signs and sweep table, and the
+        // thresholds when the ladder is the same too
+        const auto seen = first_copy.find(&P);
+        const int32_t s_i = seen == first_copy.end() ? -1 : seen->second;
+        const InstDesc* s = s_i >= 0 ? &desc[(size_t)s_i] : nullptr;
+        if (!s) first_copy.emplace(&P, i);
+        if (s) {
+            d.pos_off = s->pos_off;
+            d.adj_off16 = s->adj_off16;
+            d.adj_len16 = s->adj_len16;
+            d.cls_off = s->cls_off;
+        } else {
+            // adjacency rows by position, each padded to 16 bytes (at least one
+            // chunk); padding records name the sentinel position 32 * nw
+            d.pos_off = (int64_t)rowinfo.size();
+            d.adj_off16 = (int64_t)adjw.size() / 4;
+            // (the wide format masks padding by weight 0, so there any in-range position will do)
+            const uint32_t sentinel = sentinel_pos(d.nw);
+            int64_t cur16 = 0;
+            // engines that run the cooperative kernel with one position per lane
+            // from the start order each row's records for its gathers (any order
+            // is exact; a batch that may switch to it after retirement does not:
+            // the ordering costs ~1.4 us per row of degree 20 on the host, 0.36 s
+            // for 256 ER graphs of 1,000 vertices)
+            const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
+                                   (cfg.mode == PTX_MODE_COOP || (cfg.mode == PTX_MODE_AUTO && ninst == 1));
+            BankOrder bank_order(kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4));
+            int32_t cls_of = 0;
+            for (int32_t p = 0; p < P.n; ++p) {
+                const int32_t o = P.pos2orig[p];
+                const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
+                if (cur16 >= (1ll << 23)) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
+                rowinfo.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
+                if (fmt == kFmtUnit) {
+                    const int32_t chunks = std::max(1, (dg + 7) / 8);
+                    std::vector<uint16_t> rec((size_t)chunks * 8, (uint16_t)sentinel);
+                    for (int32_t j = 0; j < dg; ++j) {
                         const int32_t q = P.rowptr[o] + j;
-                        adjw.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
-                    } else {
-                        adjw.push_back(sentinel);  // weight 0: contributes nothing
+                        rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << d.kbit : 0u));
                     }
+                    if (coop_rows) {
+                        while (p >= P.class_start[cls_of + 1]) ++cls_of;
+                        const int32_t c0 = P.class_start[cls_of];
+                        bank_order.order(((int64_t)i << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
+                                         dg, (1u << d.kbit) - 1u);
+                    }
+                    for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
+                    cur16 += chunks;
+                } else {
+                    const int32_t chunks = std::max(1, (dg + 3) / 4);
+                    for (int32_t j = 0; j < chunks * 4; ++j) {
+                        if (j < dg) {
+                            const int32_t q = P.rowptr[o] + j;
+                            adjw.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
+                        } else {
+                            adjw.push_back(sentinel);  // weight 0: contributes nothing
+                        }
+                    }
+                    cur16 += chunks;
                 }
-                cur16 += chunks;
+            }
+            d.adj_len16 = cur16;
+            d.cls_off = (int64_t)cls.size();
+            cls.insert(cls.end(), P.class_start.begin(), P.class_start.end());
+            for (int32_t p = 0; p < P.n; ++p) {
+                hv.push_back(P.h[P.pos2orig[p]]);
+                p2o.push_back(P.pos2orig[p]);
             }
         }
-        d.adj_len16 = cur16;
-        d.cls_off = (int64_t)cls.size();
-        cls.insert(cls.end(), P.class_start.begin(), P.class_start.end());
-        for (int32_t p = 0; p < P.n; ++p) {
-            hv.push_back(P.h[P.pos2orig[p]]);
-            p2o.push_back(P.pos2orig[p]);
-        }
         d.thr_off = (int64_t)thr.size();
-        if (P.fmax > 0) {
+        if (s && ladders[(size_t)s_i] == ladders[(size_t)i]) {
+            d.thr_off = s->thr_off;
+        } else if (P.fmax > 0) {
             std::vector<uint32_t> t((size_t)nt * P.fmax);
             threshold_table(ladders[i].data(), nt, P.fmax, t.data());
             thr.insert(thr.end(), t.begin(), t.end());
@@ -359,7 +378,14 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         d.lat_dim = d.lat_L = d.lat_R = d.lat_words = 0;
         d.lat_off = 0;
         d.jn_off = -1;
-        if (nc > 0 && P.lat_dim > 0) {
+        if (s && nc > 0 && s->nc > 0) {
+            d.lat_dim = s->lat_dim;
+            d.lat_L = s->lat_L;
+            d.lat_R = s->lat_R;
+            d.lat_words = s->lat_words;
+            d.lat_off = s->lat_off;
+            d.jn_off = s->jn_off;
+        } else if (nc > 0 && P.lat_dim > 0) {
             const int32_t L = P.lat_L, R = (L + 31) / 32;
             int32_t rows = 1;
             for (int32_t k = 1; k < P.lat_dim; ++k) rows *= L;
@@ -417,7 +443,11 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         }
         d.lsw_dim = d.lsw_L = 0;
         d.lsw_off = (int64_t)lsign.size();
-        if (lattice_sweep_table(P, lsign)) {
+        if (s) {
+            d.lsw_dim = s->lsw_dim;
+            d.lsw_L = s->lsw_L;
+            d.lsw_off = s->lsw_off;
+        } else if (lattice_sweep_table(P, lsign)) {
             d.lsw_dim = P.lat_dim;
             d.lsw_L = P.lat_L;
             lsw_bytes = std::max(lsw_bytes, lsw_lds_bytes(d.nw, d.n, d.lsw_dim));

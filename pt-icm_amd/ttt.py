@@ -50,7 +50,7 @@ import surrogate as S  # noqa: E402
 
 TARGETS = os.path.join(HERE, "data", "ttt_targets.json")
 T0, T1 = 0.5, 3.0
-COPIES = 2  # independent runs of every instance in the batched paths
+COPIES = 3  # independent runs of every instance in the batched paths
 
 
 def ttt_specs(n_inst, seed=0, L=32, n_er=1000, p_er=0.02):

@@ -186,3 +186,31 @@ def test_time_to_target_warm_start_maps_slots_by_nearest_beta(ptx):
     dst.close()
     for p in probs:
         p.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_copies_of_one_problem_share_tables_without_changing_results(ptx, mode):
+    """Entries of a batch that hold the same Problem share its read-only
+    tables (rows, adjacency, classes, thresholds for equal ladders, lattice
+    tables): every entry equals the same batch built from separate Problem
+    objects, bit for bit (a 2D +-J torus with lattice cluster moves and the
+    multispin sweep, an odd torus, a +-J ER graph; one copy on another
+    ladder)."""
+    coos = [instances.torus_2d(12, seed=1), instances.torus_2d(9, seed=2),
+            instances.erdos_renyi(300, 0.03, seed=3, pm=True)]
+    shared = [ptx.Problem.from_coo(*c) for c in coos]
+    lad = np.sort(ptx.beta_linspace(0.4, 2.8, 8, True))
+    lad2 = np.sort(ptx.beta_linspace(0.5, 2.5, 8, True))
+    order = [0, 1, 2, 0, 2, 1, 0]
+    lads = [lad] * 6 + [lad2]
+    separate = [ptx.Problem.from_coo(*coos[k]) for k in order]
+    say("copies mode", mode)
+    a = ptx.Engine([shared[k] for k in order], lads, seed=9, flags=ptx.FLAG_ICM, mode=mode)
+    b = ptx.Engine(separate, lads, seed=9, flags=ptx.FLAG_ICM, mode=mode)
+    a.step(30)
+    b.step(30)
+    _same(_state(a, len(order)), _state(b, len(order)))
+    # copies run their own random streams
+    assert not np.array_equal(a.spins(0), a.spins(3))
+    a.close()
+    b.close()
