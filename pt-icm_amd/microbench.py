@@ -60,8 +60,12 @@ def main():
         else:
             kw = {"mode": a.mode, "waves_per_group": a.waves}
         eng = ptx.Engine(probs, [lad] * len(probs), seed=0, flags=ptx.FLAG_ICM, **kw)
-        eng.step(a.warmup)
-        eng.sync()
+        tw = time.perf_counter()
+        for done in range(0, a.warmup, 250):  # progress on stderr (counter runs serialise every launch)
+            eng.step(min(250, a.warmup - done))
+            eng.sync()
+            print(f"warm-up {done + min(250, a.warmup - done)} steps, {time.perf_counter() - tw:.1f} s",
+                  file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         eng.step(a.steps)
         eng.sync()
