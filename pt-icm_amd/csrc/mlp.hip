@@ -3,8 +3,9 @@
 // One GEMM kernel serves the forward pass (X . W), the input gradient
 // (G . W^T) and the weight gradient (X^T . G): operands are addressed through
 // (row, k) strides, so the transposes are free.  Each wavefront computes a
-// 32 x 32 tile with v_mfma_f32_32x32x8_bf16 (FP32 accumulate); a workgroup of
-// four wavefronts covers 64 x 64.  Epilogues fuse bias + ReLU (+ the BF16 copy
+// 32 x 32 tile with v_mfma_f32_32x32x16_bf16 (FP32 accumulate; a last step of
+// v_mfma_f32_32x32x8_bf16 when K is 8 mod 16); a workgroup of four
+// wavefronts covers 64 x 64.  Epilogues fuse bias + ReLU (+ the BF16 copy
 // the next layer reads) and the ReLU mask of the backward pass.
 #include "mlp.hpp"
 
@@ -19,6 +20,8 @@ namespace ptx {
 namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 enum : int32_t { kEpiPlain = 0, kEpiBias = 1, kEpiBiasRelu = 2, kEpiReluMask = 3 };
@@ -57,19 +60,33 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(Gemm g) {
     const int i0 = blockIdx.y * 64 + (wave >> 1) * 32;
     const int j0 = blockIdx.x * 64 + (wave & 1) * 32;
     if (i0 >= g.M || j0 >= g.N) return;
-    const int r = lane & 31, kq = (lane >> 5) * 4;
-    // lane holds A(i0 + r, k0 + kq .. +3) and B(k0 + kq .. +3, j0 + r)
-    const uint16_t* ap = g.A + (int64_t)(i0 + r) * g.sai + (int64_t)kq * g.sak;
-    const uint16_t* bp = g.B + (int64_t)(j0 + r) * g.sbj + (int64_t)kq * g.sbk;
+    const int r = lane & 31, hi = lane >> 5;
     v16f acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.0f;
-    for (int k0 = 0; k0 < g.K; k0 += 8) {
+    // 16 k per instruction (v_mfma_f32_32x32x16_bf16): lane holds
+    // A(i0 + r, k0 + 8 hi .. +7) and B(k0 + 8 hi .. +7, j0 + r)
+    const uint16_t* ap = g.A + (int64_t)(i0 + r) * g.sai + (int64_t)(8 * hi) * g.sak;
+    const uint16_t* bp = g.B + (int64_t)(j0 + r) * g.sbj + (int64_t)(8 * hi) * g.sbk;
+    int k0 = 0;
+    for (; k0 + 16 <= g.K; k0 += 16) {
+        v8s a, b;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            a[t] = (short)ap[(int64_t)(k0 + t) * g.sak];
+            b[t] = (short)bp[(int64_t)(k0 + t) * g.sbk];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc,
+                                                      0, 0, 0);
+    }
+    if (k0 < g.K) {  // K = 8 mod 16: one step of 8 (v_mfma_f32_32x32x8_bf16: k0 + 4 hi .. +3 per lane)
+        const uint16_t* a8 = g.A + (int64_t)(i0 + r) * g.sai + (int64_t)(k0 + 4 * hi) * g.sak;
+        const uint16_t* b8 = g.B + (int64_t)(j0 + r) * g.sbj + (int64_t)(k0 + 4 * hi) * g.sbk;
         v4s a, b;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            a[t] = (short)ap[(int64_t)(k0 + t) * g.sak];
-            b[t] = (short)bp[(int64_t)(k0 + t) * g.sbk];
+            a[t] = (short)a8[(int64_t)t * g.sak];
+            b[t] = (short)b8[(int64_t)t * g.sbk];
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
