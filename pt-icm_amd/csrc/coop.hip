@@ -533,7 +533,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                     icm_team<Fmt, true>(csr, hf, d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                         pub_sp + par * c.pub_words + (int64_t)r1 * nw,
                                         pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, tid, nthr,
-                                        false);
+                                        false, LatView{}, true, E[r1], E[r2]);
                     st.E += (int32_t)(r == r1 ? dE1 : dE2);
                     expand_bits(r == r1 ? sc.S1 : sc.S2, nw, sp, tid, nthr);
                     __syncthreads();
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(W* kWave) void k_coop(DevArgs a, CoopArgs c, int ns
                                              d.has_field != 0, n, nw, t, step, a.seed, c3i,
                                              pub_sp + par * c.pub_words + (int64_t)r1 * nw,
                                              pub_sp + par * c.pub_words + (int64_t)r2 * nw, sc, dE1, dE2, lane, kWave,
-                                             false, lv);
+                                             false, lv, true, E[r1], E[r2]);
                         if (lane == 0) flag[0] = st.E + (int32_t)(r == r1 ? dE1 : dE2);
                     }
                     __syncthreads();

@@ -244,7 +244,8 @@ __global__ __launch_bounds__(W* kWave) void k_icm(DevArgs a, int nw_max, int lat
                              g2, sc, dE1, dE2, tid, kWave, true, lat_view(a, d));
     } else {
         icm_team<Fmt, (W > 1)>(csr, a.hfield + d.pos_off, d.has_field != 0, d.n, d.nw, t, step, a.seed, c3i,
-                               g1, g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{});
+                               g1, g2, sc, dE1, dE2, tid, W * kWave, true, W == 1 ? lat_view(a, d) : LatView{},
+                               true, a.E[ro + r1], a.E[ro + r2]);
     }
     if (tid == 0) {
         a.E[ro + r1] += (int32_t)dE1;

@@ -1620,8 +1620,11 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
                                          int nw, int t, uint32_t step, uint64_t seed, uint32_t c3,
                                          const uint32_t* g1_in, const uint32_t* g2_in, IcmScratch sc,
                                          long long& dE1_out, long long& dE2_out, int tid, int nthr,
-                                         bool write_back = true, LatView lv = LatView{nullptr, nullptr, 0, 0, 0, 0}) {
-    // the new states are left in sc.S1/sc.S2 and, with write_back, stored to g1/g2
+                                         bool write_back = true, LatView lv = LatView{nullptr, nullptr, 0, 0, 0, 0},
+                                         bool have_e = false, long long e1 = 0, long long e2 = 0) {
+    // the new states are left in sc.S1/sc.S2 and, with write_back, stored to g1/g2;
+    // have_e: e1, e2 are the two replicas' energies (queue growth may then
+    // finish bottom-up and take the energy change from the rest of D)
     uint32_t* g1 = const_cast<uint32_t*>(g1_in);
     uint32_t* g2 = const_cast<uint32_t*>(g2_in);
     uint32_t* S1 = sc.S1;
@@ -1724,7 +1727,15 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
             // keeps it in LDS (one atomic add per wave and chunk) and reads it
             // back between two barriers at the end of each round.
             int head = 0, tail = 1;
+            bool bottom_up = false;
+            const long long gflip = dE1;  // this thread's field terms of a global flip of replica 1
             while (head < tail) {
+                // once the queued sites are at least half of the D sites not
+                // yet claimed, the rest of the component is found bottom-up
+                if (have_e && 2 * (tail - head) >= cnt - tail) {  // uniform
+                    bottom_up = true;
+                    break;
+                }
                 const int tl = tail;
                 const int idx = head + tid;
                 const bool act = idx < tl;
@@ -1822,6 +1833,72 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
             }
             csize = tail;
             cut_done = true;  // boundary sums taken during the growth
+            if (bottom_up) {
+                // Bottom-up passes: every unclaimed site of D with a claimed
+                // neighbour is claimed (a row is read only up to its first
+                // claimed neighbour), until a pass claims none.  Claimed sites
+                // are in the seed's component of D and the component is closed
+                // under D-adjacency, so V ends as exactly that component, the
+                // same set the breadth-first growth finds.
+                int added;
+                do {
+                    int mine = 0;
+                    for (int p = tid; p < n; p += nthr) {
+                        const uint32_t bit = 1u << (p & 31);
+                        if ((D[p >> 5] & bit) == 0u || (V[p >> 5] & bit) != 0u) continue;
+                        const uint8_t* row = csr.row(p);
+                        const int dg = csr.deg(p);
+                        bool hit = false;
+                        for (int ch = 0; K * ch < dg && !hit; ++ch) {
+                            int col[K], wgt[K];
+                            csr.chunk(row, ch, col, wgt);
+                            const int live = min(K, dg - ch * K);
+#pragma unroll
+                            for (int h = 0; h < K; ++h)
+                                hit = hit || (h < live && ((V[col[h] >> 5] >> (col[h] & 31)) & 1u) != 0u);
+                        }
+                        if (hit) {
+                            atomicOr(&V[p >> 5], bit);
+                            ++mine;
+                        }
+                    }
+                    team_sync<Block>();  // this pass's claims are visible to the next
+                    added = team_sum_i32<Block>(mine, sc.red, tid, nthr);
+                } while (added > 0);
+                // Energy change from the rest R = D \ C.  Flipping C in
+                // replica 1 (after its global flip, if any) gives replica 2's
+                // state with R flipped (on C s1 becomes s2, on R s1 = -s2,
+                // elsewhere s1 = s2), so dE1 = e2 - e1 + (change of flipping R
+                // in replica 2), global flip included.  Replica 2 ends as
+                // replica 1 after its global flip with R flipped, and R is a
+                // union of components of D, so dE2 = -dE1 + (the global
+                // flip's change).  The boundary sums taken for the sites
+                // grown top-down are dropped.
+                dE1 = tid == 0 ? e2 - e1 : 0;
+                for (int p = tid; p < n; p += nthr) {
+                    const uint32_t bit = 1u << (p & 31);
+                    if ((D[p >> 5] & bit) == 0u || (V[p >> 5] & bit) != 0u) continue;  // p in R
+                    const int s2 = ((S2[p >> 5] & bit) != 0u) ? 1 : -1;
+                    const uint8_t* row = csr.row(p);
+                    const int dg = csr.deg(p);
+                    int b2 = 0;  // sum over neighbours j outside R of J s2_j
+                    for (int ch = 0; K * ch < dg; ++ch) {
+                        int col[K], wgt[K];
+                        csr.chunk(row, ch, col, wgt);
+                        const int live = min(K, dg - ch * K);
+#pragma unroll
+                        for (int h = 0; h < K; ++h) {
+                            const int j = col[h];
+                            const uint32_t sh = (uint32_t)(j & 31);
+                            const bool in_r = ((D[j >> 5] >> sh) & 1u) != 0u && ((V[j >> 5] >> sh) & 1u) == 0u;
+                            const int a2 = ((S2[j >> 5] >> sh) & 1u) ? wgt[h] : -wgt[h];
+                            b2 += (h < live && !in_r) ? a2 : 0;
+                        }
+                    }
+                    dE1 += -2ll * s2 * (b2 + (has_field ? hf[p] : 0));
+                }
+                dE2 = gflip - dE1;
+            }
         }
         // energy change of flipping the cluster C in replica 1: boundary
         // edges (p in C, j not in C) and the field terms of C.  Replica 2's
