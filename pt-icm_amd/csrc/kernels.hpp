@@ -1730,9 +1730,11 @@ __device__ __forceinline__ void icm_team(const Csr<Fmt>& csr, const int32_t* hf,
             bool bottom_up = false;
             const long long gflip = dE1;  // this thread's field terms of a global flip of replica 1
             while (head < tail) {
-                // once the queued sites are at least half of the D sites not
-                // yet claimed, the rest of the component is found bottom-up
-                if (have_e && 2 * (tail - head) >= cnt - tail) {  // uniform
+                // once the queued sites are at least twice the D sites not yet
+                // claimed, the rest of the component is found bottom-up (on
+                // 512 ER G(5000, 0.004): 0.73 ms of cluster moves per step,
+                // against 0.79 ms switching at half and 0.90 ms at an eighth)
+                if (have_e && (tail - head) >= 2 * (cnt - tail)) {  // uniform
                     bottom_up = true;
                     break;
                 }
