@@ -145,7 +145,7 @@ struct BankOrder {
 // the edge signs per direction (+x, -x, +y, -y, +z, -z; bit = the coupling
 // is negative) and the shape masks (par, k = 0, k = L/2 - 1, y = 0,
 // y = L - 1, z = 0, z = L - 1).
-bool lattice_sweep_table(const Problem& P, std::vector<uint32_t>& out) {
+bool lattice_sweep_qualifies(const Problem& P) {
     const int32_t D = P.lat_dim, L = P.lat_L;
     if (D < 2 || D > 3 || L < 4 || (L & 1)) return false;
     const int32_t H = L / 2;
@@ -156,6 +156,13 @@ bool lattice_sweep_table(const Problem& P, std::vector<uint32_t>& out) {
         const int32_t o = P.pos2orig[p], x = o % L, r = o / L, y = r % L, z = r / L;
         if (p != ((x + y + z) & 1) * (P.n / 2) + r * H + x / 2) return false;
     }
+    return true;
+}
+
+bool lattice_sweep_table(const Problem& P, std::vector<uint32_t>& out) {
+    if (!lattice_sweep_qualifies(P)) return false;
+    const int32_t D = P.lat_dim, L = P.lat_L;
+    const int32_t H = L / 2;
     auto weight = [&](int32_t i, int32_t j) {
         for (int32_t q = P.rowptr[i]; q < P.rowptr[i + 1]; ++q)
             if (P.col[q] == j) return P.w[q];
@@ -246,20 +253,38 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         for (int32_t wv : pp->w) unit_ok = unit_ok && (wv == 1 || wv == -1);
     }
     fmt = unit_ok ? kFmtUnit : kFmtWide;
-    // sign table of the split sweep: unit records with a negative coupling
-    // somewhere, every instance's table within kSignTableMax bytes
-    // (and, so that every launch shape that fits without it still fits, 8
-    // waves' tables and threshold rows within 64 KiB)
+    // instances the split sweep takes through the multispin lattice path
+    // (it reads their state bits, never their spin bytes)
+    std::vector<char> lsw_q((size_t)ninst);
+    int64_t lsw_need = 0;
+    for (int32_t i = 0; i < ninst; ++i) {
+        const Problem& P = *problems[i];
+        lsw_q[(size_t)i] = lattice_sweep_qualifies(P) ? 1 : 0;
+        if (lsw_q[(size_t)i]) lsw_need = std::max<int64_t>(lsw_need, lsw_lds_bytes((P.n + 31) / 32, P.n, P.lat_dim));
+    }
+    // the split sweep's waves per workgroup (as set below)
+    int64_t reps0 = 0;
+    for (int32_t i = 0; i < ninst; ++i) reps0 += (int64_t)n_temps[i] * cfg.chains;
+    const int64_t Wsweep = cfg.waves_per_group > 0 ? std::min(cfg.waves_per_group, 8) : (reps0 < 4 * 256 ? 1 : 4);
+    // sign table of the split sweep: unit records and a negative coupling in
+    // an instance the sweep gathers for, every such instance's table within
+    // kSignTableMax bytes, and the sweep's waves' tables (or lattice states)
+    // and threshold rows within 64 KiB
     tab = 0;
     if (fmt == kFmtUnit) {
         bool any_neg = false;
         int64_t tmax = 0, fmx = 1;
-        for (const auto& pp : problems) {
-            for (int32_t wv : pp->w) any_neg = any_neg || wv < 0;
-            tmax = std::max<int64_t>(tmax, sign_table_bytes((pp->n + 31) / 32));
-            fmx = std::max<int64_t>(fmx, pp->fmax);
+        for (int32_t i = 0; i < ninst; ++i) {
+            const Problem& P = *problems[i];
+            fmx = std::max<int64_t>(fmx, P.fmax);
+            if (lsw_q[(size_t)i]) continue;
+            for (int32_t wv : P.w) any_neg = any_neg || wv < 0;
+            tmax = std::max<int64_t>(tmax, sign_table_bytes((P.n + 31) / 32));
         }
-        tab = any_neg && tmax <= kSignTableMax && 8 * (align_up(tmax, 16) + 4 * align_up(fmx, 4)) <= 64 * 1024 ? 1 : 0;
+        tab = any_neg && tmax <= kSignTableMax &&
+                      Wsweep * (align_up(std::max(tmax, lsw_need), 16) + 4 * align_up(fmx, 4)) <= 64 * 1024
+                  ? 1
+                  : 0;
     }
     for (int32_t i = 0; i < ninst; ++i) {
         const Problem& P = *problems[i];
@@ -472,11 +497,19 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
     Wi = cfg.icm_waves;
     if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
-    spin_stride = (int32_t)align_up(
-        std::max<int64_t>(tab ? sign_table_bytes(nw_max) : spin_bytes(nw_max), lsw_bytes), 16);
+    // per-wave LDS of the split sweep: the spin bytes (or sign tables) of the
+    // instances it gathers for and the multispin lattices' states; k_init
+    // expands every instance's spin bytes
+    int64_t gather_need = 0;
+    for (int32_t i = 0; i < ninst; ++i)
+        if (!lsw_q[(size_t)i]) gather_need = std::max<int64_t>(gather_need, tab ? sign_table_bytes(desc[i].nw)
+                                                                                 : spin_bytes(desc[i].nw));
+    spin_stride = (int32_t)align_up(std::max<int64_t>(gather_need, lsw_bytes), 16);
+    init_stride = (int32_t)align_up(std::max<int64_t>(spin_stride, spin_bytes(nw_max)), 16);
     thr_stride = (int32_t)align_up(std::max(fmax_max, 1), 4);  // at least one word: the sweep may read entry 0
     const int64_t lds = (int64_t)W * (spin_stride + 4ll * thr_stride);
-    if (lds > 64 * 1024) return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
+    if (lds > 64 * 1024 || (int64_t)W * init_stride > 64 * 1024)
+        return Status::err(PTX_ERR_UNSUPPORTED, "sweep workgroup exceeds 64 KiB of LDS; use fewer waves");
     // work items: largest instances first so the tail of the grid holds the small ones
     std::vector<int32_t> order(ninst);
     for (int32_t i = 0; i < ninst; ++i) order[i] = i;
@@ -655,7 +688,8 @@ static DevArgs build_args(Engine& e) {
 
 Status Engine::launch_init(int mode) {
     DevArgs a = build_args(*this);
-    const size_t lds = (size_t)W * spin_stride;
+    a.spin_stride = init_stride;  // every instance's spin bytes, lattices included
+    const size_t lds = (size_t)W * init_stride;
     const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a, mode); };
     if (fmt == kFmtUnit) {

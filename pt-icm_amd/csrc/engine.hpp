@@ -64,6 +64,7 @@ struct Engine {
     int32_t nrep_max = 0, nw_max = 0, n_max = 0, fmax_max = 1;
     int32_t lat_words_max = 0;   // largest lattice-order bit set (words)
     int32_t spin_stride = 0, thr_stride = 1;
+    int32_t init_stride = 0;  // k_init's LDS bytes per wave (spin bytes of the largest instance)
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;
     std::vector<int2> items, pairs;
     std::vector<int32_t> inst_list;  // active instances in size order (one workgroup each)
