@@ -2,6 +2,9 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
+#include <thread>
 #include <unordered_map>
 #include <cstring>
 #include <string>
@@ -135,6 +138,85 @@ struct BankOrder {
     }
 };
 
+// Order of the records within each adjacency row (unit format) for the
+// split sweep's gathers.  A wave sweeps one replica: lane l takes the groups
+// of 4 positions (cs >> 2) + l, + 64, ... of a class (positions outside the
+// class idle), and the gather of record slot s of the lane's k-th position
+// (k = p mod 4) is one byte read by every lane at once, each half-wave of 32
+// lanes in its own LDS pass; two lanes of a half-wave that read different
+// words of one bank serialise.  Rows are walked in position order; each
+// record takes, among its row's free slots, the one whose gather already
+// reads its word or reads the fewest other words in its bank (first such
+// slot).  As with the cooperative order, any order gives the same sums.
+struct SplitOrder {
+    static constexpr int kBanks = 32, kCap = 32;  // a half-wave reads at most 32 words per gather
+    int slots;                                    // row slots per k (max degree + 1)
+    std::vector<uint8_t> cnt;                     // (k, slot, bank) -> words so far
+    std::vector<uint16_t> words;                  // (k, slot, bank) -> those words
+    std::vector<int32_t> touched;
+    std::vector<int32_t> freeslots;
+    int64_t group = -1;
+    explicit SplitOrder(int max_deg)
+        : slots(max_deg + 1), cnt((size_t)4 * slots * kBanks, 0), words((size_t)4 * slots * kBanks * kCap) {}
+    // grp: (instance, class, iteration, half-wave); addr_mask keeps a record's
+    // LDS byte address (position, and the sign-table offset when in use)
+    void order(int64_t grp, int k, uint16_t* rec, int deg, uint32_t addr_mask) {
+        if (grp != group) {
+            for (int32_t t : touched) cnt[(size_t)t] = 0;
+            touched.clear();
+            group = grp;
+        }
+        freeslots.resize((size_t)deg);
+        for (int q = 0; q < deg; ++q) freeslots[(size_t)q] = q;
+        uint16_t out[kMaxDegree + 1];
+        int nfree = deg;
+        for (int j = 0; j < deg; ++j) {
+            const uint32_t word = ((uint32_t)rec[j] & addr_mask) >> 2, bank = word % kBanks;
+            int best = 0, best_cost = 1 << 30;
+            for (int f = 0; f < nfree; ++f) {
+                const int sl = freeslots[(size_t)f];
+                const size_t e = ((size_t)k * slots + sl) * kBanks + bank;
+                const int c = cnt[e];
+                int cost = c;
+                if (c > 0) {
+                    const uint16_t* wv = &words[e * kCap];
+                    for (int q = 0; q < c; ++q)
+                        if (wv[q] == (uint16_t)word) {
+                            cost = 0;
+                            break;
+                        }
+                }
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best = f;
+                    if (cost == 0) break;
+                }
+            }
+            const int sl = freeslots[(size_t)best];
+            freeslots[(size_t)best] = freeslots[(size_t)(--nfree)];
+            out[sl] = rec[j];
+            const size_t e = ((size_t)k * slots + sl) * kBanks + bank;
+            const int c = cnt[e];
+            bool have = false;
+            for (int q = 0; q < c && !have; ++q) have = words[e * kCap + q] == (uint16_t)word;
+            if (!have && c < kCap) {
+                if (c == 0) touched.push_back((int32_t)e);
+                words[e * kCap + c] = (uint16_t)word;
+                cnt[e] = (uint8_t)(c + 1);
+            }
+        }
+        std::copy(out, out + deg, rec);
+    }
+};
+
+// One problem's adjacency rows: rowinfo entries (row start in 16-byte units
+// from the problem's first row << 9 | degree) and the padded rows.
+struct RowBlock {
+    std::vector<uint32_t> info, adj;
+    int64_t len16 = 0;
+    bool too_big = false;
+};
+
 // Table of the multispin lattice sweep (InstDesc::lsw_dim), appended to
 // out; false when the instance does not qualify: a periodic L^D lattice
 // (D = 2, 3) with even L, couplings +-1, no field, and positions that are the
@@ -244,6 +326,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     const bool hist = (cfg.flags & PTX_FLAG_HISTOGRAM) != 0;
     int64_t t_off = 0;
     std::unordered_map<const Problem*, int32_t> first_copy;  // problem -> its first entry
+    int32_t deg_max = 0;
+    for (const auto& pp : problems) deg_max = std::max(deg_max, pp->max_deg);
     // record format: 16-bit when every instance has |w| = 1 and n <= 32768
     bool unit_ok = true;
     for (const auto& pp : problems) {
@@ -286,6 +370,105 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                   ? 1
                   : 0;
     }
+    // Adjacency rows of every distinct problem, by position, each padded to
+    // 16 bytes (at least one chunk); padding records name the sentinel
+    // position 32 * nw (the wide format masks padding by weight 0, so there
+    // any in-range position will do).  Built on worker threads, one problem at
+    // a time each: ordering the records for the sweeps' gathers is most of the
+    // host work of setting up a large batch.
+    std::vector<int32_t> block_of((size_t)ninst);
+    std::vector<const Problem*> block_prob;
+    {
+        std::unordered_map<const Problem*, int32_t> bidx;
+        for (int32_t i = 0; i < ninst; ++i) {
+            auto it = bidx.find(problems[i].get());
+            if (it == bidx.end()) {
+                it = bidx.emplace(problems[i].get(), (int32_t)block_prob.size()).first;
+                block_prob.push_back(problems[i].get());
+            }
+            block_of[(size_t)i] = it->second;
+        }
+    }
+    std::vector<RowBlock> blocks(block_prob.size());
+    // engines that run the cooperative kernel with one position per lane from
+    // the start order each row's records for its gathers, the others for the
+    // split sweep's (any order is exact)
+    const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
+                           (cfg.mode == PTX_MODE_COOP || (cfg.mode == PTX_MODE_AUTO && ninst == 1));
+    const int bank_threads = kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4);
+    auto build_block = [&](size_t bi, BankOrder& bank_order, SplitOrder* split_order) {
+        const Problem& P = *block_prob[bi];
+        RowBlock& out = blocks[bi];
+        const int32_t nw = (P.n + 31) / 32;
+        const int32_t kbit = tab ? sign_table_bit(nw) : 15;
+        const uint32_t sentinel = sentinel_pos(nw);
+        out.info.reserve((size_t)P.n);
+        int64_t cur16 = 0;
+        int32_t cls_of = 0;
+        std::vector<uint16_t> rec;
+        for (int32_t p = 0; p < P.n; ++p) {
+            const int32_t o = P.pos2orig[p];
+            const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
+            if (cur16 >= (1ll << 23)) {
+                out.too_big = true;
+                return;
+            }
+            out.info.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
+            if (fmt == kFmtUnit) {
+                const int32_t chunks = std::max(1, (dg + 7) / 8);
+                rec.assign((size_t)chunks * 8, (uint16_t)sentinel);
+                for (int32_t j = 0; j < dg; ++j) {
+                    const int32_t q = P.rowptr[o] + j;
+                    rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << kbit : 0u));
+                }
+                while (p >= P.class_start[cls_of + 1]) ++cls_of;
+                const int32_t c0 = P.class_start[cls_of];
+                if (coop_rows) {
+                    bank_order.order(((int64_t)bi << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
+                                     dg, (1u << kbit) - 1u);
+                } else if (split_order) {
+                    const int32_t gi = (p >> 2) - (c0 >> 2);  // group of 4 positions within the class
+                    const int64_t grp = (((int64_t)bi << 12 | cls_of) << 20) | (int64_t)(gi >> 5);  // half-wave
+                    split_order->order(grp, p & 3, rec.data(), dg, tab ? (2u << kbit) - 1u : (1u << kbit) - 1u);
+                }
+                for (int32_t c = 0; c < chunks * 4; ++c)
+                    out.adj.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
+                cur16 += chunks;
+            } else {
+                const int32_t chunks = std::max(1, (dg + 3) / 4);
+                for (int32_t j = 0; j < chunks * 4; ++j) {
+                    if (j < dg) {
+                        const int32_t q = P.rowptr[o] + j;
+                        out.adj.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
+                    } else {
+                        out.adj.push_back(sentinel);  // weight 0: contributes nothing
+                    }
+                }
+                cur16 += chunks;
+            }
+        }
+        out.len16 = cur16;
+    };
+    {
+        const bool split_rows = fmt == kFmtUnit && !coop_rows;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const size_t nth = std::min<size_t>(blocks.size(), std::max(1u, std::min(16u, hw)));
+        auto worker = [&](std::atomic<size_t>* next) {
+            BankOrder bank_order(bank_threads);
+            std::unique_ptr<SplitOrder> so(split_rows ? new SplitOrder(deg_max) : nullptr);
+            for (size_t bi; (bi = next->fetch_add(1)) < blocks.size();) build_block(bi, bank_order, so.get());
+        };
+        std::atomic<size_t> next{0};
+        if (nth <= 1) {
+            worker(&next);
+        } else {
+            std::vector<std::thread> pool;
+            for (size_t t = 0; t < nth; ++t) pool.emplace_back(worker, &next);
+            for (auto& th : pool) th.join();
+        }
+        for (const RowBlock& blk : blocks)
+            if (blk.too_big) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
+    }
     for (int32_t i = 0; i < ninst; ++i) {
         const Problem& P = *problems[i];
         const int32_t nt = n_temps[i];
@@ -326,55 +509,15 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             d.adj_len16 = s->adj_len16;
             d.cls_off = s->cls_off;
         } else {
-            // adjacency rows by position, each padded to 16 bytes (at least one
-            // chunk); padding records name the sentinel position 32 * nw
+            // this problem's rows, built before the loop
+            RowBlock& blk = blocks[(size_t)block_of[(size_t)i]];
             d.pos_off = (int64_t)rowinfo.size();
             d.adj_off16 = (int64_t)adjw.size() / 4;
-            // (the wide format masks padding by weight 0, so there any in-range position will do)
-            const uint32_t sentinel = sentinel_pos(d.nw);
-            int64_t cur16 = 0;
-            // engines that run the cooperative kernel with one position per lane
-            // from the start order each row's records for its gathers (any order
-            // is exact; a batch that may switch to it after retirement does not:
-            // the ordering costs ~1.4 us per row of degree 20 on the host, 0.36 s
-            // for 256 ER graphs of 1,000 vertices)
-            const bool coop_rows = fmt == kFmtUnit && cfg.spins_per_lane != 4 &&
-                                   (cfg.mode == PTX_MODE_COOP || (cfg.mode == PTX_MODE_AUTO && ninst == 1));
-            BankOrder bank_order(kWave * (cfg.waves_per_group == 16 ? 16 : cfg.waves_per_group >= 8 ? 8 : 4));
-            int32_t cls_of = 0;
-            for (int32_t p = 0; p < P.n; ++p) {
-                const int32_t o = P.pos2orig[p];
-                const int32_t dg = P.rowptr[o + 1] - P.rowptr[o];
-                if (cur16 >= (1ll << 23)) return Status::err(PTX_ERR_UNSUPPORTED, "instance adjacency above 128 MiB");
-                rowinfo.push_back((uint32_t)(cur16 << 9) | (uint32_t)dg);
-                if (fmt == kFmtUnit) {
-                    const int32_t chunks = std::max(1, (dg + 7) / 8);
-                    std::vector<uint16_t> rec((size_t)chunks * 8, (uint16_t)sentinel);
-                    for (int32_t j = 0; j < dg; ++j) {
-                        const int32_t q = P.rowptr[o] + j;
-                        rec[j] = (uint16_t)((uint32_t)P.orig2pos[P.col[q]] | (P.w[q] < 0 ? 1u << d.kbit : 0u));
-                    }
-                    if (coop_rows) {
-                        while (p >= P.class_start[cls_of + 1]) ++cls_of;
-                        const int32_t c0 = P.class_start[cls_of];
-                        bank_order.order(((int64_t)i << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
-                                         dg, (1u << d.kbit) - 1u);
-                    }
-                    for (int32_t c = 0; c < chunks * 4; ++c) adjw.push_back((uint32_t)rec[2 * c] | ((uint32_t)rec[2 * c + 1] << 16));
-                    cur16 += chunks;
-                } else {
-                    const int32_t chunks = std::max(1, (dg + 3) / 4);
-                    for (int32_t j = 0; j < chunks * 4; ++j) {
-                        if (j < dg) {
-                            const int32_t q = P.rowptr[o] + j;
-                            adjw.push_back((uint32_t)P.orig2pos[P.col[q]] | ((uint32_t)(uint16_t)(int16_t)P.w[q] << 16));
-                        } else {
-                            adjw.push_back(sentinel);  // weight 0: contributes nothing
-                        }
-                    }
-                    cur16 += chunks;
-                }
-            }
+            rowinfo.insert(rowinfo.end(), blk.info.begin(), blk.info.end());
+            adjw.insert(adjw.end(), blk.adj.begin(), blk.adj.end());
+            const int64_t cur16 = blk.len16;
+            std::vector<uint32_t>().swap(blk.info);
+            std::vector<uint32_t>().swap(blk.adj);
             d.adj_len16 = cur16;
             d.cls_off = (int64_t)cls.size();
             cls.insert(cls.end(), P.class_start.begin(), P.class_start.end());
