@@ -108,12 +108,16 @@ def cpu_baseline(problem_path, betas, seconds, seed=0, cores=None):
                       f"(slowest run {max(times):.1f} s); one run alone {single:.1f} s"}
 
 
-def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
+def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100, ladders="surrogate", log=print):
     """Throughput of the batched engine at the scale of the batched + surrogate
     configuration: half 2D +-J tori L = 100 (10,000 spins), half ER +-1 graphs
-    G(5000, 0.004); 64 inverse temperatures linear in beta over T 0.5..3.0
-    (ascending, so the slot rule gives the cold slots ICM), 2 chains, ICM.
-    The launch shape comes from the autotuner (8 steps per trial)."""
+    G(5000, 0.004), T 0.5..3.0, 2 chains, ICM.  ladders="surrogate": the
+    configuration's ladders -- each instance's network ladder from its 8-slot
+    pilot, judged by a 2,000-step validation run (every adjacent swap rate
+    within 0.4 +- 0.05) and replaced by the 64-slot multihistogram ladder where
+    it fails; "linear": 64 inverse temperatures linear in beta.  Ladders go to
+    the engine ascending (the slot rule then gives the cold slots ICM).  The
+    launch shape comes from the autotuner (8 steps per trial)."""
     sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
     import instances
     from concurrent.futures import ThreadPoolExecutor
@@ -123,17 +127,34 @@ def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
             c = instances.torus_2d(100, seed=seed * 100003 + i)
         else:
             c = instances.erdos_renyi(5000, 0.004, seed=seed * 100003 + i, pm=True)
-        return ptx.Problem.from_coo(*c)
+        return c, ptx.Problem.from_coo(*c)
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(8) as ex:
-        probs = list(ex.map(make, range(n_inst)))
+        made = list(ex.map(make, range(n_inst)))
+    probs = [m[1] for m in made]
     t_build = time.perf_counter() - t0
-    lad = np.sort(ptx.beta_linspace(0.5, 3.0, 64, False))
+    lad_info = {"ladders": ladders}
+    if ladders == "surrogate":
+        import surrogate as S
+        stats = [S.problem_stats(m[0]) for m in made]
+        sur = S.Surrogate(ptx)
+        t0 = time.perf_counter()
+        lads, ok, _, stages = S.choose_ladders(ptx, sur, probs, stats, 0.5, 3.0, seed=seed)
+        sur.close()
+        lads = [np.sort(l) for l in lads]
+        lad_info.update({"ladder_s": round(time.perf_counter() - t0, 2), "surrogate_accepted": int(sum(ok)),
+                         "fallback": int(len(ok) - sum(ok)),
+                         "ladder_stages_s": {k: round(v, 2) for k, v in stages.items()},
+                         "mean_slots": float(np.mean([len(l) for l in lads]))})
+        log(f"at-scale ladders {lad_info}")
+    else:
+        lads = [np.sort(ptx.beta_linspace(0.5, 3.0, 64, False))] * n_inst
+    del made
     t0 = time.perf_counter()
-    kw, trials = ptx.autotune(probs, [lad] * n_inst, seed=seed, steps=8, flags=ptx.FLAG_ICM)
+    kw, trials = ptx.autotune(probs, lads, seed=seed, steps=8, flags=ptx.FLAG_ICM)
     t_tune = time.perf_counter() - t0
-    eng = ptx.Engine(probs, [lad] * n_inst, seed=seed, flags=ptx.FLAG_ICM, use_graph=4, **kw)
+    eng = ptx.Engine(probs, lads, seed=seed, flags=ptx.FLAG_ICM, use_graph=4, **kw)
     eng.step(warmup)  # past the first steps from random states, whose cluster moves span half the lattice
     eng.sync()
     t0 = time.perf_counter()
@@ -141,15 +162,18 @@ def batched_mixed_at_scale(ptx, steps, seed, n_inst=1024, warmup=100):
     eng.sync()
     dt = time.perf_counter() - t0
     info = eng.info()
-    icm = [eng.instance_info(i).nc for i in range(2)]
+    icm = {"torus": float(np.mean([eng.instance_info(i).nc for i in range(0, n_inst, 2)])),
+           "er": float(np.mean([eng.instance_info(i).nc for i in range(1, n_inst, 2)]))}
     eng.close()
     for q in probs:
         q.close()
+    lad_text = ("surrogate ladders (8-slot pilot, network, 2,000-step judge, 64-slot multihistogram fallback)"
+                if ladders == "surrogate" else "64 betas linear")
     return {"workload": f"{n_inst // 2} 2D +-J tori L=100 + {n_inst - n_inst // 2} ER +-1 G(5000, 0.004); "
-                        "64 betas linear over T 0.5..3.0, 2 chains, ICM",
+                        f"{lad_text} over T 0.5..3.0, 2 chains, ICM",
             "value": info.spin_updates_per_step * steps / dt, "unit": "spin updates/s",
             "ms_per_step": dt * 1e3 / steps, "steps": steps, "warmup": warmup, "replicas": info.total_replicas,
-            "icm_slots": {"torus": icm[0], "er": icm[1]},
+            "mean_icm_slots": icm, **lad_info,
             "launch_shape": {"mode": info.mode, "waves_per_group": info.waves_per_group,
                              "icm_waves": info.icm_waves},
             "build_s": round(t_build, 2), "autotune_s": round(t_tune, 2),
@@ -243,6 +267,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", type=int, default=0,
                     help="0 auto (cooperative per-replica kernel for one instance), 1 split, 2 fused, 3 cooperative")
+    ap.add_argument("--scale-ladders", choices=["surrogate", "linear"], default="surrogate",
+                    help="ladders of the at-scale line: the configuration's surrogate pipeline, or linear in beta")
     ap.add_argument("--scale-steps", type=int, default=16,
                     help="timed PT steps of the 1024-instance mixed batch (0 = skip)")
     ap.add_argument("--batched-steps", type=int, default=64,
@@ -341,7 +367,8 @@ def main():
         out["batched"]["roofline"]["peak"] = lds_peak
         out["batched"]["roofline"]["frac"] = out["batched"]["roofline"]["achieved"] / lds_peak
     if args.scale_steps > 0:
-        out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed)
+        out["batched_at_scale"] = batched_mixed_at_scale(ptx, args.scale_steps, args.seed, ladders=args.scale_ladders,
+                                                         log=log)
     if args.ttt_plain > 0:
         sys.path.insert(0, os.path.join(REPO, "pt-icm_amd"))
         import ttt
