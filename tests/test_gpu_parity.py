@@ -239,6 +239,42 @@ def test_multispin_lattice_sweep_parity(ptx, oracle, dim, L, pm, icm):
     compare(eng2, oe2, what=f"L={L} dim={dim} two sweeps")
 
 
+@pytest.mark.parametrize("wpg", [2, 4, 8])
+def test_multispin_workgroup_sweep_parity(ptx, oracle, wpg):
+    """Multispin sweeps with several waves (replicas) per workgroup:
+    bit-exact with the oracle for a batch of 3D EA L = 4, 6, 8 and 2D
+    L = 6, 10, 32 lattices, with cluster moves, on ladders of 5, 7 and 3
+    slots (10, 14 and 6 replicas: workgroups whose last waves have no
+    replica), and with two sweeps per step."""
+    specs = [(3, 4), (3, 6), (3, 8), (2, 6), (2, 10), (2, 32)]
+    pairs = []
+    for dim, L in specs:
+        n, u, v, w = instances.ea_3d(L, seed=L) if dim == 3 else instances.torus_2d(L, seed=L, pm=True)
+        pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    nts = [5, 7, 3, 5, 7, 3]
+    ladders = [oracle.beta_linspace(0.3, 3.0, nt, 0) for nt in nts]
+    for sweeps in (1, 2):
+        say("multispin workgroup sweep W", wpg, "sweeps", sweeps)
+        eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=11 + sweeps, flags=ptx.FLAG_ICM, mode=1,
+                         waves_per_group=wpg, sweeps=sweeps)
+        assert eng.info().waves_per_group == wpg
+        assert all(eng.instance_info(i).lattice_sweep == specs[i][0] for i in range(len(specs)))
+        oes = [oracle.Engine(g, lad, chains=2, seed=11 + sweeps, instance=i, sweeps=sweeps, flags=oracle.FLAG_ICM)
+               for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
+        for s_ in range(3):
+            eng.step(1)
+            for oe in oes:
+                oe.step(1)
+            for i, oe in enumerate(oes):
+                compare(eng, oe, inst=i, what=f"W={wpg} sweeps={sweeps} inst {i} step {s_}")
+        eng.step(15)
+        for oe in oes:
+            oe.step(15)
+        for i, oe in enumerate(oes):
+            compare(eng, oe, inst=i, what=f"W={wpg} sweeps={sweeps} inst {i} bulk")
+        eng.close()
+
+
 def test_multispin_lattice_sweep_only_where_it_applies(ptx, oracle):
     """Odd L (three colour classes), a field, or couplings other than +-1 keep
     the gather sweep; the engine reports which instances take which."""
