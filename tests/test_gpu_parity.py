@@ -363,6 +363,27 @@ def test_multiwave_gather_sweep_parity(ptx, oracle, batch, wpg):
         compare(eng, oe, inst=i, what=f"{batch} W={wpg} inst {i} bulk")
 
 
+@pytest.mark.parametrize("kind,steps,per_launch", [("G022", 300, 500), ("lattice_2d_20.gset", 300, 500),
+                                                    ("er_small", 300, 500), ("G022", 60, 7)])
+def test_cooperative_best_state_over_long_launches(ptx, oracle, kind, steps, per_launch):
+    """The cooperative kernel without cluster moves keeps a replica's best
+    state as a log of the flips after the last new best, built into the best
+    bytes at the end of a launch or before a class that could overflow the
+    log (long launches with rare new bests); best energies, sweeps and states
+    stay bit-exact with the oracle's after every part of the run."""
+    p, g = make_pair(ptx, oracle, kind)
+    nt = 11
+    betas = oracle.beta_linspace(0.4, 3.0, nt, 0)
+    say("coop best", kind, steps, per_launch)
+    eng = ptx.Engine(p, betas, seed=41, flags=0, mode=3, steps_per_launch=per_launch)  # no cluster moves
+    oe = oracle.Engine(g, betas, chains=2, seed=41, flags=0)
+    assert eng.instance_info().nc == 0  # the kernel without cluster-move code
+    for part in (steps // 3, steps - steps // 3):
+        eng.step(part)
+        oe.step(part)
+        compare(eng, oe, what=f"{kind} after {part} more steps")
+
+
 @pytest.mark.parametrize("kind", ["G022", "field_random", "ea3d_6"])
 def test_cooperative_16_waves_parity(ptx, oracle, kind):
     """Cooperative kernel with 16 waves per replica (no ICM slot; 4 lanes
