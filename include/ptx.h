@@ -161,6 +161,11 @@ typedef struct {
                                 returns PTX_ERR_DEVICE and no further launch is enqueued.  A negative
                                 value gives up at the first poll that finds a replica missing (tests of
                                 that error path). */
+    int32_t sliced_sweep;    /* split mode: 0 (auto) sweeps the instances with couplings +-1, no field and
+                                degree <= 63 that take no multispin lattice sweep 32 replicas at a time
+                                (a workgroup per 32 replicas, each neighbour word one gather for all 32,
+                                counts bit-sliced; the same decisions) with 4 waves per workgroup, 1, 2,
+                                4 or 8 with that many; -1: every instance by the per-replica sweep */
 } ptx_engine_config;
 void ptx_engine_config_default(ptx_engine_config* cfg);
 
@@ -226,6 +231,8 @@ typedef struct {
     int64_t e_base;
     int32_t lattice_sweep;   /* 2 or 3: split-mode sweeps of this instance use the multispin lattice
                                 sweep (periodic lattice, even L, couplings +-1, no field); 0: gathers */
+    int32_t sliced_sweep;    /* 1: split-mode sweeps of this instance are replica-sliced (32 replicas per
+                                wave, see ptx_engine_config.sliced_sweep); 0: one replica per wave */
 } ptx_instance_info;
 ptx_status ptx_engine_instance_info(const ptx_engine* e, int32_t inst, ptx_instance_info* out);
 

@@ -239,6 +239,7 @@ void ptx_engine_config_default(ptx_engine_config* c) {
     c->icm_waves = 0;
     c->small_batch_coop = 0;
     c->coop_wait_us = 0;
+    c->sliced_sweep = 0;
 }
 
 ptx_status ptx_engine_create(const ptx_problem* const* problems, int32_t n_instances, const double* betas,
@@ -377,6 +378,7 @@ ptx_status ptx_engine_instance_info(const ptx_engine* e, int32_t inst, ptx_insta
     out->hist_bins = d.hist_bins;
     out->e_base = d.e_base;
     out->lattice_sweep = d.lsw_dim;
+    out->sliced_sweep = d.rs;
     return PTX_OK;
 }
 

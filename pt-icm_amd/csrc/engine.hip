@@ -346,6 +346,24 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         lsw_q[(size_t)i] = lattice_sweep_qualifies(P) ? 1 : 0;
         if (lsw_q[(size_t)i]) lsw_need = std::max<int64_t>(lsw_need, lsw_lds_bytes((P.n + 31) / 32, P.n, P.lat_dim));
     }
+    // instances the split sweep may take replica-sliced (k_sweep_rs, 32
+    // replicas per wave; kept when the engine runs the split kernels)
+    std::vector<char> rs_q((size_t)ninst);
+    for (int32_t i = 0; i < ninst; ++i) {
+        const Problem& P = *problems[i];
+        // auto: only instances whose colour classes average at least 512
+        // positions (128 groups of 4), so a workgroup's 256 lanes find work in
+        // most of a class's passes (measured: G(5000, 0.004), 625 per class,
+        // sweeps 1.4x faster; G(1000, 0.02), 125 per class, 1.6x slower), and
+        // with 32 replicas or more (full blocks share each gather 32 ways)
+        const bool big = cfg.sliced_sweep > 0 || ((int64_t)P.n >= 512ll * std::max(P.ncls, 1) &&
+                                                  (int64_t)n_temps[i] * cfg.chains >= 32);
+        rs_q[(size_t)i] = cfg.sliced_sweep >= 0 && cfg.mode != PTX_MODE_COOP && cfg.mode != PTX_MODE_FUSED &&
+                                  fmt == kFmtUnit && !lsw_q[(size_t)i] && !P.has_field && P.max_deg <= 63 && big &&
+                                  sliced_lds_bytes((P.n + 31) / 32, P.fmax) <= 64 * 1024
+                              ? 1
+                              : 0;
+    }
     // the split sweep's waves per workgroup (as set below)
     int64_t reps0 = 0;
     for (int32_t i = 0; i < ninst; ++i) reps0 += (int64_t)n_temps[i] * cfg.chains;
@@ -361,7 +379,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
         for (int32_t i = 0; i < ninst; ++i) {
             const Problem& P = *problems[i];
             fmx = std::max<int64_t>(fmx, P.fmax);
-            if (lsw_q[(size_t)i]) continue;
+            if (lsw_q[(size_t)i] || rs_q[(size_t)i]) continue;
             for (int32_t wv : P.w) any_neg = any_neg || wv < 0;
             tmax = std::max<int64_t>(tmax, sign_table_bytes((P.n + 31) / 32));
         }
@@ -378,6 +396,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // host work of setting up a large batch.
     std::vector<int32_t> block_of((size_t)ninst);
     std::vector<const Problem*> block_prob;
+    std::vector<char> block_rs;  // the problem's sweeps are replica-sliced: any record order will do
     {
         std::unordered_map<const Problem*, int32_t> bidx;
         for (int32_t i = 0; i < ninst; ++i) {
@@ -385,6 +404,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
             if (it == bidx.end()) {
                 it = bidx.emplace(problems[i].get(), (int32_t)block_prob.size()).first;
                 block_prob.push_back(problems[i].get());
+                block_rs.push_back(rs_q[(size_t)i]);
             }
             block_of[(size_t)i] = it->second;
         }
@@ -426,7 +446,7 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                 if (coop_rows) {
                     bank_order.order(((int64_t)bi << 12) | cls_of, p - c0, P.class_start[cls_of + 1] - c0, rec.data(),
                                      dg, (1u << kbit) - 1u);
-                } else if (split_order) {
+                } else if (split_order && !block_rs[bi]) {
                     const int32_t gi = (p >> 2) - (c0 >> 2);  // group of 4 positions within the class
                     const int64_t grp = (((int64_t)bi << 12 | cls_of) << 20) | (int64_t)(gi >> 5);  // half-wave
                     split_order->order(grp, p & 3, rec.data(), dg, tab ? (2u << kbit) - 1u : (1u << kbit) - 1u);
@@ -638,6 +658,9 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     if (W <= 0) W = (total_rep < 4 * 256) ? 1 : 4;  // spread few replicas over CUs, group many
     if (W > 8) W = 8;                                 // split kernels take at most 8 waves
     if (W != 1 && W != 2 && W != 4 && W != 8) return Status::err(PTX_ERR_INVALID, "waves_per_group must be 1,2,4,8,16");
+    if (cfg.sliced_sweep != 0 && cfg.sliced_sweep != -1 && cfg.sliced_sweep != 1 && cfg.sliced_sweep != 2 &&
+        cfg.sliced_sweep != 4 && cfg.sliced_sweep != 8)
+        return Status::err(PTX_ERR_INVALID, "sliced_sweep must be 0, -1, 1, 2, 4 or 8");
     Wi = cfg.icm_waves;
     if (Wi != 0 && Wi != 1 && Wi != 2 && Wi != 4) return Status::err(PTX_ERR_INVALID, "icm_waves must be 0, 1, 2 or 4");
     // per-wave LDS of the split sweep: the spin bytes (or sign tables) of the
@@ -645,8 +668,8 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
     // expands every instance's spin bytes
     int64_t gather_need = 0;
     for (int32_t i = 0; i < ninst; ++i)
-        if (!lsw_q[(size_t)i]) gather_need = std::max<int64_t>(gather_need, tab ? sign_table_bytes(desc[i].nw)
-                                                                                 : spin_bytes(desc[i].nw));
+        if (!lsw_q[(size_t)i] && !rs_q[(size_t)i])
+            gather_need = std::max<int64_t>(gather_need, tab ? sign_table_bytes(desc[i].nw) : spin_bytes(desc[i].nw));
     spin_stride = (int32_t)align_up(std::max<int64_t>(gather_need, lsw_bytes), 16);
     init_stride = (int32_t)align_up(std::max<int64_t>(spin_stride, spin_bytes(nw_max)), 16);
     thr_stride = (int32_t)align_up(std::max(fmax_max, 1), 4);  // at least one word: the sweep may read entry 0
@@ -733,6 +756,18 @@ Status Engine::create(const std::vector<std::shared_ptr<const Problem>>& problem
                                                         " waves); use split mode");
         K = cfg.steps_per_launch > 0 ? cfg.steps_per_launch : 16;
     }
+    // replica-sliced sweeps in split mode; k_sweep_rs's LDS fits the largest
+    rs_stride = 0;
+    for (int32_t i = 0; i < ninst; ++i) {
+        InstDesc& d = desc[(size_t)i];
+        d.rs = mode == PTX_MODE_SPLIT && rs_q[(size_t)i] ? 1 : 0;
+        if (d.rs) rs_stride = std::max(rs_stride, (int32_t)sliced_lds_bytes(d.nw, d.fmax));
+    }
+    // waves per 32 replicas: as configured, else 4 (measured on G(5000,
+    // 0.004), whose classes take 157 groups of 4: 4 waves 0.72 ms per sweep of
+    // 256 instances, 3 waves 0.81 ms though fewer lanes idle, 2 waves 1.15 ms,
+    // 8 waves 1.03 ms; more waves hide more latency until most lanes idle)
+    rs_W = cfg.sliced_sweep > 0 ? cfg.sliced_sweep : 4;
     // ---- device buffers ----
     PTX_TRY(d_desc.upload(desc, stream));
     PTX_TRY(d_inst_step.alloc((size_t)ninst));
@@ -871,11 +906,30 @@ Status Engine::launch_step() {
         pending.push_back(t);
         return Status::ok();
     };
-    if (cfg.sweeps > 0) {
-        const size_t lds = (size_t)W * (spin_stride + 4 * (size_t)thr_stride);
-        const dim3 grid((unsigned)items.size()), block((unsigned)(W * kWave));
+    if (cfg.sweeps > 0 && (!rs_items.empty() || !sweep_items.empty())) {
+        // one timed phase: the replica-sliced instances, then the others
         PTX_TRY(timed(0, [&] {
-            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, a); };
+            if (!rs_items.empty()) {
+                DevArgs ar = a;
+                ar.items = d_rs_items.p;
+                const int Wr = rs_W;
+                auto go = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, dim3((unsigned)rs_items.size()), dim3((unsigned)(Wr * kWave)),
+                                       (size_t)rs_stride, stream, ar);
+                };
+                switch (Wr) {
+                    case 1: go(k_sweep_rs<1>); break;
+                    case 2: go(k_sweep_rs<2>); break;
+                    case 8: go(k_sweep_rs<8>); break;
+                    default: go(k_sweep_rs<4>); break;
+                }
+            }
+            if (sweep_items.empty()) return;
+            DevArgs as = a;
+            as.items = d_sweep_items.p;
+            const size_t lds = (size_t)W * (spin_stride + 4 * (size_t)thr_stride);
+            const dim3 grid((unsigned)sweep_items.size()), block((unsigned)(W * kWave));
+            auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, stream, as); };
             if (fmt == kFmtUnit) {
                 switch (W) {
                     case 1: go(k_sweep<kFmtUnit, 1>); break;
@@ -1198,13 +1252,20 @@ Status Engine::build_lists() {
         updates_per_step += (int64_t)desc[i].nrep * P.n * cfg.sweeps;
         lds_bytes_per_step += (int64_t)desc[i].nrep * cfg.sweeps * (4 * P.nnz() + 6ll * P.n);
     }
+    sweep_items.clear();
+    rs_items.clear();
     if (act.size() >= 8) {
-        std::vector<std::vector<int2>> ix(8), px(8);
+        std::vector<std::vector<int2>> ix(8), px(8), sx(8), rx(8);
         for (size_t k = 0; k < act.size(); ++k) {
             const int32_t i = act[k];
             const int x = (int)(k % 8);
             for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) ix[x].push_back(make_int2(i, r0));
             for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) px[x].push_back(make_int2(i, t));
+            if (desc[i].rs) {
+                for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += 32) rx[x].push_back(make_int2(i, r0));
+            } else {
+                for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) sx[x].push_back(make_int2(i, r0));
+            }
         }
         auto arrange = [](const std::vector<std::vector<int2>>& per, std::vector<int2>& out) {
             size_t len = 0;
@@ -1215,9 +1276,23 @@ Status Engine::build_lists() {
         };
         arrange(ix, items);
         arrange(px, pairs);
+        arrange(sx, sweep_items);
+        arrange(rx, rs_items);
+        // an empty kind needs no idle items
+        if (std::all_of(sx.begin(), sx.end(), [](const std::vector<int2>& v) { return v.empty(); }))
+            sweep_items.clear();
+        if (std::all_of(rx.begin(), rx.end(), [](const std::vector<int2>& v) { return v.empty(); }))
+            rs_items.clear();
     } else {
         for (int32_t i : act)
             for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) items.push_back(make_int2(i, r0));
+        for (int32_t i : act) {
+            if (desc[i].rs) {
+                for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += 32) rs_items.push_back(make_int2(i, r0));
+            } else {
+                for (int32_t r0 = 0; r0 < desc[i].nrep; r0 += W) sweep_items.push_back(make_int2(i, r0));
+            }
+        }
         for (int32_t i : act)
             for (int32_t t = desc[i].nt - 1; t >= desc[i].nt - desc[i].nc; --t) pairs.push_back(make_int2(i, t));
     }
@@ -1228,6 +1303,8 @@ Status Engine::build_lists() {
         for (int32_t r = 0; r < desc[i].nrep; ++r) coop_items.push_back(make_int2(i, r));
     }
     PTX_TRY(put_list(d_items, items, stream));
+    PTX_TRY(put_list(d_sweep_items, sweep_items, stream));
+    PTX_TRY(put_list(d_rs_items, rs_items, stream));
     PTX_TRY(put_list(d_pairs, pairs, stream));
     PTX_TRY(put_list(d_order, inst_list, stream));
     PTX_TRY(put_list(d_coop_items, coop_items, stream));

@@ -67,6 +67,11 @@ struct Engine {
     int32_t init_stride = 0;  // k_init's LDS bytes per wave (spin bytes of the largest instance)
     int64_t updates_per_step = 0, lds_bytes_per_step = 0;
     std::vector<int2> items, pairs;
+    // split sweeps: k_sweep's items (instances with rs = 0) and k_sweep_rs's
+    // (instance, first of 32 replicas) items; items lists every instance (k_init)
+    std::vector<int2> sweep_items, rs_items;
+    int32_t rs_stride = 0;  // k_sweep_rs's LDS bytes (position words and threshold rows)
+    int32_t rs_W = 4;       // k_sweep_rs's waves per workgroup (32 replicas)
     std::vector<int32_t> inst_list;  // active instances in size order (one workgroup each)
     std::vector<int32_t> size_order; // all instances, largest first
     std::vector<uint8_t> active;
@@ -94,7 +99,7 @@ struct Engine {
     DevBuf<uint32_t> d_att, d_succ, d_hist;
     DevBuf<unsigned long long> d_obs_i;  // per slot: samples, sum E, sum up, sum |2 up - n| (PTX_FLAG_OBSERVE)
     DevBuf<double> d_obs_d;              // per slot: sum E^2, sum up^2
-    DevBuf<int2> d_items, d_pairs;
+    DevBuf<int2> d_items, d_pairs, d_sweep_items, d_rs_items;
     DevBuf<int32_t> d_order;
     DevBuf<uint32_t> d_inst_step;
     DevBuf<unsigned long long> d_phase;

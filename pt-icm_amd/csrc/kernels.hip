@@ -146,6 +146,67 @@ __global__ __launch_bounds__(W* kWave) void k_sweep(DevArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// k_sweep_rs: W waves per 32 replicas (item (instance, r0)) of an instance
+// swept replica-sliced (InstDesc::rs).  LDS: the per-class sums of the
+// waves, then X[32 (nw + 1)], the position words of the 32 replicas (the
+// last 32 the zero block the padding records read), then the replicas'
+// threshold rows [32][fmax].
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(W* kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_sweep_rs(DevArgs a) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    const int2 it = a.items[blockIdx.x];
+    if (it.x < 0) return;  // idle item padding an XCD's list
+    const uint32_t step = a.inst_step[it.x];
+    const InstDesc d = a.desc[it.x];
+    const int r0 = it.y, nb = min(32, d.nrep - r0), nw = d.nw;
+    SlicedShared& sh = *reinterpret_cast<SlicedShared*>(smem);
+    uint32_t* X = reinterpret_cast<uint32_t*>(smem + kSlicedShared);
+    uint32_t* T = X + 32 * (nw + 1);
+    for (int j = wave; j < nb; j += W) {
+        const int t = a.rep2slot[d.rep_off + r0 + j] % d.nt;
+        for (int m = lane; m < d.fmax; m += kWave) T[j * d.fmax + m] = a.thr[d.thr_off + (int64_t)t * d.fmax + m];
+    }
+    // replica-major state words -> position words: half-wave hw transposes
+    // block w = hw, hw + 2 W, ... (lane j brings replica r0 + j's word w);
+    // block nw is the sentinel's, zero
+    uint32_t* words = a.spins + d.spin_off + (int64_t)r0 * nw;
+    const int l32 = tid & 31;
+    for (int w = tid >> 5; w <= nw; w += 2 * W) {
+        const uint32_t v = (w < nw && l32 < nb) ? words[(int64_t)l32 * nw + w] : 0u;
+        X[32 * w + l32] = transpose_lanes32(v, l32);
+    }
+    const int64_t gr = d.rep_off + r0 + lane;
+    SweepState st{0, 0, 0, 0ull};
+    if (wave == 0 && lane < nb) {
+        st.E = a.E[gr];
+        st.bestE = a.bestE[gr];
+        st.best_sweep = a.best_sweep[gr];
+    }
+    __syncthreads();
+    const uint32_t c3 = ctr3(kTagSweep | a.tag_base, d.inst_id);
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint4* adj = reinterpret_cast<const uint4*>(a.adj + d.adj_off16 * 16);
+    uint32_t* best_words = a.best_spins + d.spin_off + (int64_t)r0 * nw;
+    for (int sw = 0; sw < a.sweeps; ++sw) {
+        const uint32_t gsweep = step * (uint32_t)a.sweeps + (uint32_t)sw;
+        sweep_sliced<W>(d, a.rowinfo + d.pos_off, adj, a.cls + d.cls_off, c3, k0, k1, r0, nb, gsweep, X, T, sh,
+                        best_words, st, tid);
+    }
+    for (int w = tid >> 5; w < nw; w += 2 * W) {
+        const uint32_t v = transpose_lanes32(X[32 * w + l32], l32);
+        if (l32 < nb) words[(int64_t)l32 * nw + w] = v;
+    }
+    if (wave == 0 && lane < nb) {
+        a.E[gr] = st.E;
+        a.bestE[gr] = st.bestE;
+        a.best_sweep[gr] = st.best_sweep;
+        a.flips[gr] += st.flips;
+    }
+}
+
 // Histogram of the slot energies after the sweeps (the point where
 // gather_samples logs them, cae_estimator.cc:117-121): energy = e_base + 2*bin.
 __device__ __forceinline__ void record_histogram(const DevArgs& a, const InstDesc& d, uint32_t step, const int32_t* E,
@@ -572,5 +633,9 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t* out, int iters, uint
     template __global__ void k_fused<F, 16, 1>(DevArgs, int);
 PTX_SPLIT_INST(kFmtWide)
 PTX_SPLIT_INST(kFmtUnit)
+template __global__ void k_sweep_rs<1>(DevArgs);
+template __global__ void k_sweep_rs<2>(DevArgs);
+template __global__ void k_sweep_rs<4>(DevArgs);
+template __global__ void k_sweep_rs<8>(DevArgs);
 
 }  // namespace ptx

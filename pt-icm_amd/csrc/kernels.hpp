@@ -620,6 +620,226 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
 }
 
 // ---------------------------------------------------------------------------
+// Replica-sliced sweep (InstDesc::rs: unit records, no field, degree <= 63).
+// One wave sweeps 32 replicas of an instance together.  LDS word X[p] holds
+// position p's spins in the 32 replicas (bit j: replica r0 + j), so one
+// gather of a neighbour's word serves all 32, and a position's agreeing-
+// neighbour counts are kept bit-sliced: six planes P[i] whose bit j is bit i
+// of replica j's count.  The decisions are sweep_replica's: lane l takes the
+// groups of 4 positions (cs >> 2) + l, + 64, ... of a class, and replica r's
+// update of position 4g + k draws word k of the Philox block (g, r, sweep)
+// and tests it against the threshold row of r's temperature, so spins,
+// energies, flip counts and best states equal the per-replica sweep's.
+// ---------------------------------------------------------------------------
+// 32 x 32 bit transpose across the 32 lanes of a half-wave (all active):
+// lane i's result has bit j = bit i of lane j's v.  Five rounds of the
+// recursive block transpose (Hacker's Delight, section 7-3, low bit first):
+// in round J, lanes k and k ^ J exchange the high J bits of each 2J-bit
+// field of the lower lane with the low J bits of the upper lane.
+__device__ __forceinline__ uint32_t transpose_lanes32(uint32_t v, int l32) {
+#pragma unroll
+    for (int J = 16; J >= 1; J >>= 1) {
+        const uint32_t m = J == 16 ? 0x0000FFFFu
+                           : J == 8 ? 0x00FF00FFu
+                           : J == 4 ? 0x0F0F0F0Fu
+                           : J == 2 ? 0x33333333u
+                                    : 0x55555555u;
+        const uint32_t u = (uint32_t)__shfl_xor((int)v, J, 32);
+        const bool hi = (l32 & J) != 0;
+        const uint32_t t = hi ? (((u >> J) ^ v) & m) : (((v >> J) ^ u) & m);
+        v ^= hi ? t : (t << J);
+    }
+    return v;
+}
+// a + b + c = s + 2 co, bitwise (co: the majority, a where a == b, else c)
+__device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uint32_t& s, uint32_t& co) {
+    const uint32_t u = a ^ b;
+    s = u ^ c;
+    co = (u & c) | (~u & a);
+}
+// P += A[0] + ... + A[7], bit-sliced (per bit position j: six-bit counters).
+// A carry-save tree reduces the eight words to a four-bit sum, which a
+// ripple carry adds to the planes.  Counts stay below 64 (degree <= 63).
+__device__ __forceinline__ void add8_planes(uint32_t (&P)[6], const uint32_t (&A)[8]) {
+    uint32_t s1, c1, s2, c2, s0, d1, t1, e1;
+    full_add(A[0], A[1], A[2], s1, c1);
+    full_add(A[3], A[4], A[5], s2, c2);
+    const uint32_t s3 = A[6] ^ A[7], c3 = A[6] & A[7];
+    full_add(s1, s2, s3, s0, d1);  // weight 1: s0; weight 2: c1, c2, c3, d1
+    full_add(c1, c2, c3, t1, e1);  // weight 2: t1, d1; weight 4: e1
+    const uint32_t b1 = t1 ^ d1, e2 = t1 & d1;
+    const uint32_t b2 = e1 ^ e2, b3 = e1 & e2;  // the sum: b3 b2 b1 s0
+    uint32_t cy = P[0] & s0, n;
+    P[0] ^= s0;
+    full_add(P[1], b1, cy, n, cy);
+    P[1] = n;
+    full_add(P[2], b2, cy, n, cy);
+    P[2] = n;
+    full_add(P[3], b3, cy, n, cy);
+    P[3] = n;
+    n = P[4] ^ cy;
+    cy &= P[4];
+    P[4] = n;
+    P[5] ^= cy;
+}
+
+// Shared state of the W waves sweeping one block of replicas: per class the
+// waves' partial sums per replica and the mask of replicas with a new best.
+struct SlicedShared {
+    int32_t dsum[8][32];
+    int32_t nacc[8][32];
+    uint32_t best_mask;
+};
+
+constexpr int kSlicedShared = (int)((sizeof(SlicedShared) + 15) / 16 * 16);
+// LDS bytes of k_sweep_rs for an instance of nw state words and threshold
+// rows of fmax entries
+__host__ __device__ inline int64_t sliced_lds_bytes(int nw, int fmax) {
+    return kSlicedShared + 4ll * 32 * (nw + 1) + 4ll * 32 * (fmax > 1 ? fmax : 1);
+}
+
+// Replica j's count from six bit planes (P[i] bit j = bit i of the count).
+__device__ __forceinline__ int plane_count(const uint32_t (&P)[6], int j) {
+    int c = (int)__builtin_amdgcn_ubfe(P[0], (uint32_t)j, 1u);
+#pragma unroll
+    for (int i = 1; i < 6; ++i) c |= (int)__builtin_amdgcn_ubfe(P[i], (uint32_t)j, 1u) << i;
+    return c;
+}
+
+// Sweeps of replicas r0 .. r0 + nb - 1 (nb <= 32) by the W waves of a
+// workgroup (thread tid of 64 W), the replicas' position words in X (32 (nw
+// + 1) words: the state's blocks, then the zero block of the sentinel
+// position), threshold rows in T (row j: replica r0 + j).  The groups of a
+// class are dealt to the 64 W lanes; wave 0's lane j < nb keeps replica
+// r0 + j's energy, best energy, best sweep and flip count in st.
+template <int W>
+__device__ __forceinline__ void sweep_sliced(const InstDesc& d, const uint32_t* __restrict__ info,
+                                             const uint4* __restrict__ adj, const int32_t* __restrict__ cls,
+                                             uint32_t c3, uint32_t k0, uint32_t k1, int r0, int nb,
+                                             uint32_t gsweep, uint32_t* X, const uint32_t* T, SlicedShared& sh,
+                                             uint32_t* __restrict__ best_words, SweepState& st, int tid) {
+    const int kb = d.kbit, nw = d.nw, wave = tid / kWave, lane = tid % kWave;
+    const uint32_t pmask = (1u << kb) - 1u;
+    const uint32_t sent2 = sentinel_pos(nw) * 0x10001u;  // two sentinel records (position 32 nw, sign 0)
+    for (int c = 0; c < d.ncls; ++c) {
+        const int cs = cls[c], ce = cls[c + 1];
+        int dacc = 0, nacc = 0;  // lane j: replica r0 + j's sums over this wave's groups of the class
+        for (int g0 = cs >> 2; 4 * g0 < ce; g0 += W * kWave) {  // uniform bound: every lane joins the reductions
+            const int g = g0 + tid, p0 = 4 * g;
+            uint32_t P[4][6], own[4];
+            int deg[4], nch[4], inm[4], mx = 0;  // inm: -1 for a position of the class, else 0
+            const uint4* row[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int p = p0 + k;
+                const bool in = p >= cs && p < ce;
+                inm[k] = -(int)in;
+                const uint32_t ri = info[in ? p : cs];  // a valid entry either way
+                deg[k] = (int)(ri & (uint32_t)kMaxDegree) & inm[k];
+                row[k] = adj + (ri >> 9);
+                nch[k] = (deg[k] + 7) >> 3;
+                mx = max(mx, nch[k]);
+                own[k] = X[in ? p : cs];  // (lanes past the class end read a valid word)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) P[k][i] = 0u;
+            }
+            // two chunks of every row per round, all loads before any gather;
+            // chunks past a row's end are replaced by sentinel records
+            for (int c0 = 0; c0 < mx; c0 += 2) {
+                uint4 q[4][2];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) q[k][h] = row[k][min(c0 + h, max(nch[k] - 1, 0))];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const bool live = c0 + h < nch[k];
+                        const uint32_t wv[4] = {live ? q[k][h].x : sent2, live ? q[k][h].y : sent2,
+                                                live ? q[k][h].z : sent2, live ? q[k][h].w : sent2};
+                        uint32_t A[8];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            // agreeing neighbours: spin bits, inverted for a negative coupling;
+                            // sentinel records read the zero block with sign 0
+                            const uint32_t r = (wv[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                            A[e] = X[r & pmask] ^ (0u - __builtin_amdgcn_ubfe(r, (uint32_t)kb, 1u));
+                        }
+                        add8_planes(P[k], A);
+                    }
+                }
+            }
+            uint32_t F[4] = {0u, 0u, 0u, 0u};
+            for (int j = 0; j < nb; ++j) {
+                int dE[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int f = 2 * plane_count(P[k], j) - deg[k];
+                    const int m = -(int)__builtin_amdgcn_ubfe(own[k], (uint32_t)j, 1u);  // -1: spin set, dE = -2 f
+                    dE[k] = (2 * ((f ^ m) - m)) & inm[k];
+                }
+                const Philox4 rnd = philox4x32_10((uint32_t)g, (uint32_t)(r0 + j), gsweep, c3, k0, k1);
+                const uint32_t* tr = T + j * d.fmax;
+                int v = 0;  // accepted dE + 65536 per acceptance (|sum dE| < 32768 over the wave)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t t = tr[max(dE[k] >> 1, 1) - 1];  // entry 0 where dE <= 0 (not used)
+                    // no short-circuit: selects, not branches
+                    const uint32_t acc =
+                        (uint32_t)inm[k] & ((uint32_t)(dE[k] <= 0) | (uint32_t)(rnd.v[k] < t));
+                    F[k] |= acc << j;
+                    v += (dE[k] + 65536) & -(int)acc;
+                }
+                v = wave_sum_i32(v);
+                const int na = (v + 32768) >> 16;
+                dacc += lane == j ? v - na * 65536 : 0;
+                nacc += lane == j ? na : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (F[k]) X[p0 + k] = own[k] ^ F[k];
+        }
+        if (lane < 32) {
+            sh.dsum[wave][lane] = dacc;
+            sh.nacc[wave][lane] = nacc;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            int ds = 0, na = 0;
+            if (lane < nb) {
+#pragma unroll
+                for (int x = 0; x < W; ++x) {
+                    ds += sh.dsum[x][lane];
+                    na += sh.nacc[x][lane];
+                }
+            }
+            st.E += ds;
+            st.flips += (unsigned long long)na;
+            const bool better = lane < nb && na > 0 && st.E < st.bestE;
+            if (better) {
+                st.bestE = st.E;
+                st.best_sweep = (int32_t)gsweep;
+            }
+            const unsigned long long bm = __ballot(better);
+            if (lane == 0) sh.best_mask = (uint32_t)bm;
+        }
+        __syncthreads();
+        const uint32_t bm = sh.best_mask;
+        if (bm) {
+            // half-wave hw packs state word w = hw, hw + 2 W, ...: after the
+            // transpose its lane j holds replica r0 + j's word w
+            const int l32 = tid & 31;
+            for (int w = tid >> 5; w < nw; w += 2 * W) {
+                const uint32_t v = transpose_lanes32(X[32 * w + l32], l32);
+                if ((bm >> l32) & 1u) best_words[(int64_t)l32 * nw + w] = v;
+            }
+            __syncthreads();  // the next class writes X
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Multispin sweep of a periodic lattice (InstDesc::lsw_dim = D > 0): one wave
 // per replica, the state as position-order bits in LDS, the same decisions
 // as sweep_replica (Replica::sweep, src/replica.cc:5-38, in colour order).

@@ -259,6 +259,10 @@ struct InstDesc {
     int64_t lsw_off;     // -> per class and window of 32 class positions, kLswGroup words: the edge
                          //    signs to the +x, -x, +y, -y, +z, -z neighbours (bit = coupling < 0), 0, 0,
                          //    and the shape masks par, k = 0, k = L/2 - 1, y = 0, y = L-1, z = 0, z = L-1, 0
+    // Replica-sliced sweep of the split kernels (k_sweep_rs, 32 replicas per
+    // wave): 1 for unit records, no field, degree <= 63 and no multispin
+    // lattice sweep, when the engine allows it; else 0
+    int32_t rs;
 };
 
 constexpr int kLswGroup = 16;  // words per (class, window) of the multispin sweep's table

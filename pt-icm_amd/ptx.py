@@ -42,7 +42,7 @@ class EngineConfig(C.Structure):
                 ("hist_start", C.c_int64), ("mode", C.c_int32), ("waves_per_group", C.c_int32),
                 ("steps_per_launch", C.c_int32), ("use_graph", C.c_int32), ("spins_per_lane", C.c_int32),
                 ("csr_placement", C.c_int32), ("icm_waves", C.c_int32), ("small_batch_coop", C.c_int32),
-                ("coop_wait_us", C.c_int32)]
+                ("coop_wait_us", C.c_int32), ("sliced_sweep", C.c_int32)]
 
 
 class EngineInfo(C.Structure):
@@ -55,7 +55,7 @@ class EngineInfo(C.Structure):
 class InstanceInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("nt", C.c_int32), ("nrep", C.c_int32), ("nc", C.c_int32),
                 ("n_colours", C.c_int32), ("hist_bins", C.c_int32), ("e_base", C.c_int64),
-                ("lattice_sweep", C.c_int32)]
+                ("lattice_sweep", C.c_int32), ("sliced_sweep", C.c_int32)]
 
 
 class SlotObservables(C.Structure):
@@ -316,7 +316,7 @@ class Engine:
 
     def __init__(self, problems, ladders, seed=0, chains=2, sweeps=1, flags=0, nc=-1, hist_start=0, mode=MODE_AUTO,
                  waves_per_group=0, steps_per_launch=0, use_graph=0, spins_per_lane=0, csr_placement=CSR_AUTO,
-                 icm_waves=0, small_batch_coop=0, coop_wait_us=0):
+                 icm_waves=0, small_batch_coop=0, coop_wait_us=0, sliced_sweep=0):
         if isinstance(problems, Problem):
             problems = [problems]
             ladders = [ladders]
@@ -327,7 +327,7 @@ class Engine:
         cfg.hist_start, cfg.mode, cfg.waves_per_group = hist_start, mode, waves_per_group
         cfg.steps_per_launch, cfg.use_graph, cfg.spins_per_lane = steps_per_launch, use_graph, spins_per_lane
         cfg.csr_placement, cfg.icm_waves, cfg.small_batch_coop = csr_placement, icm_waves, small_batch_coop
-        cfg.coop_wait_us = coop_wait_us
+        cfg.coop_wait_us, cfg.sliced_sweep = coop_wait_us, sliced_sweep
         self.cfg = cfg
         arr = (C.c_void_p * len(self.problems))(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h
                                                   for p in self.problems])

@@ -346,7 +346,8 @@ def test_multiwave_gather_sweep_parity(ptx, oracle, batch, wpg):
     nts = [5, 7, 3][: len(pairs)]
     ladders = [oracle.beta_linspace(0.5, 3.0, nt, 0) for nt in nts]
     say("multiwave gather", batch, "W", wpg)
-    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=21, flags=ptx.FLAG_ICM, mode=1, waves_per_group=wpg)
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=21, flags=ptx.FLAG_ICM, mode=1, waves_per_group=wpg,
+                     sliced_sweep=-1)
     assert eng.info().waves_per_group == wpg
     oes = [oracle.Engine(g, lad, chains=2, seed=21, instance=i, flags=oracle.FLAG_ICM)
            for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
@@ -361,6 +362,52 @@ def test_multiwave_gather_sweep_parity(ptx, oracle, batch, wpg):
         oe.step(12)
     for i, oe in enumerate(oes):
         compare(eng, oe, inst=i, what=f"{batch} W={wpg} inst {i} bulk")
+
+
+@pytest.mark.parametrize("sliced,sweeps", [(0, 1), (0, 2), (1, 1), (2, 2), (4, 2), (8, 1), (-1, 1)])
+def test_replica_sliced_sweep_parity(ptx, oracle, sliced, sweeps):
+    """Split mode with the replica-sliced sweep (32 replicas per workgroup of
+    1, 2, 4 or 8 waves, neighbour counts bit-sliced) and without it:
+    bit-exact with the oracle for the instances it takes (G022 with replica
+    blocks of 32 + 32 + 16, a +-J ER graph with 10 replicas, an odd torus
+    whose rows are one chunk, a +-J G(5000, 0.004) with 32 replicas) next to
+    ones it leaves to the other sweeps (a field, degree above 63, an even
+    torus on the multispin sweep), all in the 16-bit record format.  Chosen
+    automatically (sliced = 0) it takes only instances with 32 replicas or
+    more whose colour classes average 512 positions or more: here the
+    G(5000, 0.004) graph."""
+    pairs = [make_pair(ptx, oracle, k) for k in ["G022", "er_small"]]
+    n, u, v, w = instances.torus_2d(9, seed=4)
+    pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    n, u, v, w = instances.erdos_renyi(200, 0.05, seed=6, pm=True)
+    h = np.where(np.arange(n) % 5 == 0, 1.0, 0.0)
+    pairs.append((ptx.Problem.from_coo(n, u, v, w, h), oracle.Graph.from_coo(n, u, v, w, h)))
+    n, u, v, w = instances.erdos_renyi(300, 0.3, seed=5, pm=True)
+    pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    n, u, v, w = instances.torus_2d(8, seed=2)
+    pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    n, u, v, w = instances.erdos_renyi(5000, 0.004, seed=7, pm=True)
+    pairs.append((ptx.Problem.from_coo(n, u, v, w), oracle.Graph.from_coo(n, u, v, w)))
+    want = {0: [0, 0, 0, 0, 0, 0, 1], -1: [0] * 7}.get(sliced, [1, 1, 1, 0, 0, 0, 1])
+    nts = [40, 5, 7, 6, 6, 6, 16]
+    ladders = [oracle.beta_linspace(0.4, 3.0, nt, 0) for nt in nts]
+    say("replica-sliced", sliced, "sweeps", sweeps)
+    eng = ptx.Engine([pp for pp, _ in pairs], ladders, seed=23, sweeps=sweeps, flags=ptx.FLAG_ICM, mode=1,
+                     sliced_sweep=sliced)
+    assert [eng.instance_info(i).sliced_sweep for i in range(len(pairs))] == want
+    oes = [oracle.Engine(g, lad, chains=2, seed=23, sweeps=sweeps, instance=i, flags=oracle.FLAG_ICM)
+           for i, ((_, g), lad) in enumerate(zip(pairs, ladders))]
+    for s_ in range(3):
+        eng.step(1)
+        for oe in oes:
+            oe.step(1)
+        for i, oe in enumerate(oes):
+            compare(eng, oe, inst=i, what=f"sliced={sliced} inst {i} step {s_}")
+    eng.step(20)
+    for oe in oes:
+        oe.step(20)
+    for i, oe in enumerate(oes):
+        compare(eng, oe, inst=i, what=f"sliced={sliced} inst {i} bulk")
 
 
 @pytest.mark.parametrize("kind,steps,per_launch", [("G022", 300, 500), ("lattice_2d_20.gset", 300, 500),
