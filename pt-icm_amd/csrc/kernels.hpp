@@ -761,10 +761,14 @@ __device__ __forceinline__ void sweep_sliced(const InstDesc& d, const uint32_t* 
                         uint32_t A[8];
 #pragma unroll
                         for (int e = 0; e < 8; ++e) {
-                            // agreeing neighbours: spin bits, inverted for a negative coupling;
-                            // sentinel records read the zero block with sign 0
-                            const uint32_t r = (wv[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-                            A[e] = X[r & pmask] ^ (0u - __builtin_amdgcn_ubfe(r, (uint32_t)kb, 1u));
+                            // agreeing neighbours: spin bits, inverted for a negative coupling
+                            // (sign 0 or -1 by a signed one-bit extract); sentinel records read
+                            // the zero block with sign 0
+                            const uint32_t w32 = wv[e >> 1];
+                            const uint32_t pos = (e & 1) ? __builtin_amdgcn_ubfe(w32, 16u, (uint32_t)kb) : (w32 & pmask);
+                            const uint32_t sgn =
+                                (uint32_t)__builtin_amdgcn_sbfe((int)w32, (e & 1) ? 16u + (uint32_t)kb : (uint32_t)kb, 1u);
+                            A[e] = X[pos] ^ sgn;
                         }
                         add8_planes(P[k], A);
                     }
