@@ -6,6 +6,8 @@
     python pt-icm_amd/microbench.py batched --steps 32        # 4096 3D EA instances, autotuned launch shape
     python pt-icm_amd/microbench.py batched --mode 1 --waves 1 --steps 32   # split kernels, one wave per replica
     python pt-icm_amd/microbench.py coop --launches 4   # G022 single instance, cooperative kernel, 500 steps/launch
+    python pt-icm_amd/microbench.py er --instances 256 --steps 8   # ER +-1 G(5000, 0.004), split kernels
+    python pt-icm_amd/microbench.py er --instances 256 --steps 8 --sliced -1   # ... per-replica sweep
 
 The copy kernel (k_copy16) also calibrates the FETCH_SIZE / WRITE_SIZE
 counters: a run of it under rocprofv3 --pmc reads back known byte counts.
@@ -29,7 +31,9 @@ def load_ptx():
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["lds", "copy", "batched", "coop"])
+    ap.add_argument("what", choices=["lds", "copy", "batched", "coop", "er"])
+    ap.add_argument("--sliced", type=int, default=0,
+                    help="er: the engine's sliced_sweep (0 automatic, -1 per-replica sweep, 1/2/4/8 waves)")
     ap.add_argument("--launches", type=int, default=4, help="coop: timed launches of 500 PT steps")
     ap.add_argument("--icm", action="store_true", help="coop: ascending ladder, so the cold slots run cluster moves")
     ap.add_argument("--short", type=int, default=0,
@@ -76,6 +80,34 @@ def main():
                           "spin_updates_per_s": info.spin_updates_per_step * a.steps / dt,
                           "lds_bytes_per_step": info.lds_bytes_per_step,
                           "steps_per_launch": info.steps_per_launch}))
+    elif a.what == "er":
+        # ER +-1 G(5000, 0.004) instances (8 colour classes of ~625
+        # positions), 64 inverse temperatures linear over T 0.5..3.0, 2
+        # chains, cluster moves, split kernels: the sweep's time per step
+        import time
+        import numpy as np
+        sys.path.insert(0, HERE)
+        import instances
+        n_inst = a.instances if a.instances != 4096 else 256
+        probs = [ptx.Problem.from_coo(*instances.erdos_renyi(5000, 0.004, seed=i, pm=True)) for i in range(n_inst)]
+        lad = np.sort(ptx.beta_linspace(0.5, 3.0, 64, False))
+        eng = ptx.Engine(probs, [lad] * n_inst, seed=0, flags=ptx.FLAG_ICM, mode=ptx.MODE_SPLIT, waves_per_group=1,
+                         icm_waves=1, sliced_sweep=a.sliced)
+        eng.step(a.warmup)
+        eng.sync()
+        eng.timing(True)
+        t0 = time.perf_counter()
+        eng.step(a.steps)
+        eng.sync()
+        dt = time.perf_counter() - t0
+        t = eng.timing()
+        info = eng.info()
+        sliced = sum(eng.instance_info(i).sliced_sweep for i in range(n_inst))
+        eng.close()
+        print(json.dumps({"instances": n_inst, "sliced_instances": sliced, "steps": a.steps,
+                          "ms_per_step": dt * 1e3 / a.steps, "sweep_ms_per_step": t.sweep_ms / max(a.steps, 1),
+                          "icm_ms_per_step": t.icm_ms / max(a.steps, 1),
+                          "spin_updates_per_s": info.spin_updates_per_step * a.steps / dt}))
     elif a.what == "coop":
         # G022 with a fixed 22-slot ladder over the range the 64-slot
         # preprocess ladder spans (beta 1.667 .. 0.367): descending, as the
