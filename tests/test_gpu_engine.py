@@ -214,3 +214,48 @@ def test_copies_of_one_problem_share_tables_without_changing_results(ptx, mode):
     assert not np.array_equal(a.spins(0), a.spins(3))
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("waves", [0, 4])
+def test_replica_sliced_sweep_with_pilot_histograms_observables_and_checkpoints(ptx, waves):
+    """The replica-sliced sweep equals the per-replica sweep bit for bit in a
+    pilot run with histograms (one chain, every pair exchanged), with
+    observables, and across a checkpoint taken by one engine and resumed by
+    the other (+-J ER graphs of 5,000 and 600 vertices; waves = 0 takes the
+    engine's own choice, the larger graph only, 4 forces the sliced sweep on
+    both)."""
+    coos = [instances.erdos_renyi(5000, 0.004, seed=2, pm=True), instances.erdos_renyi(600, 0.03, seed=4, pm=True)]
+    probs = [ptx.Problem.from_coo(*c) for c in coos]
+    pil = [ptx.pilot_betas(0.5, 3.0, 40, False)[0]] * 2
+    kw = dict(seed=13, chains=1, flags=ptx.FLAG_PILOT | ptx.FLAG_HISTOGRAM, mode=ptx.MODE_SPLIT)
+    a = ptx.Engine(probs, pil, sliced_sweep=waves, **kw)
+    b = ptx.Engine(probs, pil, sliced_sweep=-1, **kw)
+    assert a.instance_info(0).sliced_sweep == 1 and b.instance_info(0).sliced_sweep == 0
+    a.step(40)
+    b.step(40)
+    for i in range(2):
+        assert np.array_equal(a.histograms(i)[0], b.histograms(i)[0])
+    _same(_state(a, 2), _state(b, 2))
+    a.close()
+    b.close()
+    lads = [np.sort(ptx.beta_linspace(0.5, 3.0, 24, False))] * 2
+    kw = dict(seed=17, flags=ptx.FLAG_ICM | ptx.FLAG_OBSERVE, mode=ptx.MODE_SPLIT)
+    a = ptx.Engine(probs, lads, sliced_sweep=waves, **kw)
+    b = ptx.Engine(probs, lads, sliced_sweep=-1, **kw)
+    a.step(25)
+    b.step(25)
+    for i in range(2):
+        oa, ob = a.observables(i), b.observables(i)
+        assert all(np.array_equal(oa[f], ob[f]) for f in oa)
+    # a checkpoint of the sliced engine resumed by the per-replica one, and back
+    blob = a.save_state()
+    b.load_state(blob)
+    a.step(15)
+    b.step(15)
+    _same(_state(a, 2), _state(b, 2))
+    b.load_state(a.save_state())
+    a.step(5)
+    b.step(5)
+    _same(_state(a, 2), _state(b, 2))
+    a.close()
+    b.close()
