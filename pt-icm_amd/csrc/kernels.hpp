@@ -621,15 +621,16 @@ __device__ __forceinline__ void sweep_replica(const Csr<Fmt>& csr, const int32_t
 
 // ---------------------------------------------------------------------------
 // Replica-sliced sweep (InstDesc::rs: unit records, no field, degree <= 63).
-// One wave sweeps 32 replicas of an instance together.  LDS word X[p] holds
-// position p's spins in the 32 replicas (bit j: replica r0 + j), so one
-// gather of a neighbour's word serves all 32, and a position's agreeing-
-// neighbour counts are kept bit-sliced: six planes P[i] whose bit j is bit i
-// of replica j's count.  The decisions are sweep_replica's: lane l takes the
-// groups of 4 positions (cs >> 2) + l, + 64, ... of a class, and replica r's
-// update of position 4g + k draws word k of the Philox block (g, r, sweep)
-// and tests it against the threshold row of r's temperature, so spins,
-// energies, flip counts and best states equal the per-replica sweep's.
+// The W waves of a workgroup sweep 32 replicas of an instance together.  LDS
+// word X[p] holds position p's spins in the 32 replicas (bit j: replica
+// r0 + j), so one gather of a neighbour's word serves all 32, and a
+// position's agreeing-neighbour counts are kept bit-sliced: six planes P[i]
+// whose bit j is bit i of replica j's count.  The decisions are
+// sweep_replica's: thread t of the 64 W takes the groups of 4 positions
+// (cs >> 2) + t, + 64 W, ... of a class, and replica r's update of position
+// 4g + k draws word k of the Philox block (g, r, sweep) and tests it against
+// the threshold row of r's temperature, so spins, energies, flip counts and
+// best states equal the per-replica sweep's.
 // ---------------------------------------------------------------------------
 // 32 x 32 bit transpose across the 32 lanes of a half-wave (all active):
 // lane i's result has bit j = bit i of lane j's v.  Five rounds of the
